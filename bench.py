@@ -1,0 +1,187 @@
+#!/usr/bin/env python3
+"""Benchmark: simulated cell-ms per wall-second of the CRU update on one MI355X.
+
+A "step" is one time step (dt = 0.025 ms) of the hot path over the whole
+batch: the CRU kernel over every CRU, the in-kernel whole-cell reduction and
+membrane ODE, and (every 100 steps) the output-row reduction.  Inputs live in
+HBM before the timed region starts; nothing is copied to the host inside it.
+
+Workloads (synthetic, model ICs; no datasets involved):
+  cell  : one 100x20x10-CRU cell (20,000 CRUs), RyR multiplier Ku2 = 2.5,
+          I_Kr scaled to 0, CL 2000 ms, FP32 production precision, seed 1
+  batch : 256 independent cells, lattices drawn from {50x10x10, 75x15x10,
+          100x20x10}, Ku2 ~ U[1,3], I_Kr scale ~ U[0,1], CL in {1000,2000,3000},
+          seed = cell index, same precision
+The JSON line reports the `cell` workload as `value` and the batch beside it.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = "simulated cell-ms/wall-s (1 cell, 256-cell batch); EAD-map GPU-time vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def alg_bytes_per_cru_step(precision):
+    """Algorithmic HBM bytes of one interior CRU-step: every live state word
+    read once and written once (neighbour re-reads, ghost cells and the
+    output-step extras excluded).  FP32 mode: 55 float words (8 ci, 5x8 fast
+    buffers, cs, cp, Ancx, 4 sarcolemmal buffers) + 18 double words (8 cnsr,
+    8 slow troponin, cjsr, CSQN-bound) + 3 packed 32-bit gating/flag words.
+    FP64 mode: 73 double words + the 3 packed words."""
+    if precision == 1:
+        return 2 * (55 * 4 + 18 * 8 + 3 * 4)
+    return 2 * (73 * 8 + 3 * 4)
+
+
+def make_cell(prec):
+    from ryr_hyperactivity_amd import CellDesc, Params
+    p = Params.default(precision=prec)
+    d = CellDesc.default(nx=100, ny=20, nz=10, cl=2000.0, Ku2=2.5, ikr_scale=0.0, seed=1)
+    return p, [d]
+
+
+def make_batch(prec, ncells=256, seed=2024):
+    from ryr_hyperactivity_amd import CellDesc, Params
+    rng = np.random.default_rng(seed)
+    dims = [(50, 10, 10), (75, 15, 10), (100, 20, 10)]
+    cells = []
+    for c in range(ncells):
+        nx, ny, nz = dims[rng.integers(0, 3)]
+        cells.append(CellDesc.default(nx=nx, ny=ny, nz=nz, Ku2=float(rng.uniform(1, 3)),
+                                      ikr_scale=float(rng.uniform(0, 1)),
+                                      cl=float([1000.0, 2000.0, 3000.0][rng.integers(0, 3)]), seed=c))
+    return Params.default(precision=prec), cells
+
+
+def run_gpu(p, cells, steps, warmup):
+    from ryr_hyperactivity_amd import CellEngine
+    ncru = sum(c.ncru for c in cells)
+    with CellEngine(p, cells, device=0) as eng:
+        eng.step(warmup)
+        eng.sync()
+        eng.kernel_time()                      # reset the event window
+        t0 = time.perf_counter()
+        eng.step(steps)
+        eng.sync()
+        wall = time.perf_counter() - t0
+        ev_ms, launches = eng.kernel_time()
+        v = eng.scalars()[:, 1]
+        assert np.all(np.isfinite(v)), "non-finite membrane potential"
+    sim_ms = len(cells) * steps * p.dt
+    per_launch_s = ev_ms * 1e-3 / max(launches, 1)
+    bpl = ncru * alg_bytes_per_cru_step(p.precision)
+    return {
+        "cell_ms_per_s": sim_ms / wall,
+        "wall_s": wall,
+        "ms_per_step": 1e3 * wall / steps,
+        "event_ms": ev_ms,
+        "launches": launches,
+        "kernel_us": per_launch_s * 1e6,
+        "ncru": ncru,
+        "bytes_per_launch": bpl,
+        "achieved_GBs": bpl / per_launch_s / 1e9,
+    }
+
+
+def cpu_baseline(p, cell, nsteps):
+    """The FP64 oracle (OpenMP over CRUs) on this host, on a bounded sample."""
+    from oracle.binding import OracleCell, lib
+    o = OracleCell(p, cell)
+    o.step(10)
+    t0 = time.perf_counter()
+    o.step(nsteps)
+    wall = time.perf_counter() - t0
+    return {"value": nsteps * p.dt / wall, "unit": "simulated cell-ms/wall-s",
+            "cores": lib().orc_num_threads(), "kind": "port",
+            "sample": f"{nsteps} steps ({nsteps * p.dt:g} ms) of one {cell.nx}x{cell.ny}x{cell.nz} cell, FP64 "
+                      f"oracle, {lib().orc_num_threads()} OpenMP threads, {platform.processor() or 'x86_64'}",
+            "wall_s": wall}
+
+
+def traffic_from_profiles(workload):
+    path = os.path.join(ROOT, "profiles", f"traffic_k_cru_step_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None, help="timed time steps (default 8000 cell / 1200 batch)")
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--precision", choices=["f32", "f64"], default="f32")
+    ap.add_argument("--workload", choices=["cell", "batch", "both"], default="both")
+    ap.add_argument("--batch-steps", type=int, default=1200)
+    ap.add_argument("--cpu-steps", type=int, default=1000)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (independent cells shard trivially across GPUs; "
+              "no multi-GPU path is built). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    prec = 1 if args.precision == "f32" else 0
+    steps = args.steps if args.steps is not None else 8000
+    warmup = args.warmup if args.warmup is not None else 400
+
+    p, cells = make_cell(prec)
+    if args.workload in ("cell", "both"):
+        r = run_gpu(p, cells, steps, warmup)
+        print(f"# cell: {r}", file=sys.stderr, flush=True)
+    batch = None
+    if args.workload in ("batch", "both"):
+        pb, bcells = make_batch(prec)
+        rb = run_gpu(pb, bcells, args.batch_steps, 100)
+        print(f"# batch: {rb}", file=sys.stderr, flush=True)
+        batch = {"value": rb["cell_ms_per_s"], "unit": "simulated cell-ms/wall-s", "ncells": len(bcells),
+                 "ncru": rb["ncru"], "steps": args.batch_steps, "ms_per_step": rb["ms_per_step"],
+                 "roofline": {"bound": "hbm", "achieved": rb["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": rb["achieved_GBs"] / HBM_PEAK_GBS,
+                              "traffic": traffic_from_profiles("batch"),
+                              "kernel_us": rb["kernel_us"], "bytes_per_launch": rb["bytes_per_launch"]}}
+        if args.workload == "batch":
+            r = rb
+            steps = args.batch_steps
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(make_cell(0)[0], cells[0], args.cpu_steps)
+        print(f"# cpu: {cpu}", file=sys.stderr, flush=True)
+    wl = "cell" if args.workload != "batch" else "batch"
+    out = {
+        "metric": METRIC,
+        "value": r["cell_ms_per_s"],
+        "unit": "simulated cell-ms/wall-s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if prec == 1 else "f64",
+        "data": "synthetic (model initial conditions, Philox seeds)",
+        "config": {"workload": "1 cell 100x20x10 CRUs, Ku2=2.5, I_Kr=0, CL 2000 ms" if wl == "cell"
+                   else "256-cell uneven batch", "dt_ms": p.dt, "precision": args.precision,
+                   "ncru": r["ncru"]},
+        "roofline": {"bound": "hbm", "achieved": r["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": r["achieved_GBs"] / HBM_PEAK_GBS, "traffic": traffic_from_profiles(wl),
+                     "kernel": "k_cru_step", "kernel_us": r["kernel_us"],
+                     "bytes_per_launch": r["bytes_per_launch"]},
+        "cpu_baseline": cpu,
+        "batch": batch,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,1224 @@
+/*
+ * cru_device.h - gfx950 device code of the CRU update: one CRU per lane, SoA
+ * state in HBM, the per-step kernel, the in-kernel whole-cell reduction and
+ * membrane ODE, and the output-row reduction kernel.
+ *
+ * Reference (colinmrees/RyR_hyperactivity): Compute (cuda_Cell_Detailed.cu:
+ * 1187-1692) with its leaf physics (subroutine_cuda.c, buffers.c, reesica.h),
+ * the host reduction and membrane ODE (cuda_Cell_Detailed.cu:427-739) and the
+ * output block (:777-1081).
+ *
+ * Layout (per handle; all cells concatenated, each cell's CRUs padded to a
+ * multiple of 256 so a workgroup never straddles two cells):
+ *   slot g = cell offset + n,  n = i + X*(j + Y*k)   (x fastest)
+ *   sub-voxel fields: 8 planes, element (ii, g) at ii*P + g
+ *   ci, cnsr, cs: two buffers, read buffer = step parity
+ * The reference's ghost shell is not stored: every ghost value it writes is a
+ * copy of the interior sub-voxel adjacent to it (:1601-1652), which is the
+ * very sub-voxel whose stencil reads it, so a neighbour outside the interior
+ * is replaced by the centre value.  The differences the reference forms
+ * (x_nb - x_c) are then exactly zero, as with the ghost copies.
+ *
+ * Precision: TF = type of fast concentrations and buffers, TS = type of the
+ * large slowly varying pools (cnsr, cjsr, cjsrb, slow troponin).  FP64 mode:
+ * TF = TS = double, with the reference's operand order everywhere, so results
+ * are bit-identical to the FP64 oracle.  FP32 mode: TF = float, TS = double.
+ * Gating, I_CaL/I_NCX per CRU, the reductions and the membrane ODE are FP64
+ * in both modes and compiled with contraction off.
+ */
+#ifndef RYR_CRU_DEVICE_H
+#define RYR_CRU_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "common/philox.h"
+#include "common/detmath.h"
+
+#define RYR_BLOCK 256
+#define RYR_NRYR 100
+#define RYR_DEV __device__ __forceinline__
+
+/* ---------------------------------------------------------------- structs */
+struct RyrCellInfo {
+    int X, Y, Z, N;
+    long long off;          /* first slot */
+    int first_block, nblocks;
+    unsigned long long seed;
+    double Ku, Ku2, Kb, svica, svncx, svikr, sviup_tot, Cm, dN;
+    long long ncl, noff, ndur;
+    int nlcc, line_j, line_k, pad_;
+};
+
+/* membrane state of a cell plus the V-only factors hoisted for its next step */
+struct RyrMem {
+    double m[13];           /* v xm xh xj xr xs1 xs2 qks xtos ytos xtof ytof xnai */
+    double cur[10];         /* csubt xicatto xinacato xina xiks xikr xik1 xinak xitos xitof */
+    double stim, out_ica, out_ncx, out_ina;
+    long long step;         /* steps completed */
+    /* lcc_markov rates (reesica.h:37-59) */
+    double alphad, betad, alphaf, betaf;
+    /* lcccurrent GHK factors (subroutine_cuda.c:233-243) */
+    double ghk_A, ghk_e2, ghk_D;
+    int ghk_small, pad_;
+    /* ncx factors (subroutine_cuda.c:56-67) */
+    double ncx_n1, ncx_n2, ncx_t1, ncx_t3a, ncx_den2;
+};
+
+struct RyrDD { double hi, lo; };
+
+/* kernel arguments (identical for every step: the step index lives in RyrMem,
+ * so a captured graph replays without new arguments) */
+struct RyrKArgs {
+    long long P;            /* padded slots */
+    double dt;
+    int scheme, iso2, spo, ncells;
+    void *ci[2], *cnsr[2], *cs[2];
+    void *catf, *cats, *cacal, *casr, *camyo, *mgmyo;
+    void *cp, *cjsr, *cjsrb, *Ancx, *casar, *casarh, *casarj, *casarhj;
+    uint32_t *lccw, *ryrw, *aux;          /* aux: nl (bits 0-15) | nspark (bit 16) */
+    double *d_xire, *d_xinaca, *d_up, *d_leak;   /* per CRU, written on output steps */
+    const RyrCellInfo *cells;
+    RyrMem *mem;
+    const int *block_cell;
+    RyrDD *partial;         /* 3 per block */
+    unsigned int *arrive;   /* per cell */
+    unsigned long long *counters;  /* per cell x RYR_C_COUNT */
+    double *diag;           /* per cell x 45 */
+    long long *diag_step;   /* per cell */
+    double *line;           /* per cell x maxX x 22 (offset line_off[c]) */
+    const long long *line_off;
+};
+
+#define RYR_NCNT 5
+
+/* ------------------------------------------------------------- reductions */
+RYR_DEV RyrDD dd_add(RyrDD a, RyrDD b)
+{
+    RYR_STRICT_FP
+    double s = a.hi + b.hi;
+    double bb = s - a.hi;
+    double e = (a.hi - (s - bb)) + (b.hi - bb);
+    e = e + (a.lo + b.lo);
+    RyrDD r;
+    r.hi = s + e;
+    r.lo = e - (r.hi - s);
+    return r;
+}
+
+/* Pairwise tree over the 64 lanes of a wave in index order (xor 1,2,...,32);
+ * dd_add is commutative bit for bit, so every lane ends with the same value. */
+RYR_DEV RyrDD wave_tree(RyrDD v)
+{
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        RyrDD w;
+        w.hi = __shfl_xor(v.hi, o, 64);
+        w.lo = __shfl_xor(v.lo, o, 64);
+        v = dd_add(v, w);
+    }
+    return v;
+}
+
+/* ------------------------------------------------------------- buffers.c */
+template <typename T>
+RYR_DEV T buf_simple(T b, T c, T dt, T kon, T koff, T B)
+{
+    if (kon * c * dt > T(1)) return T(0.95) / dt * (B - b) - koff * b;
+    return kon * c * (B - b) - koff * b;
+}
+#define RYR_BSAR (42 * ((0.5 / 8.) * 8.0 / (0.025)))
+#define RYR_BSARH (15. * ((0.5 / 8.) * 8.0 / (0.025)))
+template <typename T> RYR_DEV T b_tropf(T b, T c, T dt) { return buf_simple<T>(b, c, dt, T(0.0327), T(0.0196), T(70.)); }
+template <typename T> RYR_DEV T b_trops(T b, T c, T dt) { return buf_simple<T>(b, c, dt, T(0.00254), T(0.000033), T(134.)); }
+template <typename T> RYR_DEV T b_bucal(T b, T c, T dt) { return buf_simple<T>(b, c, dt, T(0.0543), T(0.238), T(24.)); }
+template <typename T> RYR_DEV T b_busr(T b, T c, T dt) { return buf_simple<T>(b, c, dt, T(0.1), T(0.06), T(19.)); }
+template <typename T> RYR_DEV T b_busar(T b, T c, T dt) { return buf_simple<T>(b, c, dt, T(0.1), T(1.3), T(RYR_BSAR)); }
+template <typename T> RYR_DEV T b_busarh(T b, T c, T dt) { return buf_simple<T>(b, c, dt, T(0.1), T(0.03), T(RYR_BSARH)); }
+/* busarj/busarhj carry a factor ba = 1 that changes nothing (buffers.c:137-155) */
+template <typename T>
+RYR_DEV T b_myoca(T CaMyo, T MgMyo, T c, T dt)
+{
+    if (T(13.8) * c * dt > T(1)) return T(0.95) / dt * (T(140) - CaMyo - MgMyo) - T(0.46) * CaMyo;
+    return T(13.8) * c * (T(140) - CaMyo - MgMyo) - T(0.46) * CaMyo;
+}
+template <typename T>
+RYR_DEV T b_myomg(T CaMyo, T MgMyo, T dt)
+{
+    if (T(0.0157 * 500) * dt > T(1)) return T(0.95) / dt * (T(140) - CaMyo - MgMyo) - T(0.057) * MgMyo;
+    return T(0.0157 * 500) * (T(140) - CaMyo - MgMyo) - T(0.057) * MgMyo;
+}
+
+/* --------------------------------------------------- SERCA and leak per voxel */
+RYR_DEV double uptake_d(double ci, double cnsr)
+{
+    RYR_STRICT_FP
+    double a = ryr_pow(ci / 0.123, 1.787), b = ryr_pow(cnsr / 1700.0, 1.787);
+    return 1.0 * 0.3 * (a - b) / (1.0 + a + b);
+}
+RYR_DEV float powf_fast(float x, float y)
+{
+    /* x >= 0; x == 0 gives 0 for y > 0 */
+    return x > 0.0f ? __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x)) : 0.0f;
+}
+RYR_DEV float uptake_f(float ci, float cnsr)
+{
+    float a = powf_fast(ci * (1.0f / 0.123f), 1.787f), b = powf_fast(cnsr * (1.0f / 1700.0f), 1.787f);
+    return 0.3f * (a - b) / (1.0f + a + b);
+}
+template <typename T> RYR_DEV T leak_t(T cnsr, T ci)
+{
+    return T(0.00001035) * (cnsr - ci) * (cnsr * cnsr) / ((cnsr * cnsr) + T(500.0) * T(500.0));
+}
+
+/* -------------------------------------------- hoisted V-only factors (FP64) */
+RYR_DEV void vfactors(RyrMem *m, int iso2)
+{
+    RYR_STRICT_FP
+    const double v = m->m[0], xnai = m->m[12];
+    /* reesica.h:21-59 */
+    double dv5 = 5, dvk = 8, fv5 = -22.8, fvk = 9.1;
+    if (iso2) { dv5 = 0; fv5 = -28; fvk = 8.5; }
+    double dinf = 1.0 / (1.0 + ryr_exp(-(v - dv5) / dvk));
+    double taudin = 1.0 / ((1.0 - ryr_exp(-(v - dv5) / dvk)) / (0.035 * (v - dv5)) * dinf);
+    if ((v < 0.0001) && (v > -0.0001)) taudin = 0.035 * dvk / dinf;
+    double finf = 1. - 1.0 / (1.0 + ryr_exp(-(v - fv5) / fvk)) / (1. + ryr_exp((v - 60) / 12.));
+    double q = 0.0337 * (v + 10.5);
+    double taufin = (0.02 - 0.007 * ryr_exp(-(q * q)));
+    m->alphad = dinf * taudin;
+    m->betad = (1 - dinf) * taudin;
+    m->alphaf = (finf) * taufin;
+    m->betaf = (1 - finf) * taufin;
+    /* subroutine_cuda.c:233-243 */
+    double za = v * 96.485 / 8.314 / 308;
+    m->ghk_small = __builtin_fabs(za) < 0.001;
+    m->ghk_e2 = ryr_exp(2.0 * za);
+    m->ghk_A = 4.0 * 11.9 * za * 96.485 * 0.341;
+    m->ghk_D = m->ghk_e2 - 1.0;
+    /* subroutine_cuda.c:56-67 */
+    double e1 = ryr_exp(0.35 * za);
+    double e2n = ryr_exp((0.35 - 1.0) * za);
+    double r = xnai / 12.3;
+    m->ncx_t1 = 0.00359 * ((140.0) * (140.0) * (140.0)) * (1.0 + r * r * r);
+    double p3 = xnai * xnai * xnai;
+    m->ncx_t3a = (1.3 + (1 * 1.8)) * p3;
+    m->ncx_n1 = e1 * p3 * (1 * 1.8);
+    m->ncx_n2 = e2n * ((140.0) * (140.0) * (140.0));
+    m->ncx_den2 = 1.0 + 0.27 * e2n;
+}
+
+/* ---------------------------------------------------------- gating (FP64) */
+RYR_DEV int lcc_step(int i, double cpx, double dt, double ragg, const RyrMem &m, double betac)
+{
+    RYR_STRICT_FP
+    const double alphac = 0.22, alphafca = 0.012 / 2.;
+    double betafca = 0.175 / (1 + (25 / cpx) * (25 / cpx));
+    double rig = ragg / dt;
+    if ((i % 2)) { if (rig < alphac) return i - 1; else rig -= alphac; }
+    else         { if (rig < betac) return i + 1; else rig -= betac; }
+    if (((i / 2) % 2)) { if (rig < m.alphad) return i - 2; else rig -= m.alphad; }
+    else               { if (rig < m.betad) return i + 2; else rig -= m.betad; }
+    if (((i / 4) % 2)) { if (rig < m.alphaf) return i - 4; else rig -= m.alphaf; }
+    else               { if (rig < m.betaf) return i + 4; else rig -= m.betaf; }
+    if (((i / 8) % 2)) { if (rig < alphafca) return i - 8; else rig -= alphafca; }
+    else               { if (rig < betafca) return i + 8; else rig -= betafca; }
+    return i;
+}
+
+/* Knuth product-of-uniforms Poisson count, or the dithered Gaussian with
+ * rejection outside Knuth's range (subroutine_cuda.c:332-363 and its seven
+ * copies). */
+RYR_DEV int ryr_transition(ryr_stream *s, int n, double p, int *fails)
+{
+    RYR_STRICT_FP
+    double lam = ryr_exp(-(n) * p);
+    int out = -1;
+    if (n <= 1 || p < 0.2 || (n <= 5 && p < 0.3)) {
+        int kk = 0;
+        double puu = 1.0;
+        while (puu >= lam && kk < 195) {
+            kk++;
+            double re = ryr_stream_uniform(s);
+            puu = puu * (re);
+        }
+        out = kk - 1;
+    } else {
+        int kk = 0;
+        while (out < 0) {
+            double u1 = ryr_stream_uniform(s);
+            double u2 = ryr_stream_uniform(s);
+            uint32_t r = ryr_stream_u32(s);
+            out = (int)(__builtin_floor(n * p + __builtin_sqrt(n * p * (1.0 - p)) *
+                                        __builtin_sqrt(-2.0 * ryr_log(1.0 - u1)) *
+                                        ryr_cos(2.0 * 3.1415926 * u2)) + r % 2);
+            kk++;
+            if (kk > 200) { out = 0; (*fails)++; }
+        }
+    }
+    return out;
+}
+
+/* subroutine_cuda.c:251-669; counts in/out: nou, ncu, nob, ncb */
+RYR_DEV void ryr_gating(ryr_stream *s, double Ku, double Ku2, double Kb, double cp, double cjsr,
+                        double dt, int &nou, int &ncu, int &nob, int &ncb, int *fails)
+{
+    RYR_STRICT_FP
+    double q1 = (5000 / cjsr) * (5000 / cjsr);
+    double x2 = Ku / cp;
+    double q2 = x2 * x2;
+    double pku = 4.5 * 1.5 * Ku2 * 1 / (1 + q1) * 1 / (1 + q2);
+    double pkb = 4.5 * 1.5 / 36. * Kb * 1 / (1 + q1) * 1 / (1 + q2);
+    double pkuminus = 1. / 1., pkbminus = 1. / 1.;
+    if (pku * dt > 1.0) pku = 1.0 / dt;
+    if (pkb * dt > 1.0) pkb = 1.0 / dt;
+    if (pkuminus * dt > 1.0) pkuminus = 1.0 / dt;
+    if (pkbminus * dt > 1.0) pkbminus = 1.0 / dt;
+    double roo2 = 5000.0 / (1.0 + ryr_pow(850.0 / (cjsr), 23.0));
+    double aub = (-1.0 + __builtin_sqrt(1.0 + 8.0 * 460.0 * roo2)) / (4.0 * roo2 * 460.0) / (1.0);
+    double abu = 1.0 / (0.2 * 5 * 220.);
+
+    int n_ou_cu = ryr_transition(s, nou, pkuminus * dt, fails);
+    if (n_ou_cu > RYR_NRYR) n_ou_cu = RYR_NRYR;
+    int n_cu_ou = ryr_transition(s, ncu, pku * dt, fails);
+    if (n_cu_ou > RYR_NRYR) n_cu_ou = RYR_NRYR;
+    double pau = (pkb < 1e-16) ? 0 : aub * dt;
+    int n_ou_ob = ryr_transition(s, nou, pau, fails);
+    if (n_ou_ob > RYR_NRYR) n_ou_ob = RYR_NRYR;
+    double pbu = (pkb < 1e-16) ? abu * dt * 36 : abu * dt * (pku / pkb);
+    int n_ob_ou = ryr_transition(s, nob, pbu, fails);
+    if (n_ob_ou > RYR_NRYR) n_ob_ou = RYR_NRYR;
+    int n_cb_cu = ryr_transition(s, ncb, abu * dt, fails);
+    if (n_cb_cu > RYR_NRYR) n_cb_cu = RYR_NRYR;
+    int n_cu_cb = ryr_transition(s, ncu, aub * dt, fails);
+    if (n_cu_cb > RYR_NRYR) n_cu_cb = RYR_NRYR;
+    int n_ob_cb = ryr_transition(s, nob, pkbminus * dt, fails);
+    int n_cb_ob = ryr_transition(s, ncb, pkb * dt, fails);
+    if (n_cb_ob > RYR_NRYR) n_cb_ob = RYR_NRYR;
+
+    if (n_ou_ob + n_ou_cu > nou) {
+        if (n_ou_cu >= n_ou_ob) n_ou_cu = 0; else n_ou_ob = 0;
+        if (n_ou_ob > nou) n_ou_ob = 0;
+        else if (n_ou_cu > nou) n_ou_cu = 0;
+    }
+    if (n_ob_ou + n_ob_cb > nob) {
+        if (n_ob_ou >= n_ob_cb) n_ob_ou = 0; else n_ob_cb = 0;
+        if (n_ob_cb > nob) n_ob_cb = 0;
+        else if (n_ob_ou > nob) n_ob_ou = 0;
+    }
+    if (n_cu_ou + n_cu_cb > ncu) {
+        if (n_cu_cb >= n_cu_ou) n_cu_cb = 0; else n_cu_ou = 0;
+        if (n_cu_ou > ncu) n_cu_ou = 0;
+        else if (n_cu_cb > ncu) n_cu_cb = 0;
+    }
+    nou += -n_ou_ob - n_ou_cu + n_ob_ou + n_cu_ou;
+    nou = nou < 0 ? 0 : (nou > RYR_NRYR ? RYR_NRYR : nou);
+    nob += -n_ob_ou - n_ob_cb + n_ou_ob + n_cb_ob;
+    nob = nob < 0 ? 0 : (nob > RYR_NRYR ? RYR_NRYR : nob);
+    ncu += -n_cu_ou - n_cu_cb + n_ou_cu + n_cb_cu;
+    ncu = ncu < 0 ? 0 : (ncu > RYR_NRYR ? RYR_NRYR : ncu);
+    ncb = RYR_NRYR - nou - nob - ncu;
+    ncb = ncb < 0 ? 0 : (ncb > RYR_NRYR ? RYR_NRYR : ncb);
+}
+
+/* -------------------------------------------- membrane ODE (FP64, one lane) */
+/* cuda_Cell_Detailed.cu:447-739: whole-cell currents from the three sums,
+ * gates by Rush-Larsen, [Na]i and V by forward Euler. */
+RYR_DEV void membrane_step(RyrMem *M, const RyrCellInfo &C, double sum_icat, double sum_inaca,
+                           double sum_cs, double dt, double stim)
+{
+    RYR_STRICT_FP
+    double *s = M->m;
+    double v = s[0], xm = s[1], xh = s[2], xj = s[3], xr = s[4], xs1 = s[5], xs2 = s[6];
+    double qks = s[7], xtos = s[8], ytos = s[9], xtof = s[10], ytof = s[11], xnai = s[12];
+    const double xki = 140.0, xko = 5.40;
+    const double frt = (96.485 / 8.314 / 308.0);
+
+    double csubt = sum_cs / C.dN;
+    double xicatto = sum_icat / C.Cm * 0.0965 * (0.00126) * 2.0 * ((0.5) + 1.);
+    double xinacato = sum_inaca / C.Cm * 0.0965 * (0.025);
+
+    double ena = (1.0 / frt) * ryr_log(140.0 / xnai);
+    double am = 0.32 * (v + 47.13) / (1.0 - ryr_exp(-0.1 * (v + 47.13)));
+    double bm = 0.08 * ryr_exp(-v / 11.0);
+    double ah, bh, aj, bj;
+    if (v < -40.0) {
+        ah = 0.135 * ryr_exp((80.0 + v) / (-6.8));
+        bh = 3.56 * ryr_exp(0.079 * v) + 310000.0 * ryr_exp(0.35 * v);
+        aj = (-127140.0 * ryr_exp(0.2444 * v) - 0.00003474 * ryr_exp(-0.04391 * v))
+             * ((v + 37.78) / (1.0 + ryr_exp(0.311 * (v + 79.23))));
+        bj = (0.1212 * ryr_exp(-0.01052 * v)) / (1.0 + ryr_exp(-0.1378 * (v + 40.14)));
+    } else {
+        ah = 0.00;
+        bh = 1.00 / (0.130 * (1.00 + ryr_exp((v + 10.66) / (-11.10))));
+        aj = 0.00;
+        bj = (0.3 * ryr_exp(-0.00000025350 * v)) / (1.0 + ryr_exp(-0.10 * (v + 32.00)));
+    }
+    double tauh = 1.00 / (ah + bh);
+    double tauj = 1.00 / (aj + bj);
+    double taum = 1.00 / (am + bm);
+    double gna = 15;
+    double gnaleak = 0.3e-3 * 5;
+    double xina = gna * xh * xj * xm * xm * xm * (v - ena) + gnaleak * (v - ena);
+    xh = ah / (ah + bh) - ((ah / (ah + bh)) - xh) * ryr_exp(-dt / tauh);
+    xj = aj / (aj + bj) - ((aj / (aj + bj)) - xj) * ryr_exp(-dt / tauj);
+    xm = am / (am + bm) - ((am / (am + bm)) - xm) * ryr_exp(-dt / taum);
+
+    double ek = (1.00 / frt) * ryr_log(xko / xki);
+    double gss = __builtin_sqrt(xko / 5.40);
+    double xkrv1 = 0.001380 * (v + 7.00) / (1.0 - ryr_exp(-0.123 * (v + 7.00)));
+    double xkrv2 = 0.000610 * (v + 10.00) / (ryr_exp(0.1450 * (v + 10.00)) - 1.00);
+    double taukr = 1.00 / (xkrv1 + xkrv2);
+    double xkrinf = 1.00 / (1.00 + ryr_exp(-(v + 50.00) / 7.50));
+    double rg = 1.00 / (1.00 + ryr_exp((v + 33.00) / 22.40));
+    double gkr = 0.0078360;
+    double xikr = C.svikr * gkr * gss * xr * rg * (v - ek);
+    xr = xkrinf - (xkrinf - xr) * ryr_exp(-dt / taukr);
+
+    double prnak = 0.0183300;
+    double qks_inf = 0.2 * (1 + 0.8 / (1 + ryr_pow((0.28 / csubt), 3)));
+    double tauqks = 1000.00;
+    double eks = (1.00 / frt) * ryr_log((xko + prnak * 140.0) / (xki + prnak * xnai));
+    double xs1ss = 1.0 / (1.0 + ryr_exp(-(v - 1.500) / 16.700));
+    double tauxs = 1.00 / (0.0000719 * (v + 30.00) / (1.00 - ryr_exp(-0.1480 * (v + 30.0)))
+                           + 0.0001310 * (v + 30.00) / (ryr_exp(0.06870 * (v + 30.00)) - 1.00));
+    tauxs = 0.95 * tauxs;
+    double gksx = 0.2000;
+    double sviks_iso = 1.;
+    double xiks = 1.0 * sviks_iso * gksx * qks * xs1 * xs2 * (v - eks);
+    xs1 = xs1ss - (xs1ss - xs1) * ryr_exp(-dt / tauxs);
+    xs2 = xs1ss - (xs1ss - xs2) * ryr_exp(-dt / tauxs);
+    qks = qks + dt * (qks_inf - qks) / tauqks;
+
+    double gkix = 0.600;
+    double gki = gkix * (__builtin_sqrt(xko / 5.4));
+    double aki = 1.02 / (1.0 + ryr_exp(0.2385 * (v - ek - 59.215)));
+    double bki = (0.49124 * ryr_exp(0.08032 * (v - ek + 5.476)) + ryr_exp(0.061750 * (v - ek - 594.31)))
+                 / (1.0 + ryr_exp(-0.5143 * (v - ek + 4.753)));
+    double xkin = aki / (aki + bki);
+    double xik1 = (1.0) * gki * xkin * (v - ek);
+
+    double rt1 = -(v + 3.0) / 15.00;
+    double rt2 = (v + 33.5) / 10.00;
+    double rt3 = (v + 60.00) / 10.00;
+    double xtos_inf = 1.00 / (1.0 + ryr_exp(rt1));
+    double ytos_inf = 1.00 / (1.00 + ryr_exp(rt2));
+    double rs_inf = 1.00 / (1.00 + ryr_exp(rt2));
+    double txs = 9.00 / (1.00 + ryr_exp(-rt1)) + 0.50;
+    double tys = 3000.00 / (1.0 + ryr_exp(rt3)) + 30.00;
+    double gtos = 0.040;
+    double xitos = 1.0 * gtos * xtos * (ytos + 0.50 * rs_inf) * (v - ek);
+    xtos = xtos_inf - (xtos_inf - xtos) * ryr_exp(-dt / txs);
+    ytos = ytos_inf - (ytos_inf - ytos) * ryr_exp(-dt / tys);
+
+    double xtof_inf = xtos_inf;
+    double ytof_inf = ytos_inf;
+    double rt4 = -(v / 30.00) * (v / 30.00);
+    double rt5 = (v + 33.50) / 10.00;
+    double txf = 3.50 * ryr_exp(rt4) + 1.50;
+    double tyf = 20.0 / (1.0 + ryr_exp(rt5)) + 20.00;
+    double gtof = 0.10;
+    double xitof = 1.0 * gtof * xtof * ytof * (v - ek);
+    xtof = xtof_inf - (xtof_inf - xtof) * ryr_exp(-dt / txf);
+    ytof = ytof_inf - (ytof_inf - ytof) * ryr_exp(-dt / tyf);
+
+    double xkmko = 1.50, xkmnai = 12.00, xibarnak = 1.5000, hh = 1.00;
+    double sigma = (ryr_exp(140.0 / 67.30) - 1.00) / 7.00;
+    double fnak = 1.00 / (1.0 + 0.1245 * ryr_exp(-0.1 * v * frt) + 0.0365 * sigma * ryr_exp(-v * frt));
+    double xinak = 1.0 * xibarnak * fnak * (1.0 / (1.0 + ryr_pow((xkmnai / xnai), hh))) * xko / (xko + xkmko);
+
+    double wcainv = 1.0 / 50.0;
+    double conv = 0.18 * 12500;
+    double xinaca = xinacato;
+    double trick = 1.0;
+    double xrr = trick * (1.0 / wcainv / conv) / 1000.0;
+    double dnai = -xrr * (xina + 3.0 * xinak + 3.0 * xinaca);
+    xnai += dnai * dt;
+    double dvh = -(xina + xik1 + 1 * xikr + 1 * xiks + xitos + 1 * xitof + 1. * xinacato + 1 * xicatto + xinak) + stim;
+    v += dvh * dt;
+
+    s[0] = v; s[1] = xm; s[2] = xh; s[3] = xj; s[4] = xr; s[5] = xs1; s[6] = xs2;
+    s[7] = qks; s[8] = xtos; s[9] = ytos; s[10] = xtof; s[11] = ytof; s[12] = xnai;
+    double *cur = M->cur;
+    cur[0] = csubt; cur[1] = xicatto; cur[2] = xinacato; cur[3] = xina; cur[4] = xiks;
+    cur[5] = xikr; cur[6] = xik1; cur[7] = xinak; cur[8] = xitos; cur[9] = xitof;
+}
+
+RYR_DEV double stim_of(const RyrCellInfo &C, long long n, double amp)
+{
+    if (C.ncl <= 0) return 0.0;
+    long long ph = (n - C.noff) % C.ncl;
+    if (ph < 0) ph += C.ncl;
+    return ph < C.ndur ? amp : 0.0;
+}
+
+RYR_DEV int xcd_remap(int b, int nb)
+{
+    /* consecutive logical blocks on the same XCD (8 XCDs, round-robin dispatch) */
+    int per = nb >> 3;
+    if (b < (per << 3)) return (b & 7) * per + (b >> 3);
+    return b;
+}
+
+/* load a neighbour value on the fine grid; `inside` false -> the mirrored
+ * ghost, i.e. the centre value */
+template <typename T>
+RYR_DEV T nb_load(const T *__restrict__ f, long long idx, bool inside, T centre)
+{
+    return inside ? f[idx] : centre;
+}
+
+/* ------------------------------------------------------------ step kernel */
+template <typename TF, typename TS>
+__global__ __launch_bounds__(RYR_BLOCK) void k_cru_step(RyrKArgs a, double stim_amp)
+{
+    __shared__ RyrDD s_red[3][RYR_BLOCK / 64];
+    __shared__ int s_last;
+
+    const int bid = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int c = a.block_cell[bid];
+    const RyrCellInfo C = a.cells[c];
+    RyrMem *M = a.mem + c;
+    const long long nstep = M->step;
+    const int par = (int)(nstep & 1);
+    const int spo = a.spo;
+    const bool reset = (nstep == 0) || ((nstep - 1) % spo == 0);
+    const bool diag = (nstep % spo) == 0;
+    const int n = (bid - C.first_block) * RYR_BLOCK + (int)threadIdx.x;
+    const bool active = n < C.N;
+    const long long P = a.P;
+    const double dt = a.dt;
+
+    double r_icat = 0.0, r_inaca = 0.0, r_cs = 0.0;
+
+    if (active) {
+        const long long g = C.off + n;
+        const int X = C.X, Y = C.Y, Z = C.Z;
+        const int i = n % X, j = (n / X) % Y, k = n / (X * Y);
+        const long long sy = X, sz = (long long)X * Y;
+        const TF *__restrict__ ci0 = (const TF *)a.ci[par];
+        TF *__restrict__ ci1 = (TF *)a.ci[par ^ 1];
+        const TS *__restrict__ cn0 = (const TS *)a.cnsr[par];
+        TS *__restrict__ cn1 = (TS *)a.cnsr[par ^ 1];
+        const TF *__restrict__ cs0 = (const TF *)a.cs[par];
+        TF *__restrict__ cs1 = (TF *)a.cs[par ^ 1];
+        TF *catf = (TF *)a.catf, *cacal = (TF *)a.cacal, *casr = (TF *)a.casr;
+        TF *camyo = (TF *)a.camyo, *mgmyo = (TF *)a.mgmyo;
+        TS *cats = (TS *)a.cats;
+        TF *cpp = (TF *)a.cp, *ancx = (TF *)a.Ancx;
+        TF *casar = (TF *)a.casar, *casarh = (TF *)a.casarh, *casarj = (TF *)a.casarj, *casarhj = (TF *)a.casarhj;
+        TS *cjsrp = (TS *)a.cjsr, *cjsrbp = (TS *)a.cjsrb;
+
+        const double v = M->m[0];
+        TF cil[8];
+        TS cnl[8];
+#pragma unroll
+        for (int ii = 0; ii < 8; ++ii) { cil[ii] = ci0[ii * P + g]; cnl[ii] = cn0[ii * P + g]; }
+        const TF csv = cs0[g];
+        TF cpv = cpp[g];
+        const TS cjsrv = cjsrp[g];
+        uint32_t lw = a.lccw[g], rw = a.ryrw[g], ax = a.aux[g];
+        if (reset) ax = 0;
+        int nl = (int)(ax & 0xffffu), nspark = (int)((ax >> 16) & 1u);
+        int nfail = 0, nclampcp = 0, nclampcs = 0;
+
+        ryr_stream rs;
+        ryr_stream_init(&rs, C.seed, (uint32_t)n, (uint64_t)nstep);
+
+        /* LCC gating and I_CaL (cuda_Cell_Detailed.cu:1248-1272) */
+        const double cp_d = (double)cpv, cs_d = (double)csv;
+        double icat1, xicat;
+        {
+            RYR_STRICT_FP
+            const double cpx = (cp_d + cs_d * (0.5)) / (1 + (0.5));
+            const double betac = a.iso2 ? 2.0 : 4.0;
+            int nlcp = 0;
+            uint32_t nw = lw;
+            for (int ll = 0; ll < C.nlcc; ++ll) {
+                int st = (int)((lw >> (4 * ll)) & 15u);
+                int jj = lcc_step(st, cpx, dt, ryr_stream_uniform(&rs), *M, betac);
+                nw = (nw & ~(15u << (4 * ll))) | ((uint32_t)jj << (4 * ll));
+                if (jj == 0) { ++nl; ++nlcp; }
+            }
+            lw = nw;
+            double X_ = cpx / 1000.0 * M->ghk_e2 - (1 * 1.8);
+            double ica = M->ghk_small ? 2.0 * 11.9 * 96.485 * 0.341 * X_ : M->ghk_A * X_ / M->ghk_D;
+            if (ica > 0.0) ica = 0.0;
+            icat1 = C.svica * (double)(nlcp) * ica;
+            xicat = icat1 + 0.0 + 0.0 * ((0.025) / (0.00126));
+        }
+
+        /* RyR gating, spark, release (:1274-1293) */
+        int nou = (int)(rw & 255u), ncu = (int)((rw >> 8) & 255u), nob = (int)((rw >> 16) & 255u),
+            ncb = (int)(rw >> 24);
+        const int prevsp = nou + nob;
+        ryr_gating(&rs, C.Ku, C.Ku2, C.Kb, cp_d, (double)cjsrv, dt, nou, ncu, nob, ncb, &nfail);
+        if (prevsp < 5 && (nou + nob) >= 5) nspark = 1;
+        double xire;
+        {
+            RYR_STRICT_FP
+            double po = (double)(nou + nob) / (double)(RYR_NRYR);
+            double Jmax = (0.0147 * 11.5);
+            xire = 1.0 * Jmax * po * ((double)cjsrv - cp_d) / (0.00126) / 1.0;
+            if (xire < 0) xire = 0;
+        }
+
+        /* NCX (:1294-1304, subroutine_cuda.c:47-75), FP64 */
+        double xinaca;
+        TF ancx_new;
+        {
+            RYR_STRICT_FP
+            double cs = cs_d / 1000.0;
+            double Ka = (double)ancx[g];
+            double t2 = (87.5) * (87.5) * (87.5) * cs * (1.0 + cs / 0.00359);
+            double t3 = M->ncx_t3a + cs * ((140.0) * (140.0) * (140.0));
+            double Inaca = Ka * (21.0 / 1.0) * (M->ncx_n1 - M->ncx_n2 * cs)
+                           / ((M->ncx_t1 + t2 + t3) * M->ncx_den2);
+            double r = 0.0003 / cs;
+            double Ancxdot = (1.0 / (1.0 + r * r * r) - Ka) / 150.;
+            ancx_new = (TF)(Ka + Ancxdot * dt);
+            xinaca = C.svncx * 1.0 * Inaca;
+        }
+        ancx[g] = ancx_new;
+
+        /* exchange fluxes and JSR (:1309-1318) */
+        const TF cpT = cpv, csT = csv;
+        const TF dtT = (TF)dt;
+        const TF diffpi0 = (cpT - cil[0]) / TF(0.1) / TF(2.);
+        const TF diffpi1 = (cpT - cil[1]) / TF(0.1) / TF(2.);
+        const TF diffsi0 = (csT - cil[0]) / TF(0.04) / TF(2.);
+        const TF diffsi1 = (csT - cil[1]) / TF(0.04) / TF(2.);
+        const TS diffjn0 = (cjsrv - cnl[0]) / TS(25.0 * 1.0 / 4.) / TS(2.);
+        const TS diffjn1 = (cjsrv - cnl[1]) / TS(25.0 * 1.0 / 4.) / TS(2.);
+        TS dotcjsr, cjsrb_new;
+        {
+            RYR_STRICT_FP
+            const double cj = (double)cjsrv;
+            double roo2 = 5000.0 / (1.0 + ryr_pow(850.0 / cj, 23.0));
+            double mono = (-1.0 + __builtin_sqrt(1.0 + 8.0 * 460.0 * roo2)) / (4.0 * roo2 * 460.0);
+            double ene = mono * 20 + (1.0 - mono) * 30;
+            double enedot = (20 - 30) * (-mono + 1.0 / (4.0 * 460.0 * mono * roo2 + 1.0)) * (23.0 / cj)
+                            * (1.0 - roo2 / 5000.0);
+            double beta = 1.0 / (1.0 + (460.0 * 600.0 * ene + 460.0 * enedot * cj * (cj + 600.0))
+                                       / ((600.0 + cj) * (600.0 + cj)));
+            double flux = ((-(double)diffjn0 - (double)diffjn1) - xire * (0.00126) * 1.0 / (0.02 * 1));
+            dotcjsr = (TS)(beta * flux);
+            cjsrb_new = (TS)((double)cjsrbp[g] + (1 - beta) * flux * dt);
+        }
+
+        /* cytosol/NSR diffusion, buffers, updates (:1305-1393 / :1557-1586) */
+        const TF sviup = (TF)C.sviup_tot;
+        TF dotci01[2];
+        double up_acc = 0.0, leak_acc = 0.0;
+#pragma unroll
+        for (int ii = 0; ii < 8; ++ii) {
+            const long long pl = ii * P;
+            TF xiup;
+            if (sizeof(TF) == 8) xiup = (TF)((double)sviup * uptake_d((double)cil[ii], (double)cnl[ii]));
+            else xiup = sviup * (TF)uptake_f((float)cil[ii], (float)cnl[ii]);
+            const TF xileak = TF(1.0) * leak_t<TF>((TF)cnl[ii], cil[ii]);
+            up_acc = up_acc + (double)xiup / 8.;
+            leak_acc = leak_acc + (double)xileak / 8.;
+            /* neighbours on the fine grid (bit0 z, bit1 y, bit2 x) */
+            TF inN, inS, inE, inW, inT, inB;
+            TS nnN, nnS, nnE, nnW, nnT, nnB;
+            if (ii & 1) {
+                bool in = k + 1 < Z;
+                inN = nb_load<TF>(ci0, (ii - 1) * P + g + sz, in, cil[ii]);
+                nnN = nb_load<TS>(cn0, (ii - 1) * P + g + sz, in, cnl[ii]);
+                inS = cil[ii - 1]; nnS = cnl[ii - 1];
+            } else {
+                bool in = k > 0;
+                inN = cil[ii + 1]; nnN = cnl[ii + 1];
+                inS = nb_load<TF>(ci0, (ii + 1) * P + g - sz, in, cil[ii]);
+                nnS = nb_load<TS>(cn0, (ii + 1) * P + g - sz, in, cnl[ii]);
+            }
+            if (ii & 2) {
+                bool in = j + 1 < Y;
+                inE = nb_load<TF>(ci0, (ii - 2) * P + g + sy, in, cil[ii]);
+                nnE = nb_load<TS>(cn0, (ii - 2) * P + g + sy, in, cnl[ii]);
+                inW = cil[ii - 2]; nnW = cnl[ii - 2];
+            } else {
+                bool in = j > 0;
+                inE = cil[ii + 2]; nnE = cnl[ii + 2];
+                inW = nb_load<TF>(ci0, (ii + 2) * P + g - sy, in, cil[ii]);
+                nnW = nb_load<TS>(cn0, (ii + 2) * P + g - sy, in, cnl[ii]);
+            }
+            if (ii & 4) {
+                bool in = i + 1 < X;
+                inT = nb_load<TF>(ci0, (ii - 4) * P + g + 1, in, cil[ii]);
+                nnT = nb_load<TS>(cn0, (ii - 4) * P + g + 1, in, cnl[ii]);
+                inB = cil[ii - 4]; nnB = cnl[ii - 4];
+            } else {
+                bool in = i > 0;
+                inT = cil[ii + 4]; nnT = cnl[ii + 4];
+                inB = nb_load<TF>(ci0, (ii + 4) * P + g - 1, in, cil[ii]);
+                nnB = nb_load<TS>(cn0, (ii + 4) * P + g - 1, in, cnl[ii]);
+            }
+            const TF cc = cil[ii];
+            const TS cn = cnl[ii];
+            const TS xicoupn = ((nnN - cn) / TS(1.8 * 0.7) + (nnS - cn) / TS(1.8 * 0.7) +
+                                (nnE - cn) / TS(1.8 * 0.7) + (nnW - cn) / TS(1.8 * 0.7) +
+                                (nnT - cn) / TS(6. * 0.7) + (nnB - cn) / TS(6. * 0.7));
+            const TF xicoupi = ((inN - cc) / TF((0.33 * 2.) * 0.7) + (inS - cc) / TF((0.33 * 2.) * 0.7) +
+                                (inE - cc) / TF((0.33 * 2.) * 0.7) + (inW - cc) / TF((0.33 * 2.) * 0.7) +
+                                (inT - cc) / TF((0.7 * 2.) * 0.7) + (inB - cc) / TF((0.7 * 2.) * 0.7));
+            TS dotcnsr = (TS)(xiup - xileak) * TS(0.5 / 8.) / TS(0.025 / 8.) + xicoupn;
+            const TF v_catf = catf[pl + g], v_cacal = cacal[pl + g], v_casr = casr[pl + g];
+            const TF v_camyo = camyo[pl + g], v_mgmyo = mgmyo[pl + g];
+            const TS v_cats = cats[pl + g];
+            const TF f_tf = b_tropf<TF>(v_catf, cc, dtT);
+            const TS f_ts = b_trops<TS>(v_cats, (TS)cc, (TS)dtT);
+            const TF f_cal = b_bucal<TF>(v_cacal, cc, dtT);
+            const TF f_sr = b_busr<TF>(v_casr, cc, dtT);
+            const TF f_myo = b_myoca<TF>(v_camyo, v_mgmyo, cc, dtT);
+            const TF buffers = (f_tf + (TF)f_ts + f_cal + f_sr + f_myo);
+            TF dotci = (-xiup + xileak) - buffers + xicoupi;
+            if (ii < 2) {
+                dotcnsr += (ii == 0 ? diffjn0 : diffjn1) * TS(0.02 * 1) / TS(0.025 / 8.);
+                if (a.scheme == 0) {
+                    dotci += (ii == 0 ? diffsi0 : diffsi1) * TF((0.025) / (0.5 / 8.));
+                    dotci += (ii == 0 ? diffpi0 : diffpi1) * TF((0.00126) / (0.5 / 8.));
+                }
+                dotci01[ii] = dotci;
+            }
+            if (ii >= 2 || a.scheme == 0) {
+                TF cin = cc + dotci * dtT;
+                if (cin < TF(0)) cin = TF(0);
+                ci1[pl + g] = cin;
+            }
+            cn1[pl + g] = cn + dotcnsr * (TS)dtT;
+            TF n_catf = v_catf + f_tf * dtT;
+            if (n_catf < TF(0)) n_catf = TF(0);
+            TS n_cats = v_cats + f_ts * (TS)dtT;
+            if (n_cats < TS(0)) n_cats = TS(0);
+            TF n_cacal = v_cacal + f_cal * dtT;
+            if (n_cacal < TF(0)) n_cacal = TF(0);
+            TF n_casr = v_casr + f_sr * dtT;
+            if (n_casr < TF(0)) n_casr = TF(0);
+            TF n_camyo = v_camyo + f_myo * dtT;
+            if (n_camyo < TF(0)) n_camyo = TF(0);
+            TF n_mgmyo = v_mgmyo + b_myomg<TF>(n_camyo, v_mgmyo, dtT) * dtT;
+            if (n_mgmyo < TF(0)) n_mgmyo = TF(0);
+            catf[pl + g] = n_catf; cats[pl + g] = n_cats; cacal[pl + g] = n_cacal;
+            casr[pl + g] = n_casr; camyo[pl + g] = n_camyo; mgmyo[pl + g] = n_mgmyo;
+        }
+
+        /* cleft / sub-membrane sub-steps */
+        TF v_casar = casar[g], v_casarh = casarh[g], v_casarj = casarj[g], v_casarhj = casarhj[g];
+        const TF dth = dtT / TF(5);
+        const TF cs_up = (k + 1 < Z) ? cs0[g + sz] : csv;
+        const TF cs_dn = (k > 0) ? cs0[g - sz] : csv;
+        const TF icatT = (TF)icat1, xinacaT = (TF)xinaca, xireT = (TF)xire;
+        TF csn = csv;
+        if (a.scheme == 0) {
+            /* scheme A (:1396-1464) */
+            const TF cpstart = cpT;
+            for (int it = 0; it < 5; ++it) {
+                const TF fsar = b_busar<TF>(v_casar, csn, dth);
+                const TF fsarh = b_busarh<TF>(v_casarh, csn, dth);
+                const TF csbuffers = (fsar + fsarh);
+                const TF csdiff = (cs_up + cs_dn - TF(2) * csv) / (TF(1.42));
+                const TF dotcs = TF((0.00126) / (0.025)) * (cpv - csn) / (TF(0.0283))
+                                 - TF(0.5) * icatT * TF(0.00126) / TF(0.025)
+                                 + xinacaT
+                                 - (csv - cil[0]) / TF(0.04) / TF(2.)
+                                 - (csv - cil[1]) / TF(0.04) / TF(2.)
+                                 + csdiff
+                                 - csbuffers
+                                 - TF(0);
+                const TF cpbuff = b_busar<TF>(v_casarj, cpv, dth);
+                const TF cpbuffh = b_busarh<TF>(v_casarhj, cpv, dth);
+                const TF cpdot = ((csn - cpv) / TF(0.0283) +
+                                  (cil[0] - cpstart) / TF(0.1) / TF(2.) +
+                                  (cil[1] - cpstart) / TF(0.1) / TF(2.) +
+                                  xireT - icatT - cpbuff - cpbuffh);
+                v_casarj += cpbuff * dtT / TF(5);
+                if (v_casarj < TF(0)) v_casarj = TF(0);
+                v_casarhj += cpbuffh * dtT / TF(5);
+                if (v_casarhj < TF(0)) v_casarhj = TF(0);
+                cpv += cpdot * dtT / TF(5);
+                v_casar += fsar * dtT / TF(5);
+                if (v_casar < TF(0)) v_casar = TF(0);
+                v_casarh += fsarh * dtT / TF(5);
+                if (v_casarh < TF(0)) v_casarh = TF(0);
+                csn += (dotcs) * dth;
+            }
+        } else {
+            /* scheme B (:1479-1553) */
+            for (int it = 0; it < 5; ++it) {
+                const TF dpi0 = (cpv - cil[0]) / TF(0.1) / TF(2.);
+                const TF dpi1 = (cpv - cil[1]) / TF(0.1) / TF(2.);
+                const TF dsi0 = (csn - cil[0]) / TF(0.04) / TF(2.);
+                const TF dsi1 = (csn - cil[1]) / TF(0.04) / TF(2.);
+                dotci01[0] += dsi0 * TF((0.025) / (0.5 / 8.)) / TF(5);
+                dotci01[1] += dsi1 * TF((0.025) / (0.5 / 8.)) / TF(5);
+                dotci01[0] += dpi0 * TF((0.00126) / (0.5 / 8.)) / TF(5);
+                dotci01[1] += dpi1 * TF((0.00126) / (0.5 / 8.)) / TF(5);
+                const TF fsar = b_busar<TF>(v_casar, csn, dth);
+                const TF fsarh = b_busarh<TF>(v_casarh, csn, dth);
+                const TF csbuffers = (fsar + fsarh);
+                const TF csdiff = (cs_up + cs_dn - TF(2) * csn) / (TF(1.42));
+                const TF dotcs = TF((0.00126) / (0.025)) * (cpv - TF(0.5 * 0.0283) * icatT - csn) / (TF(0.0283))
+                                 + xinacaT
+                                 - (csn - cil[0]) / TF(0.04) / TF(2.)
+                                 - (csn - cil[1]) / TF(0.04) / TF(2.)
+                                 + csdiff
+                                 - csbuffers
+                                 - TF(0);
+                const TF cpbuff = b_busar<TF>(v_casarj, cpv, dth);
+                const TF cpbuffh = b_busarh<TF>(v_casarhj, cpv, dth);
+                const TF cpdot = ((csT - cpv) / TF(0.0283) +
+                                  (cil[0] - cpv) / TF(0.1) / TF(2.) +
+                                  (cil[1] - cpv) / TF(0.1) / TF(2.) +
+                                  xireT - icatT - cpbuff - cpbuffh);
+                v_casar += fsar * dth;
+                if (v_casar < TF(0)) v_casar = TF(0);
+                v_casarh += fsarh * dth;
+                if (v_casarh < TF(0)) v_casarh = TF(0);
+                csn += (dotcs) * dth;
+                v_casarj += cpbuff * dth;
+                if (v_casarj < TF(0)) v_casarj = TF(0);
+                v_casarhj += cpbuffh * dth;
+                if (v_casarhj < TF(0)) v_casarhj = TF(0);
+                cpv += cpdot * dth;
+            }
+#pragma unroll
+            for (int ii = 0; ii < 2; ++ii) {
+                TF cin = cil[ii] + dotci01[ii] * dtT;
+                if (cin < TF(0)) cin = TF(0);
+                ci1[ii * P + g] = cin;
+            }
+        }
+        if (cpv < TF(0)) { cpv = TF(0); ++nclampcp; }
+        if (csn < TF(0)) { csn = TF(1e-6); ++nclampcs; }
+        casar[g] = v_casar; casarh[g] = v_casarh; casarj[g] = v_casarj; casarhj[g] = v_casarhj;
+        cpp[g] = cpv;
+        cs1[g] = csn;
+        cjsrp[g] = cjsrv + dotcjsr * (TS)dtT;
+        cjsrbp[g] = cjsrb_new;
+        a.lccw[g] = lw;
+        a.ryrw[g] = (uint32_t)nou | ((uint32_t)ncu << 8) | ((uint32_t)nob << 16) | ((uint32_t)ncb << 24);
+        a.aux[g] = (uint32_t)(nl & 0xffff) | ((uint32_t)nspark << 16);
+        if (diag) {
+            a.d_xire[g] = xire;
+            a.d_xinaca[g] = xinaca;
+            a.d_up[g] = up_acc;
+            a.d_leak[g] = leak_acc;
+        }
+        if (nfail | nclampcp | nclampcs) {
+            unsigned long long *cnt = a.counters + (long long)c * RYR_NCNT;
+            if (nfail) atomicAdd(cnt + 0, (unsigned long long)nfail);
+            if (nclampcp) atomicAdd(cnt + 1, 1ull);
+            if (nclampcs) atomicAdd(cnt + 2, 1ull);
+        }
+        r_icat = xicat;
+        r_inaca = xinaca;
+        r_cs = (double)csn;
+    }
+
+    /* whole-cell sums: pairwise double-double tree over this block's 256
+     * CRUs (index order), one partial per block */
+    {
+        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        RyrDD t0 = {r_icat, 0.0}, t1 = {r_inaca, 0.0}, t2 = {r_cs, 0.0};
+        t0 = wave_tree(t0);
+        t1 = wave_tree(t1);
+        t2 = wave_tree(t2);
+        if (lane == 0) { s_red[0][wv] = t0; s_red[1][wv] = t1; s_red[2][wv] = t2; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            RyrDD *pp = a.partial + 3LL * bid;
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+                pp[q] = dd_add(dd_add(s_red[q][0], s_red[q][1]), dd_add(s_red[q][2], s_red[q][3]));
+            __threadfence();
+            unsigned int prev = atomicAdd(a.arrive + c, 1u);
+            s_last = (prev == (unsigned int)(C.nblocks - 1));
+        }
+        __syncthreads();
+    }
+    if (!s_last) return;
+
+    /* last block of the cell: chunk partials in order, membrane ODE, factors
+     * for the next step (cuda_Cell_Detailed.cu:430-739) */
+    if (threadIdx.x == 0) {
+        __threadfence();
+        RyrDD s0 = {0.0, 0.0}, s1 = {0.0, 0.0}, s2 = {0.0, 0.0};
+        const RyrDD *pp = a.partial + 3LL * C.first_block;
+        for (int b = 0; b < C.nblocks; ++b) {
+            s0 = dd_add(s0, pp[3 * b + 0]);
+            s1 = dd_add(s1, pp[3 * b + 1]);
+            s2 = dd_add(s2, pp[3 * b + 2]);
+        }
+        const double stim = stim_of(C, nstep, stim_amp);
+        membrane_step(M, C, s0.hi, s1.hi, s2.hi, dt, stim);
+        M->stim = stim;
+        M->out_ica += M->cur[1];
+        M->out_ncx += M->cur[2];
+        M->out_ina += M->cur[3];
+        const double vnew = M->m[0];
+        if (!(vnew == vnew) || vnew == __builtin_inf() || vnew == -__builtin_inf())
+            atomicAdd(a.counters + (long long)c * RYR_NCNT + 4, 1ull);
+        if (diag) {
+            /* the membrane columns of the output row (:1041-1072) */
+            double *r = a.diag + (long long)c * 45;
+            double spod = (double)spo;
+            r[0] = (double)nstep * dt; r[2] = vnew;
+            r[3] = M->out_ncx / spod; r[4] = M->out_ica / spod; r[6] = M->cur[0];
+            r[10] = M->m[12]; r[11] = M->cur[4]; r[12] = M->cur[5]; r[13] = M->cur[6];
+            r[14] = M->cur[7]; r[15] = M->cur[8]; r[16] = M->cur[9];
+            r[17] = M->out_ina / spod; r[18] = M->m[4]; r[38] = M->m[5]; r[39] = M->m[7];
+            M->out_ica = 0; M->out_ncx = 0; M->out_ina = 0;
+            a.diag_step[c] = nstep;
+        }
+        vfactors(M, a.iso2);
+        M->step = nstep + 1;
+        a.arrive[c] = 0u;
+        __threadfence();
+    }
+}
+
+/* ------------------------------------------------------- output-row kernel */
+/* One workgroup per cell; the sums run chunk by chunk in the same order as
+ * the step reduction (pairwise tree in a chunk, chunks in order). */
+#define RYR_DIAG_Q 21
+template <typename TF, typename TS>
+__global__ __launch_bounds__(RYR_BLOCK) void k_diag(RyrKArgs a)
+{
+    __shared__ RyrDD s_red[RYR_DIAG_Q][RYR_BLOCK / 64];
+    __shared__ unsigned long long s_cnt[16];
+    const int c = blockIdx.x;
+    const RyrCellInfo C = a.cells[c];
+    RyrMem *M = a.mem + c;
+    if (a.diag_step[c] != M->step - 1) return;   /* not an output step */
+    const int par = (int)(M->step & 1);          /* buffer holding the new state */
+    const long long P = a.P;
+    const TF *ci = (const TF *)a.ci[par];
+    const TS *cn = (const TS *)a.cnsr[par];
+    const TF *cs = (const TF *)a.cs[par];
+    const TF *catf = (const TF *)a.catf, *cacal = (const TF *)a.cacal, *casr = (const TF *)a.casr;
+    const TF *camyo = (const TF *)a.camyo;
+    const TS *cats = (const TS *)a.cats;
+    const TF *cp = (const TF *)a.cp, *ancx = (const TF *)a.Ancx;
+    const TF *casar = (const TF *)a.casar, *casarh = (const TF *)a.casarh;
+    const TF *casarj = (const TF *)a.casarj, *casarhj = (const TF *)a.casarhj;
+    const TS *cjsr = (const TS *)a.cjsr, *cjsrb = (const TS *)a.cjsrb;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (threadIdx.x < 16) s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+
+    RyrDD acc[RYR_DIAG_Q];
+#pragma unroll
+    for (int q = 0; q < RYR_DIAG_Q; ++q) { acc[q].hi = 0.0; acc[q].lo = 0.0; }
+    unsigned long long ic[13] = {0};  /* tnou tnob tncu tncb tn1..tn7 sparks runaway */
+
+    for (int base = 0; base < C.N; base += RYR_BLOCK) {
+        const int n = base + (int)threadIdx.x;
+        double val[RYR_DIAG_Q];
+#pragma unroll
+        for (int q = 0; q < RYR_DIAG_Q; ++q) val[q] = 0.0;
+        if (n < C.N) {
+            RYR_STRICT_FP
+            const long long g = C.off + n;
+            const uint32_t rw = a.ryrw[g], lw = a.lccw[g], ax = a.aux[g];
+            const int nou = rw & 255, ncu = (rw >> 8) & 255, nob = (rw >> 16) & 255, ncb = rw >> 24;
+            double xin = a.d_xinaca[g];
+            val[0] = xin < 0 ? xin : 0.0;
+            val[1] = (double)ancx[g];
+            val[2] = (double)cp[g];
+            val[3] = (double)cjsr[g];
+            val[4] = (double)cjsrb[g];
+            val[5] = (double)(nou + nob) / (double)(RYR_NRYR);
+            val[6] = (double)ncb;
+            val[7] = (double)casar[g];
+            val[8] = (double)casarh[g];
+            val[9] = (double)casarj[g];
+            val[10] = (double)casarhj[g];
+            double s_ci = 0, s_cn = 0, s_tf = 0, s_ts = 0, s_sr = 0, s_my = 0, s_cal = 0;
+            for (int ii = 0; ii < 8; ++ii) {
+                const long long e = ii * P + g;
+                s_ci = s_ci + (double)ci[e] / 8.;
+                s_cn = s_cn + (double)cn[e] / 8.;
+                s_tf = s_tf + (double)catf[e] / 8.;
+                s_ts = s_ts + (double)cats[e] / 8.;
+                s_sr = s_sr + (double)casr[e] / 8.;
+                s_my = s_my + (double)camyo[e] / 8.;
+                s_cal = s_cal + (double)cacal[e] / 8.;
+                if ((double)ci[e] > 1000) ic[12]++;
+            }
+            val[11] = s_ci; val[12] = s_cn; val[13] = s_tf; val[14] = s_ts; val[15] = s_sr;
+            val[16] = s_my; val[17] = s_cal;
+            val[18] = a.d_leak[g];
+            val[19] = a.d_up[g];
+            val[20] = a.d_xire[g];
+            ic[0] += nou; ic[1] += nob; ic[2] += ncu; ic[3] += ncb;
+            ic[11] += (ax >> 16) & 1u;
+            for (int jj = 0; jj < C.nlcc; ++jj) {
+                const int st = (lw >> (4 * jj)) & 15;
+                /* histogram of cuda_Cell_Detailed.cu:904-923 */
+                if (st == 0) { ic[10]++; ic[9]++; continue; }
+                if (st & 1) ic[4]++;
+                if (st & 2) ic[5]++;
+                if (st & 4) ic[6]++;
+                if (st & 8) ic[7]++;
+                if ((st & 4) && !(st & 8)) ic[8]++;
+                if (st == 1) ic[9]++;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < RYR_DIAG_Q; ++q) {
+            RyrDD t = {val[q], 0.0};
+            t = wave_tree(t);
+            if (lane == 0) s_red[q][wv] = t;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int q = 0; q < RYR_DIAG_Q; ++q)
+                acc[q] = dd_add(acc[q], dd_add(dd_add(s_red[q][0], s_red[q][1]), dd_add(s_red[q][2], s_red[q][3])));
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < 13; ++q)
+        if (ic[q]) atomicAdd(&s_cnt[q], ic[q]);
+    __syncthreads();
+
+    if (threadIdx.x == 0) {
+        RYR_STRICT_FP
+        const double dN = C.dN;
+        double *r = a.diag + (long long)c * 45;
+        double ncxfwd = acc[0].hi / dN, outAncx = acc[1].hi / dN, cproxit = acc[2].hi / dN;
+        double cjsrt = acc[3].hi / dN, csqnbt = acc[4].hi / dN, poto = acc[5].hi / dN;
+        double pbcto = acc[6].hi / dN / 100.;
+        double casart = acc[7].hi / dN, casarht = acc[8].hi / dN, casarjt = acc[9].hi / dN;
+        double casarhjt = acc[10].hi / dN;
+        double cit = acc[11].hi / dN, cnsrt = acc[12].hi / dN, catft = acc[13].hi / dN;
+        double catst = acc[14].hi / dN, casrt = acc[15].hi / dN, camyot = acc[16].hi / dN;
+        double cacalt = acc[17].hi / dN, leakt = acc[18].hi / dN, upt = acc[19].hi / dN;
+        double ire = acc[20].hi / dN;
+        double csubt = M->cur[0];
+        double Total_Ca = (cit + catft + catst + casrt + camyot + cacalt) * (0.5 / 8.) * 8. +
+                          (csubt + casart + casarht) * (0.025) +
+                          (cproxit + casarjt + casarhjt) * (0.00126) +
+                          (cnsrt) * (0.025 / 8.) * 8. +
+                          (cjsrt + csqnbt) * (0.02 * 1);
+        double Total_Ci = (cit + catft + catst + casrt + camyot + cacalt) * (0.5 / 8.) * 8. +
+                          (csubt + casart + casarht) * (0.025) +
+                          (cproxit + casarjt + casarhjt) * (0.00126);
+        r[1] = cit; r[5] = cproxit; r[7] = cjsrt; r[8] = cnsrt; r[9] = poto;
+        r[19] = ncxfwd * ((0.025) / (0.00126)); r[20] = pbcto; r[21] = cacalt; r[22] = catft;
+        r[23] = leakt; r[24] = upt; r[25] = ire;
+        r[26] = (double)s_cnt[0]; r[27] = (double)s_cnt[1]; r[28] = (double)s_cnt[2]; r[29] = (double)s_cnt[3];
+        for (int q = 0; q < 7; ++q) r[30 + q] = (double)s_cnt[4 + q];
+        r[37] = outAncx; r[40] = (double)s_cnt[11]; r[41] = catst; r[42] = csqnbt;
+        r[43] = Total_Ci; r[44] = Total_Ca;
+        if (s_cnt[12]) a.counters[(long long)c * RYR_NCNT + 3] += s_cnt[12];
+    }
+    /* linescan row at y = line_j, z = line_k (interior indices) */
+    double *L = a.line + a.line_off[c];
+    for (int x = threadIdx.x; x < C.X; x += RYR_BLOCK) {
+        const int n = x + C.X * (C.line_j + C.Y * C.line_k);
+        const long long g = C.off + n;
+        const uint32_t rw = a.ryrw[g], ax = a.aux[g];
+        double *o = L + (long long)x * 22;
+        o[0] = (double)(M->step - 1) * a.dt; o[1] = (double)(x + 1); o[2] = (double)ci[g];
+        o[3] = (double)((rw >> 16) & 255); o[4] = (double)(rw & 255); o[5] = (double)(rw >> 24);
+        o[6] = (double)((rw >> 8) & 255); o[7] = (double)(ax & 0xffff); o[8] = (double)cs[g];
+        o[9] = (double)cp[g]; o[10] = (double)casar[g]; o[11] = (double)casarh[g];
+        o[12] = (double)camyo[g]; o[13] = (double)((const TF *)a.mgmyo)[g]; o[14] = (double)casr[g];
+        o[15] = (double)cacal[g]; o[16] = (double)cats[g]; o[17] = (double)catf[g];
+        o[18] = a.d_xire[g]; o[19] = (double)cjsr[g]; o[20] = (double)ancx[g]; o[21] = a.d_xinaca[g];
+    }
+}
+
+/* ---------------------------------------------------- set-up / data access */
+/* per-cell initial values, computed on the host from Init
+ * (cuda_Cell_Detailed.cu:1108-1182) */
+struct RyrInit {
+    double ci0, cjsr0, cjsrb0, catf0, cats0, cacal0, casr0, camyo0, mgmyo0;
+    double casar0, casarh0, ancx0;
+    int ncb0, ncu0;
+    uint32_t lcc0;
+    int pad_;
+};
+
+template <typename TF, typename TS>
+__global__ __launch_bounds__(RYR_BLOCK) void k_init(RyrKArgs a, const RyrInit *__restrict__ init)
+{
+    const int b = blockIdx.x;
+    const int c = a.block_cell[b];
+    const RyrCellInfo C = a.cells[c];
+    const long long g = (long long)b * RYR_BLOCK + threadIdx.x;
+    const long long n = g - C.off;
+    if (n >= C.N) return;
+    const RyrInit I = init[c];
+    const long long P = a.P;
+    for (int buf = 0; buf < 2; ++buf) {
+        TF *ci = (TF *)a.ci[buf];
+        TS *cn = (TS *)a.cnsr[buf];
+        for (int ii = 0; ii < 8; ++ii) { ci[ii * P + g] = (TF)I.ci0; cn[ii * P + g] = (TS)I.cjsr0; }
+        ((TF *)a.cs[buf])[g] = (TF)I.ci0;
+    }
+    for (int ii = 0; ii < 8; ++ii) {
+        const long long e = ii * P + g;
+        ((TF *)a.catf)[e] = (TF)I.catf0;
+        ((TS *)a.cats)[e] = (TS)I.cats0;
+        ((TF *)a.cacal)[e] = (TF)I.cacal0;
+        ((TF *)a.casr)[e] = (TF)I.casr0;
+        ((TF *)a.camyo)[e] = (TF)I.camyo0;
+        ((TF *)a.mgmyo)[e] = (TF)I.mgmyo0;
+    }
+    ((TF *)a.cp)[g] = (TF)I.ci0;
+    ((TS *)a.cjsr)[g] = (TS)I.cjsr0;
+    ((TS *)a.cjsrb)[g] = (TS)I.cjsrb0;
+    ((TF *)a.Ancx)[g] = (TF)I.ancx0;
+    ((TF *)a.casar)[g] = (TF)I.casar0;
+    ((TF *)a.casarh)[g] = (TF)I.casarh0;
+    ((TF *)a.casarj)[g] = (TF)I.casar0;
+    ((TF *)a.casarhj)[g] = (TF)I.casarh0;
+    a.lccw[g] = I.lcc0;
+    a.ryrw[g] = (uint32_t)0 | ((uint32_t)I.ncu0 << 8) | ((uint32_t)0 << 16) | ((uint32_t)I.ncb0 << 24);
+    a.aux[g] = 0u;
+    a.d_xire[g] = 0.0; a.d_xinaca[g] = 0.0; a.d_up[g] = 0.0; a.d_leak[g] = 0.0;
+}
+
+/* Non-template kernels: defined in exactly one unit (the API). */
+#ifdef RYR_DEFINE_COMMON_KERNELS
+__global__ void k_vfactors(RyrMem *mem, int ncells, int iso2)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < ncells) vfactors(mem + c, iso2);
+}
+#else
+__global__ void k_vfactors(RyrMem *mem, int ncells, int iso2);
+#endif
+
+/* field ids follow enum ryr_field of the public header */
+template <typename TF, typename TS>
+RYR_DEV double field_get(const RyrKArgs &a, int f, int cur, long long e, long long g, int p)
+{
+    switch (f) {
+    case 0: return (double)((const TF *)a.ci[cur])[e];
+    case 1: return (double)((const TS *)a.cnsr[cur])[e];
+    case 2: return (double)((const TF *)a.catf)[e];
+    case 3: return (double)((const TS *)a.cats)[e];
+    case 4: return (double)((const TF *)a.cacal)[e];
+    case 5: return (double)((const TF *)a.casr)[e];
+    case 6: return (double)((const TF *)a.camyo)[e];
+    case 7: return (double)((const TF *)a.mgmyo)[e];
+    case 8: return (double)((const TF *)a.cs[cur])[g];
+    case 9: return (double)((const TF *)a.cp)[g];
+    case 10: return (double)((const TS *)a.cjsr)[g];
+    case 11: return (double)((const TS *)a.cjsrb)[g];
+    case 12: return (double)((const TF *)a.Ancx)[g];
+    case 13: return (double)((const TF *)a.casar)[g];
+    case 14: return (double)((const TF *)a.casarh)[g];
+    case 15: return (double)((const TF *)a.casarj)[g];
+    case 16: return (double)((const TF *)a.casarhj)[g];
+    case 17: return (double)(a.ryrw[g] & 255u);
+    case 18: return (double)((a.ryrw[g] >> 8) & 255u);
+    case 19: return (double)((a.ryrw[g] >> 16) & 255u);
+    case 20: return (double)(a.ryrw[g] >> 24);
+    case 21: return (double)((a.lccw[g] >> (4 * p)) & 15u);
+    case 22: return (double)(a.aux[g] & 0xffffu);
+    case 23: return (double)((a.aux[g] >> 16) & 1u);
+    }
+    return 0.0;
+}
+
+template <typename TF, typename TS>
+RYR_DEV void field_set(const RyrKArgs &a, int f, int cur, long long e, long long g, int p, double v)
+{
+    switch (f) {
+    case 0: ((TF *)a.ci[cur])[e] = (TF)v; break;
+    case 1: ((TS *)a.cnsr[cur])[e] = (TS)v; break;
+    case 2: ((TF *)a.catf)[e] = (TF)v; break;
+    case 3: ((TS *)a.cats)[e] = (TS)v; break;
+    case 4: ((TF *)a.cacal)[e] = (TF)v; break;
+    case 5: ((TF *)a.casr)[e] = (TF)v; break;
+    case 6: ((TF *)a.camyo)[e] = (TF)v; break;
+    case 7: ((TF *)a.mgmyo)[e] = (TF)v; break;
+    case 8: ((TF *)a.cs[cur])[g] = (TF)v; break;
+    case 9: ((TF *)a.cp)[g] = (TF)v; break;
+    case 10: ((TS *)a.cjsr)[g] = (TS)v; break;
+    case 11: ((TS *)a.cjsrb)[g] = (TS)v; break;
+    case 12: ((TF *)a.Ancx)[g] = (TF)v; break;
+    case 13: ((TF *)a.casar)[g] = (TF)v; break;
+    case 14: ((TF *)a.casarh)[g] = (TF)v; break;
+    case 15: ((TF *)a.casarj)[g] = (TF)v; break;
+    case 16: ((TF *)a.casarhj)[g] = (TF)v; break;
+    case 17: a.ryrw[g] = (a.ryrw[g] & ~255u) | ((uint32_t)v & 255u); break;
+    case 18: a.ryrw[g] = (a.ryrw[g] & ~(255u << 8)) | (((uint32_t)v & 255u) << 8); break;
+    case 19: a.ryrw[g] = (a.ryrw[g] & ~(255u << 16)) | (((uint32_t)v & 255u) << 16); break;
+    case 20: a.ryrw[g] = (a.ryrw[g] & ~(255u << 24)) | (((uint32_t)v & 255u) << 24); break;
+    case 21: a.lccw[g] = (a.lccw[g] & ~(15u << (4 * p))) | (((uint32_t)v & 15u) << (4 * p)); break;
+    case 22: a.aux[g] = (a.aux[g] & ~0xffffu) | ((uint32_t)v & 0xffffu); break;
+    case 23: a.aux[g] = (a.aux[g] & 0xffffu) | (((uint32_t)v & 1u) << 16); break;
+    }
+}
+
+/* planes*N elements, element e = p*N + n (one thread each) */
+template <typename TF, typename TS>
+__global__ void k_export(RyrKArgs a, int cell, int field, int planes, double *out)
+{
+    const RyrCellInfo C = a.cells[cell];
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)planes * C.N) return;
+    const int p = (int)(e / C.N);
+    const long long n = e % C.N;
+    const long long g = C.off + n;
+    const int cur = (int)(a.mem[cell].step & 1);
+    out[e] = field_get<TF, TS>(a, field, cur, (long long)p * a.P + g, g, p);
+}
+
+template <typename TF, typename TS>
+__global__ void k_import(RyrKArgs a, int cell, int field, int planes, const double *in)
+{
+    const RyrCellInfo C = a.cells[cell];
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)planes * C.N) return;
+    const int p = (int)(e / C.N);
+    const long long n = e % C.N;
+    const long long g = C.off + n;
+    const int cur = (int)(a.mem[cell].step & 1);
+    field_set<TF, TS>(a, field, cur, (long long)p * a.P + g, g, p, in[e]);
+}
+
+/* frozen-input gating probe: LCC loop (:1248-1261) then ryrgating */
+#ifdef RYR_DEFINE_COMMON_KERNELS
+__global__ void k_gating_probe(long long nq, unsigned long long seed, const uint32_t *cru,
+                               unsigned long long step, double Ku, double Ku2, double Kb, double dt,
+                               int iso2, int32_t *counts, int32_t *lcc, const double *conc,
+                               int32_t *draws, int32_t *nopen, int32_t *fails)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    ryr_stream rs;
+    ryr_stream_init(&rs, seed, cru[q], step);
+    const double cp = conc[4 * q], cjsr = conc[4 * q + 1], cs = conc[4 * q + 2], v = conc[4 * q + 3];
+    RyrMem m;
+    m.m[0] = v;
+    m.m[12] = 10.0;
+    vfactors(&m, iso2);
+    int no = 0;
+    {
+        RYR_STRICT_FP
+        const double cpx = (cp + cs * (0.5)) / (1 + (0.5));
+        const double betac = iso2 ? 2.0 : 4.0;
+        for (int ll = 0; ll < 8 && lcc[8 * q + ll] < 16; ll++) {
+            int jj = lcc_step(lcc[8 * q + ll], cpx, dt, ryr_stream_uniform(&rs), m, betac);
+            lcc[8 * q + ll] = jj;
+            if (jj == 0) ++no;
+        }
+    }
+    int nou = counts[4 * q], ncu = counts[4 * q + 1], nob = counts[4 * q + 2], ncb = counts[4 * q + 3];
+    int f = 0;
+    ryr_gating(&rs, Ku, Ku2, Kb, cp, cjsr, dt, nou, ncu, nob, ncb, &f);
+    counts[4 * q] = nou; counts[4 * q + 1] = ncu; counts[4 * q + 2] = nob; counts[4 * q + 3] = ncb;
+    draws[q] = (int32_t)rs.slot;
+    nopen[q] = no;
+    fails[q] = f;
+}
+#endif
+
+#endif /* RYR_CRU_DEVICE_H */

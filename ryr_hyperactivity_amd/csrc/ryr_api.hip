@@ -1,0 +1,661 @@
+/*
+ * ryr_api.hip - host side of the C ABI declared in include/ryr_cru.h.
+ *
+ * Replaces the reference's main() plumbing around Compute
+ * (cuda_Cell_Detailed.cu:232-1105): set-up and ICs, the per-step launch,
+ * the output-interval copies, teardown.  The per-step host work of the
+ * reference (D2H of 516 KB, serial sums, membrane ODE) moved on device, so a
+ * step here is one kernel launch; every `spo`-step interval (one output row)
+ * is captured once as a HIP graph and replayed.
+ */
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "ryr_cru.h"
+#define RYR_DEFINE_COMMON_KERNELS
+#include "cru_launch.h"
+
+struct ryr_handle {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    ryr_params p{};
+    std::vector<ryr_cell_desc> desc;
+    std::vector<RyrCellInfo> info;
+    std::vector<long long> line_off;
+    int ncells = 0, nblocks = 0;
+    long long P = 0;
+    long long step = 0;
+    RyrKArgs a{};
+    std::vector<std::pair<void *, size_t>> allocs;   /* every device allocation, in save order */
+    double *stage = nullptr;
+    size_t stage_n = 0;
+    hipGraphExec_t graph = nullptr;
+    bool graph_failed = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timing_open = false;
+    long long launches = 0;
+    std::string err;
+};
+
+static std::string g_create_err;
+
+#define RYR_HIPCHECK(h, call)                                                       \
+    do {                                                                            \
+        hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess) {                                                     \
+            (h)->err = std::string(#call) + ": " + hipGetErrorString(e_);           \
+            return RYR_E_HIP;                                                       \
+        }                                                                           \
+    } while (0)
+
+extern "C" {
+
+int ryr_abi_version(void) { return RYR_ABI_VERSION; }
+
+void ryr_params_default(ryr_params *p)
+{
+    std::memset(p, 0, sizeof *p);
+    p->dt = 0.025;
+    p->scheme = RYR_SCHEME_A;
+    p->precision = RYR_PREC_F64;
+    p->iso2 = 1;
+    p->tapsi = 0;
+    p->lqt = 1;
+    p->nlcc = 4;
+    p->steps_per_output = 100;
+    p->stim_amp = 80.0;
+    p->stim_dur = 1.0;
+    p->stim_offset = 100.0;
+    p->ci0 = 0.0944;
+    p->v0 = -80.0;
+    p->cm_mode = 0;
+    p->cm = 45.0;
+}
+
+void ryr_cell_desc_default(ryr_cell_desc *d)
+{
+    std::memset(d, 0, sizeof *d);
+    d->nx = 62; d->ny = 26; d->nz = 10;
+    d->seed = 1;
+    d->cl = 4000.0;
+    d->cjsr0 = 740.0;
+    d->ncx_scale = 1.0;
+    d->Ku = 10.0;
+    d->Kb = 1.0;
+    d->ica_scale = 0.7;
+    d->Ku2 = 1.0;
+    d->ikr_scale = 1.0;
+}
+
+const char *ryr_last_error(const ryr_handle *h)
+{
+    return h ? h->err.c_str() : g_create_err.c_str();
+}
+
+}  /* extern "C" */
+
+/* --- host restatement of Init's closed forms (cuda_Cell_Detailed.cu:1118-1174,
+ * subroutine_cuda.c:179-195); contraction off in this unit */
+static double host_luminal_SS(double cjsr)
+{
+    double roo2 = 5000.0 / (1.0 + ryr_pow(850.0 / cjsr, 23.0));
+    double mono = (-1.0 + std::sqrt(1.0 + 8.0 * 460.0 * roo2)) / (4.0 * roo2 * 460.0);
+    double ene = mono * 20 + (1.0 - mono) * 30;
+    return 460.0 * ene * cjsr / (600.0 + cjsr);
+}
+static double host_fracbound(double cjsr_b)
+{
+    double roo2 = 5000.0 / (1.0 + ryr_pow(850.0 / (cjsr_b), 23.0));
+    double aub = (-1.0 + std::sqrt(1.0 + 8.0 * 460.0 * roo2)) / (4.0 * roo2 * 460.0) / (1.0);
+    double abu = 1.0 / (0.2 * 5 * 220.);
+    return 1 / (1 + abu / aub);
+}
+
+static int dev_alloc(ryr_handle *h, void **ptr, size_t bytes)
+{
+    RYR_HIPCHECK(h, hipMalloc(ptr, bytes > 0 ? bytes : 16));
+    RYR_HIPCHECK(h, hipMemset(*ptr, 0, bytes > 0 ? bytes : 16));
+    h->allocs.push_back({*ptr, bytes});
+    return RYR_OK;
+}
+
+static int validate(const ryr_params *p, const ryr_cell_desc *cells, int ncells, std::string &msg)
+{
+    if (!p || !cells || ncells <= 0) { msg = "null parameters or ncells <= 0"; return RYR_E_ARG; }
+    if (!(p->dt > 0.0) || p->dt > 1.0) { msg = "dt must be in (0, 1] ms"; return RYR_E_ARG; }
+    if (p->scheme != RYR_SCHEME_A && p->scheme != RYR_SCHEME_B) { msg = "scheme must be A (0) or B (1)"; return RYR_E_ARG; }
+    if (p->precision != RYR_PREC_F64 && p->precision != RYR_PREC_F32) { msg = "precision must be 0 or 1"; return RYR_E_ARG; }
+    if (p->nlcc < 0 || p->nlcc > 8) { msg = "nlcc must be in [0, 8]"; return RYR_E_ARG; }
+    if (p->steps_per_output < 1) { msg = "steps_per_output must be >= 1"; return RYR_E_ARG; }
+    if (!(p->cm > 0.0)) { msg = "cm must be > 0"; return RYR_E_ARG; }
+    for (int c = 0; c < ncells; ++c) {
+        const ryr_cell_desc &d = cells[c];
+        if (d.nx < 1 || d.ny < 1 || d.nz < 1) { msg = "cell " + std::to_string(c) + ": lattice dims must be >= 1"; return RYR_E_ARG; }
+        long long N = (long long)d.nx * d.ny * d.nz;
+        if (N > (1LL << 30)) { msg = "cell " + std::to_string(c) + ": lattice too large"; return RYR_E_ARG; }
+        if (!(d.cl > 0.0)) { msg = "cell " + std::to_string(c) + ": cl must be > 0"; return RYR_E_ARG; }
+        if (!(d.cjsr0 > 0.0) || !(d.Ku > 0.0)) { msg = "cell " + std::to_string(c) + ": cjsr0 and Ku must be > 0"; return RYR_E_ARG; }
+        double r = d.cl / p->dt;
+        if (std::fabs(r - std::llround(r)) > 1e-6 * r) { msg = "cell " + std::to_string(c) + ": cl must be a multiple of dt"; return RYR_E_ARG; }
+    }
+    return RYR_OK;
+}
+
+extern "C" int ryr_create(int device, const ryr_params *p, const ryr_cell_desc *cells, int ncells,
+                          ryr_handle **out)
+{
+    if (!out) { g_create_err = "out is null"; return RYR_E_ARG; }
+    *out = nullptr;
+    std::string msg;
+    int rc = validate(p, cells, ncells, msg);
+    if (rc) { g_create_err = msg; return rc; }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        g_create_err = "no HIP device available";
+        return RYR_E_NODEV;
+    }
+    if (device < 0 || device >= ndev) { g_create_err = "device index out of range"; return RYR_E_ARG; }
+
+    ryr_handle *h = new ryr_handle();
+    h->device = device;
+    h->p = *p;
+    h->desc.assign(cells, cells + ncells);
+    h->ncells = ncells;
+    auto fail = [&](int code) { g_create_err = h->err; ryr_destroy(h); return code; };
+    if (hipSetDevice(device) != hipSuccess) { h->err = "hipSetDevice failed"; return fail(RYR_E_HIP); }
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        h->err = "hipStreamCreate failed";
+        return fail(RYR_E_HIP);
+    }
+    (void)hipEventCreate(&h->ev0);
+    (void)hipEventCreate(&h->ev1);
+
+    /* layout: each cell padded to whole workgroups */
+    std::vector<int> block_cell;
+    long long lo = 0;
+    h->info.resize(ncells);
+    h->line_off.resize(ncells);
+    for (int c = 0; c < ncells; ++c) {
+        const ryr_cell_desc &d = cells[c];
+        RyrCellInfo &I = h->info[c];
+        std::memset(&I, 0, sizeof I);
+        I.X = d.nx; I.Y = d.ny; I.Z = d.nz;
+        I.N = d.nx * d.ny * d.nz;
+        I.nblocks = (I.N + RYR_BLOCK - 1) / RYR_BLOCK;
+        I.first_block = (int)block_cell.size();
+        I.off = (long long)I.first_block * RYR_BLOCK;
+        for (int b = 0; b < I.nblocks; ++b) block_cell.push_back(c);
+        I.seed = d.seed;
+        I.Ku = d.Ku; I.Ku2 = d.Ku2; I.Kb = d.Kb;
+        I.svica = ((0.666) * (1.2) * (d.ica_scale));
+        I.svncx = (2.5 * d.ncx_scale);
+        I.svikr = p->lqt ? 0.0 : 3.0 * d.ikr_scale;
+        double sviup = (1.0 + 0.75);
+        if (p->iso2) sviup = (1.0 + 0.75) * 1.75;
+        if (p->tapsi) sviup = 0;
+        double sviup_lqt = (p->lqt && p->tapsi) ? 0.05 : 0.;
+        I.sviup_tot = sviup + sviup_lqt;
+        I.Cm = (p->cm_mode == 0) ? p->cm * (double)I.N / 16120.0 : p->cm;
+        I.dN = (double)I.N;
+        I.ncl = (long long)std::llround(d.cl / p->dt);
+        I.noff = (long long)std::llround(p->stim_offset / p->dt);
+        I.ndur = (long long)std::llround(p->stim_dur / p->dt);
+        I.nlcc = p->nlcc;
+        int jy = (d.ny + 2) / 2;            /* ny/2 with the ghost shell (:973) */
+        int jz = 5 < d.nz ? 5 : d.nz;        /* :972, clamped to the interior */
+        I.line_j = jy - 1;
+        I.line_k = jz - 1;
+        h->line_off[c] = lo;
+        lo += (long long)d.nx * RYR_LINE_COLS;
+    }
+    h->nblocks = (int)block_cell.size();
+    h->P = (long long)h->nblocks * RYR_BLOCK;
+    const long long P = h->P;
+    const bool f32 = p->precision == RYR_PREC_F32;
+    const size_t tf = f32 ? 4 : 8, ts = 8;
+
+    RyrKArgs &a = h->a;
+    a.P = P;
+    a.dt = p->dt;
+    a.scheme = p->scheme;
+    a.iso2 = p->iso2;
+    a.spo = p->steps_per_output;
+    a.ncells = ncells;
+#define ALLOC(ptr, bytes) do { void *q_ = nullptr; if (dev_alloc(h, &q_, (bytes))) return fail(RYR_E_HIP); (ptr) = decltype(ptr)(q_); } while (0)
+    for (int b = 0; b < 2; ++b) {
+        ALLOC(a.ci[b], 8 * P * tf);
+        ALLOC(a.cnsr[b], 8 * P * ts);
+        ALLOC(a.cs[b], P * tf);
+    }
+    ALLOC(a.catf, 8 * P * tf); ALLOC(a.cats, 8 * P * ts); ALLOC(a.cacal, 8 * P * tf);
+    ALLOC(a.casr, 8 * P * tf); ALLOC(a.camyo, 8 * P * tf); ALLOC(a.mgmyo, 8 * P * tf);
+    ALLOC(a.cp, P * tf); ALLOC(a.cjsr, P * ts); ALLOC(a.cjsrb, P * ts); ALLOC(a.Ancx, P * tf);
+    ALLOC(a.casar, P * tf); ALLOC(a.casarh, P * tf); ALLOC(a.casarj, P * tf); ALLOC(a.casarhj, P * tf);
+    ALLOC(a.lccw, P * 4); ALLOC(a.ryrw, P * 4); ALLOC(a.aux, P * 4);
+    ALLOC(a.d_xire, P * 8); ALLOC(a.d_xinaca, P * 8); ALLOC(a.d_up, P * 8); ALLOC(a.d_leak, P * 8);
+    RyrCellInfo *d_cells; RyrMem *d_mem; int *d_bc; RyrDD *d_part; unsigned int *d_arr;
+    unsigned long long *d_cnt; double *d_diag; long long *d_dstep; double *d_line; long long *d_loff;
+    ALLOC(d_mem, sizeof(RyrMem) * ncells);
+    ALLOC(d_cnt, sizeof(unsigned long long) * ncells * RYR_NCNT);
+    ALLOC(d_diag, sizeof(double) * ncells * RYR_DIAG_COLS);
+    ALLOC(d_dstep, sizeof(long long) * ncells);
+    ALLOC(d_line, sizeof(double) * (lo > 0 ? lo : 1));
+    ALLOC(d_cells, sizeof(RyrCellInfo) * ncells);
+    ALLOC(d_bc, sizeof(int) * h->nblocks);
+    ALLOC(d_part, sizeof(RyrDD) * 3 * h->nblocks);
+    ALLOC(d_arr, sizeof(unsigned int) * ncells);
+    ALLOC(d_loff, sizeof(long long) * ncells);
+#undef ALLOC
+    a.cells = d_cells; a.mem = d_mem; a.block_cell = d_bc; a.partial = d_part; a.arrive = d_arr;
+    a.counters = d_cnt; a.diag = d_diag; a.diag_step = d_dstep; a.line = d_line; a.line_off = d_loff;
+
+    auto H2D = [&](void *dst, const void *src, size_t n) {
+        return hipMemcpy(dst, src, n, hipMemcpyHostToDevice) == hipSuccess;
+    };
+    if (!H2D(d_cells, h->info.data(), sizeof(RyrCellInfo) * ncells) ||
+        !H2D(d_bc, block_cell.data(), sizeof(int) * h->nblocks) ||
+        !H2D(d_loff, h->line_off.data(), sizeof(long long) * ncells)) {
+        h->err = "hipMemcpy of cell tables failed";
+        return fail(RYR_E_HIP);
+    }
+    if (hipMemset(d_dstep, 0xff, sizeof(long long) * ncells) != hipSuccess) {
+        h->err = "hipMemset failed";
+        return fail(RYR_E_HIP);
+    }
+
+    /* initial values (cuda_Cell_Detailed.cu:1108-1182, :350-366) */
+    std::vector<RyrInit> init(ncells);
+    std::vector<RyrMem> mem(ncells);
+    for (int c = 0; c < ncells; ++c) {
+        const double ci_basal = p->ci0;
+        const double cjsr_b = p->tapsi ? 10 : cells[c].cjsr0;
+        RyrInit &I = init[c];
+        std::memset(&I, 0, sizeof I);
+        I.ci0 = ci_basal;
+        I.cjsr0 = cjsr_b;
+        I.cjsrb0 = host_luminal_SS(cjsr_b);
+        I.catf0 = 0.0327 * ci_basal * 70. / (0.0327 * ci_basal + 0.0196);
+        I.cats0 = 0.00254 * ci_basal * ((134.)) / (0.00254 * ci_basal + 0.000033);
+        I.cacal0 = 0.0543 * ci_basal * 24. / (0.0543 * ci_basal + 0.238);
+        I.casr0 = 0.1 * ci_basal * 19. / (0.1 * ci_basal + 0.06);
+        I.camyo0 = 3.;
+        I.mgmyo0 = 136.209996;
+        I.casar0 = 0.1 * ci_basal * RYR_BSAR / (0.1 * ci_basal + 1.3);
+        I.casarh0 = 0.1 * ci_basal * RYR_BSARH / (0.1 * ci_basal + 0.03);
+        I.ancx0 = 0.025;
+        double fb = host_fracbound(cjsr_b);
+        I.ncb0 = (int)(fb * RYR_NRYR);
+        I.ncu0 = RYR_NRYR - (int)(fb * RYR_NRYR);
+        uint32_t lw = 0;
+        for (int ll = 0; ll < 8; ++ll) lw |= (uint32_t)(ll < p->nlcc ? 3 : 0) << (4 * ll);
+        I.lcc0 = lw;
+        RyrMem &M = mem[c];
+        std::memset(&M, 0, sizeof M);
+        M.m[0] = p->v0; M.m[1] = 0.0010; M.m[2] = 1.00; M.m[3] = 1.00; M.m[4] = 0.00;
+        M.m[5] = 0.08433669901; M.m[6] = M.m[5]; M.m[7] = 0.20; M.m[8] = 0.010; M.m[9] = 1.00;
+        M.m[10] = 0.020; M.m[11] = 0.80;
+        M.m[12] = 6.1 * ((9000) / (cells[c].cl + 5000));
+        M.step = 0;
+    }
+    RyrInit *d_init = nullptr;
+    if (hipMalloc((void **)&d_init, sizeof(RyrInit) * ncells) != hipSuccess ||
+        !H2D(d_init, init.data(), sizeof(RyrInit) * ncells) ||
+        !H2D(d_mem, mem.data(), sizeof(RyrMem) * ncells)) {
+        if (d_init) hipFree(d_init);
+        h->err = "upload of initial values failed";
+        return fail(RYR_E_HIP);
+    }
+    hipError_t e = f32 ? ryr_launch_init_f32(a, d_init, h->nblocks, h->stream)
+                       : ryr_launch_init_f64(a, d_init, h->nblocks, h->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_vfactors, dim3((ncells + 63) / 64), dim3(64), 0, h->stream, d_mem, ncells, p->iso2);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    hipFree(d_init);
+    if (e != hipSuccess) {
+        h->err = std::string("initialisation kernels failed: ") + hipGetErrorString(e);
+        return fail(RYR_E_HIP);
+    }
+    *out = h;
+    return RYR_OK;
+}
+
+static hipError_t launch_one(ryr_handle *h, long long s)
+{
+    const bool f32 = h->p.precision == RYR_PREC_F32;
+    hipError_t e = f32 ? ryr_launch_step_f32(h->a, h->p.stim_amp, h->nblocks, h->stream)
+                       : ryr_launch_step_f64(h->a, h->p.stim_amp, h->nblocks, h->stream);
+    if (e != hipSuccess) return e;
+    if (s % h->p.steps_per_output == 0)
+        e = f32 ? ryr_launch_diag_f32(h->a, h->ncells, h->stream) : ryr_launch_diag_f64(h->a, h->ncells, h->stream);
+    return e;
+}
+
+/* one output interval (spo steps starting at an output step) as a graph */
+static bool build_graph(ryr_handle *h)
+{
+    if (h->graph) return true;
+    if (h->graph_failed) return false;
+    hipGraph_t g = nullptr;
+    const long long spo = h->p.steps_per_output;
+    bool ok = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (ok) {
+        for (long long s = 0; s < spo && ok; ++s) ok = launch_one(h, s) == hipSuccess;
+        hipError_t e = hipStreamEndCapture(h->stream, &g);
+        ok = ok && e == hipSuccess && g;
+    }
+    if (ok) ok = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0) == hipSuccess;
+    if (g) hipGraphDestroy(g);
+    if (!ok) {
+        h->graph = nullptr;
+        h->graph_failed = true;
+        (void)hipGetLastError();
+    }
+    return ok;
+}
+
+extern "C" int ryr_step(ryr_handle *h, int64_t nsteps)
+{
+    if (!h || nsteps < 0) return RYR_E_ARG;
+    RYR_HIPCHECK(h, hipSetDevice(h->device));
+    if (nsteps == 0) return RYR_OK;
+    if (!h->timing_open) {
+        RYR_HIPCHECK(h, hipEventRecord(h->ev0, h->stream));
+        h->timing_open = true;
+        h->launches = 0;
+    }
+    const long long spo = h->p.steps_per_output;
+    while (nsteps > 0) {
+        if (h->step % spo == 0 && nsteps >= spo && spo >= 4 && build_graph(h)) {
+            RYR_HIPCHECK(h, hipGraphLaunch(h->graph, h->stream));
+            h->step += spo;
+            h->launches += spo;
+            nsteps -= spo;
+            continue;
+        }
+        RYR_HIPCHECK(h, launch_one(h, h->step));
+        h->step += 1;
+        h->launches += 1;
+        nsteps -= 1;
+    }
+    RYR_HIPCHECK(h, hipEventRecord(h->ev1, h->stream));
+    return RYR_OK;
+}
+
+extern "C" int ryr_sync(ryr_handle *h)
+{
+    if (!h) return RYR_E_ARG;
+    RYR_HIPCHECK(h, hipSetDevice(h->device));
+    RYR_HIPCHECK(h, hipStreamSynchronize(h->stream));
+    return RYR_OK;
+}
+
+extern "C" int ryr_kernel_time(ryr_handle *h, double *ms, int64_t *launches)
+{
+    if (!h || !ms || !launches) return RYR_E_ARG;
+    RYR_HIPCHECK(h, hipSetDevice(h->device));
+    *ms = 0.0;
+    *launches = 0;
+    if (!h->timing_open) return RYR_OK;
+    RYR_HIPCHECK(h, hipEventSynchronize(h->ev1));
+    float f = 0.0f;
+    RYR_HIPCHECK(h, hipEventElapsedTime(&f, h->ev0, h->ev1));
+    *ms = (double)f;
+    *launches = h->launches;
+    h->timing_open = false;
+    return RYR_OK;
+}
+
+extern "C" int ryr_read_scalars(ryr_handle *h, double *out)
+{
+    if (!h || !out) return RYR_E_ARG;
+    RYR_HIPCHECK(h, hipSetDevice(h->device));
+    RYR_HIPCHECK(h, hipStreamSynchronize(h->stream));
+    std::vector<RyrMem> mem(h->ncells);
+    RYR_HIPCHECK(h, hipMemcpy(mem.data(), h->a.mem, sizeof(RyrMem) * h->ncells, hipMemcpyDeviceToHost));
+    for (int c = 0; c < h->ncells; ++c) {
+        const RyrMem &M = mem[c];
+        double *o = out + (long long)c * RYR_S_COUNT;
+        o[RYR_S_T] = (double)(M.step > 0 ? M.step - 1 : 0) * h->p.dt;
+        for (int q = 0; q < 13; ++q) o[RYR_S_V + q] = M.m[q];
+        o[RYR_S_CSUBT] = M.cur[0]; o[RYR_S_ICAL] = M.cur[1]; o[RYR_S_INCX] = M.cur[2];
+        o[RYR_S_INA] = M.cur[3]; o[RYR_S_IKS] = M.cur[4]; o[RYR_S_IKR] = M.cur[5];
+        o[RYR_S_IK1] = M.cur[6]; o[RYR_S_INAK] = M.cur[7]; o[RYR_S_ITOS] = M.cur[8];
+        o[RYR_S_ITOF] = M.cur[9]; o[RYR_S_STIM] = M.stim; o[RYR_S_OUT_ICA] = M.out_ica;
+        o[RYR_S_OUT_NCX] = M.out_ncx; o[RYR_S_OUT_INA] = M.out_ina; o[RYR_S_STEP] = (double)M.step;
+    }
+    return RYR_OK;
+}
+
+static int field_planes(int f)
+{
+    if (f < 0 || f >= RYR_F_COUNT) return 0;
+    return (f <= RYR_F_MGMYO || f == RYR_F_LCC) ? 8 : 1;
+}
+
+static int ensure_stage(ryr_handle *h, size_t n)
+{
+    if (h->stage_n >= n) return RYR_OK;
+    if (h->stage) hipFree(h->stage);
+    h->stage = nullptr;
+    h->stage_n = 0;
+    RYR_HIPCHECK(h, hipMalloc((void **)&h->stage, n * sizeof(double)));
+    h->stage_n = n;
+    return RYR_OK;
+}
+
+extern "C" int ryr_read_field(ryr_handle *h, int cell, int field, double *out, int64_t n)
+{
+    if (!h || !out || cell < 0 || cell >= h->ncells) return RYR_E_ARG;
+    const int planes = field_planes(field);
+    if (!planes) { h->err = "unknown field"; return RYR_E_ARG; }
+    const long long nel = (long long)planes * h->info[cell].N;
+    if (n < nel) { h->err = "output buffer too small"; return RYR_E_ARG; }
+    RYR_HIPCHECK(h, hipSetDevice(h->device));
+    if (int rc = ensure_stage(h, (size_t)nel)) return rc;
+    const bool f32 = h->p.precision == RYR_PREC_F32;
+    RYR_HIPCHECK(h, f32 ? ryr_launch_export_f32(h->a, cell, field, planes, nel, h->stage, h->stream)
+                        : ryr_launch_export_f64(h->a, cell, field, planes, nel, h->stage, h->stream));
+    RYR_HIPCHECK(h, hipStreamSynchronize(h->stream));
+    RYR_HIPCHECK(h, hipMemcpy(out, h->stage, nel * sizeof(double), hipMemcpyDeviceToHost));
+    return RYR_OK;
+}
+
+extern "C" int ryr_write_field(ryr_handle *h, int cell, int field, const double *in, int64_t n)
+{
+    if (!h || !in || cell < 0 || cell >= h->ncells) return RYR_E_ARG;
+    const int planes = field_planes(field);
+    if (!planes) { h->err = "unknown field"; return RYR_E_ARG; }
+    const long long nel = (long long)planes * h->info[cell].N;
+    if (n < nel) { h->err = "input buffer too small"; return RYR_E_ARG; }
+    RYR_HIPCHECK(h, hipSetDevice(h->device));
+    if (int rc = ensure_stage(h, (size_t)nel)) return rc;
+    RYR_HIPCHECK(h, hipStreamSynchronize(h->stream));
+    RYR_HIPCHECK(h, hipMemcpy(h->stage, in, nel * sizeof(double), hipMemcpyHostToDevice));
+    const bool f32 = h->p.precision == RYR_PREC_F32;
+    RYR_HIPCHECK(h, f32 ? ryr_launch_import_f32(h->a, cell, field, planes, nel, h->stage, h->stream)
+                        : ryr_launch_import_f64(h->a, cell, field, planes, nel, h->stage, h->stream));
+    RYR_HIPCHECK(h, hipStreamSynchronize(h->stream));
+    return RYR_OK;
+}
+
+extern "C" int ryr_write_scalar(ryr_handle *h, int cell, int s, double value)
+{
+    if (!h || cell < 0 || cell >= h->ncells || s < RYR_S_V || s > RYR_S_XNAI) return RYR_E_ARG;
+    RYR_HIPCHECK(h, hipSetDevice(h->device));
+    RYR_HIPCHECK(h, hipStreamSynchronize(h->stream));
+    RyrMem M;
+    RYR_HIPCHECK(h, hipMemcpy(&M, h->a.mem + cell, sizeof M, hipMemcpyDeviceToHost));
+    M.m[s - RYR_S_V] = value;
+    RYR_HIPCHECK(h, hipMemcpy(h->a.mem + cell, &M, sizeof M, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_vfactors, dim3((h->ncells + 63) / 64), dim3(64), 0, h->stream, h->a.mem,
+                       h->ncells, h->p.iso2);
+    RYR_HIPCHECK(h, hipGetLastError());
+    RYR_HIPCHECK(h, hipStreamSynchronize(h->stream));
+    return RYR_OK;
+}
+
+extern "C" int ryr_read_diag(ryr_handle *h, double *out, int64_t *step_out)
+{
+    if (!h || !out) return RYR_E_ARG;
+    RYR_HIPCHECK(h, hipSetDevice(h->device));
+    RYR_HIPCHECK(h, hipStreamSynchronize(h->stream));
+    RYR_HIPCHECK(h, hipMemcpy(out, h->a.diag, sizeof(double) * h->ncells * RYR_DIAG_COLS, hipMemcpyDeviceToHost));
+    if (step_out) {
+        long long s = -1;
+        RYR_HIPCHECK(h, hipMemcpy(&s, h->a.diag_step, sizeof s, hipMemcpyDeviceToHost));
+        *step_out = s;
+    }
+    return RYR_OK;
+}
+
+extern "C" int ryr_read_linescan(ryr_handle *h, int cell, double *out, int64_t n)
+{
+    if (!h || !out || cell < 0 || cell >= h->ncells) return RYR_E_ARG;
+    const long long nel = (long long)h->info[cell].X * RYR_LINE_COLS;
+    if (n < nel) { h->err = "output buffer too small"; return RYR_E_ARG; }
+    RYR_HIPCHECK(h, hipSetDevice(h->device));
+    RYR_HIPCHECK(h, hipStreamSynchronize(h->stream));
+    RYR_HIPCHECK(h, hipMemcpy(out, h->a.line + h->line_off[cell], sizeof(double) * nel, hipMemcpyDeviceToHost));
+    return RYR_OK;
+}
+
+extern "C" int ryr_read_counters(ryr_handle *h, int64_t *out)
+{
+    if (!h || !out) return RYR_E_ARG;
+    RYR_HIPCHECK(h, hipSetDevice(h->device));
+    RYR_HIPCHECK(h, hipStreamSynchronize(h->stream));
+    std::vector<unsigned long long> v((size_t)h->ncells * RYR_NCNT);
+    RYR_HIPCHECK(h, hipMemcpy(v.data(), h->a.counters, sizeof(unsigned long long) * v.size(), hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < v.size(); ++q) out[q] = (int64_t)v[q];
+    return RYR_OK;
+}
+
+extern "C" int ryr_cru_steps(ryr_handle *h, int64_t *out)
+{
+    if (!h || !out) return RYR_E_ARG;
+    long long n = 0;
+    for (const auto &I : h->info) n += I.N;
+    *out = (int64_t)(n * h->step);
+    return RYR_OK;
+}
+
+/* snapshot: header + every device allocation in allocation order */
+struct RyrSnapHeader {
+    char magic[8];
+    int version, precision, ncells, pad_;
+    long long P, step, nallocs;
+};
+
+extern "C" int ryr_save(ryr_handle *h, const char *path)
+{
+    if (!h || !path) return RYR_E_ARG;
+    RYR_HIPCHECK(h, hipSetDevice(h->device));
+    RYR_HIPCHECK(h, hipStreamSynchronize(h->stream));
+    FILE *f = std::fopen(path, "wb");
+    if (!f) { h->err = std::string("cannot open ") + path; return RYR_E_ARG; }
+    RyrSnapHeader hd;
+    std::memset(&hd, 0, sizeof hd);
+    std::memcpy(hd.magic, "RYRSNAP", 8);
+    hd.version = RYR_ABI_VERSION; hd.precision = h->p.precision; hd.ncells = h->ncells;
+    hd.P = h->P; hd.step = h->step; hd.nallocs = (long long)h->allocs.size();
+    bool ok = std::fwrite(&hd, sizeof hd, 1, f) == 1;
+    std::vector<char> buf;
+    for (auto &al : h->allocs) {
+        if (!ok) break;
+        buf.resize(al.second);
+        ok = hipMemcpy(buf.data(), al.first, al.second, hipMemcpyDeviceToHost) == hipSuccess;
+        long long sz = (long long)al.second;
+        ok = ok && std::fwrite(&sz, sizeof sz, 1, f) == 1 && std::fwrite(buf.data(), 1, al.second, f) == al.second;
+    }
+    std::fclose(f);
+    if (!ok) { h->err = "snapshot write failed"; return RYR_E_HIP; }
+    return RYR_OK;
+}
+
+extern "C" int ryr_load(ryr_handle *h, const char *path)
+{
+    if (!h || !path) return RYR_E_ARG;
+    RYR_HIPCHECK(h, hipSetDevice(h->device));
+    RYR_HIPCHECK(h, hipStreamSynchronize(h->stream));
+    FILE *f = std::fopen(path, "rb");
+    if (!f) { h->err = std::string("cannot open ") + path; return RYR_E_ARG; }
+    RyrSnapHeader hd;
+    bool ok = std::fread(&hd, sizeof hd, 1, f) == 1 && std::memcmp(hd.magic, "RYRSNAP", 8) == 0 &&
+              hd.version == RYR_ABI_VERSION && hd.precision == h->p.precision && hd.ncells == h->ncells &&
+              hd.P == h->P && hd.nallocs == (long long)h->allocs.size();
+    std::vector<char> buf;
+    for (auto &al : h->allocs) {
+        if (!ok) break;
+        long long sz = 0;
+        ok = std::fread(&sz, sizeof sz, 1, f) == 1 && sz == (long long)al.second;
+        if (!ok) break;
+        buf.resize(al.second);
+        ok = std::fread(buf.data(), 1, al.second, f) == al.second &&
+             hipMemcpy(al.first, buf.data(), al.second, hipMemcpyHostToDevice) == hipSuccess;
+    }
+    std::fclose(f);
+    if (!ok) { h->err = "snapshot does not match this handle or is corrupt"; return RYR_E_STATE; }
+    h->step = hd.step;
+    return RYR_OK;
+}
+
+extern "C" void ryr_destroy(ryr_handle *h)
+{
+    if (!h) return;
+    hipSetDevice(h->device);
+    if (h->stream) hipStreamSynchronize(h->stream);
+    if (h->graph) hipGraphExecDestroy(h->graph);
+    for (auto &al : h->allocs) hipFree(al.first);
+    if (h->stage) hipFree(h->stage);
+    if (h->ev0) hipEventDestroy(h->ev0);
+    if (h->ev1) hipEventDestroy(h->ev1);
+    if (h->stream) hipStreamDestroy(h->stream);
+    delete h;
+}
+
+extern "C" int ryr_gating_probe(int device, int64_t n, uint64_t seed, const uint32_t *cru, uint64_t step,
+                                double Ku, double Ku2, double Kb, double dt, int iso2,
+                                int32_t *counts, int32_t *lcc, const double *conc,
+                                int32_t *draws, int32_t *nopen, int32_t *fails)
+{
+    if (n <= 0 || !cru || !counts || !lcc || !conc || !draws || !nopen || !fails || !(dt > 0)) {
+        g_create_err = "ryr_gating_probe: bad arguments";
+        return RYR_E_ARG;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) { g_create_err = "no HIP device"; return RYR_E_NODEV; }
+    if (hipSetDevice(device) != hipSuccess) { g_create_err = "hipSetDevice failed"; return RYR_E_HIP; }
+    uint32_t *d_cru = nullptr; int32_t *d_counts = nullptr, *d_lcc = nullptr, *d_draws = nullptr;
+    int32_t *d_nopen = nullptr, *d_fails = nullptr; double *d_conc = nullptr;
+    bool ok = hipMalloc((void **)&d_cru, n * 4) == hipSuccess &&
+              hipMalloc((void **)&d_counts, n * 16) == hipSuccess &&
+              hipMalloc((void **)&d_lcc, n * 32) == hipSuccess &&
+              hipMalloc((void **)&d_conc, n * 32) == hipSuccess &&
+              hipMalloc((void **)&d_draws, n * 4) == hipSuccess &&
+              hipMalloc((void **)&d_nopen, n * 4) == hipSuccess &&
+              hipMalloc((void **)&d_fails, n * 4) == hipSuccess;
+    ok = ok && hipMemcpy(d_cru, cru, n * 4, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(d_counts, counts, n * 16, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(d_lcc, lcc, n * 32, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(d_conc, conc, n * 32, hipMemcpyHostToDevice) == hipSuccess;
+    if (ok) {
+        hipLaunchKernelGGL(k_gating_probe, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, (long long)n,
+                           (unsigned long long)seed, d_cru, (unsigned long long)step, Ku, Ku2, Kb, dt, iso2,
+                           d_counts, d_lcc, d_conc, d_draws, d_nopen, d_fails);
+        ok = hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess;
+    }
+    ok = ok && hipMemcpy(counts, d_counts, n * 16, hipMemcpyDeviceToHost) == hipSuccess &&
+         hipMemcpy(lcc, d_lcc, n * 32, hipMemcpyDeviceToHost) == hipSuccess &&
+         hipMemcpy(draws, d_draws, n * 4, hipMemcpyDeviceToHost) == hipSuccess &&
+         hipMemcpy(nopen, d_nopen, n * 4, hipMemcpyDeviceToHost) == hipSuccess &&
+         hipMemcpy(fails, d_fails, n * 4, hipMemcpyDeviceToHost) == hipSuccess;
+    hipFree(d_cru); hipFree(d_counts); hipFree(d_lcc); hipFree(d_conc);
+    hipFree(d_draws); hipFree(d_nopen); hipFree(d_fails);
+    if (!ok) { g_create_err = "ryr_gating_probe: HIP error"; return RYR_E_HIP; }
+    return RYR_OK;
+}

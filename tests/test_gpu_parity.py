@@ -1,0 +1,165 @@
+"""GPU parity: the HIP CRU update against the FP64 oracle.
+
+FP64 mode is bit-exact by construction (same Philox draws, same detmath
+transcendentals, same operand order, contraction off, same canonical
+reduction order), so these tests compare with `==`, not a tolerance:
+state fields, membrane scalars, output rows, linescans, counters.
+"""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from ryr_hyperactivity_amd import CellDesc, CellEngine, Params, PREC_F32, SCHEME_A, SCHEME_B, FIELDS
+from ryr_hyperactivity_amd import format_wholetest_row, format_linescan, gating_probe
+from ryr_hyperactivity_amd._abi import S
+from oracle.binding import OracleCell
+from oracle import binding as orc
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _assert_same_state(eng, cell, o, label):
+    for f in FIELDS:
+        g = eng.field(cell, f)
+        r = o.field(f)
+        if not np.array_equal(g, r):
+            idx = np.argwhere(g != r)
+            raise AssertionError(f"{label}: field {f} differs at {len(idx)} places, first {idx[:3].tolist()}: "
+                                 f"gpu {g[tuple(idx[0])]!r} oracle {r[tuple(idx[0])]!r}")
+    gs = eng.scalars()[cell]
+    os_ = o.scalars()
+    assert np.array_equal(gs, os_), f"{label}: scalars differ: {np.nonzero(gs != os_)[0]}"
+
+
+@pytest.mark.parametrize("scheme", [SCHEME_A, SCHEME_B])
+@pytest.mark.parametrize("dims", [(7, 5, 3), (20, 13, 5)])
+def test_fp64_bitexact_through_an_action_potential(scheme, dims):
+    p = Params.default(scheme=scheme, stim_offset=2.0)
+    d = CellDesc.default(nx=dims[0], ny=dims[1], nz=dims[2], cl=1000.0, seed=11)
+    print(f"case scheme={scheme} dims={dims}", flush=True)
+    o = OracleCell(p, d)
+    with CellEngine(p, d) as eng:
+        for chunk in (1, 99, 100, 37, 163):          # eager steps and graph replays
+            eng.step(chunk)
+            o.step(chunk)
+            _assert_same_state(eng, 0, o, f"after +{chunk}")
+            grow, gst = eng.diag()
+            orow, ost = o.diag()
+            assert gst == ost
+            if gst >= 0:
+                assert np.array_equal(grow[0], orow), np.nonzero(grow[0] != orow)
+                assert format_wholetest_row(grow[0]) == format_wholetest_row(orow)
+                assert np.array_equal(eng.linescan(0), o.linescan())
+        assert np.array_equal(eng.counters()[0], o.counters())
+        # the action potential fired (stimulus at step 80)
+        assert eng.scalars()[0][S["V"]] > -60.0
+
+
+def test_gating_probe_bitexact_including_gaussian_branch():
+    rng = np.random.default_rng(3)
+    n = 4096
+    cru = rng.integers(0, 2**31, n, dtype=np.uint32)
+    counts = np.zeros((n, 4), np.int32)
+    nou = rng.integers(0, 60, n)
+    nob = rng.integers(0, 100 - nou + 1)
+    ncu = rng.integers(0, 100 - nou - nob + 1)
+    counts[:, 0], counts[:, 1], counts[:, 2] = nou, ncu, nob
+    counts[:, 3] = 100 - nou - nob - ncu
+    lcc = np.full((n, 8), 16, np.int32)
+    lcc[:, :4] = rng.integers(0, 16, (n, 4))
+    conc = np.stack([10 ** rng.uniform(-2, 2.5, n), rng.uniform(100, 1000, n),
+                     10 ** rng.uniform(-2, 1.5, n), rng.uniform(-90, 50, n)], axis=1)
+    for dt in (0.025, 0.4):   # 0.4 ms pushes p*dt past 0.2: the Gaussian branch
+        for iso2 in (0, 1):
+            print(f"case dt={dt} iso2={iso2}", flush=True)
+            g = gating_probe(5, cru, 12345, 3.333, 2.5, 18.0, dt, iso2, counts, lcc, conc)
+            r = orc.gating_probe(5, cru, 12345, 3.333, 2.5, 18.0, dt, iso2, counts, lcc, conc)
+            for a, b, name in zip(g, r, ["counts", "lcc", "draws", "nopen", "fails"]):
+                assert np.array_equal(a, b), name
+            if dt > 0.1:
+                assert (g[2] > 40).any()    # Gaussian draws consumed 3 slots per try
+
+
+def test_batch_equals_cells_run_alone():
+    p = Params.default(stim_offset=2.0)
+    cells = [CellDesc.default(nx=9, ny=4, nz=3, cl=1000.0, seed=1, Ku2=1.0),
+             CellDesc.default(nx=30, ny=10, nz=2, cl=2000.0, seed=2, Ku2=2.5, ikr_scale=0.3),
+             CellDesc.default(nx=5, ny=5, nz=5, cl=1000.0, seed=3, Ku=3.333, Kb=18.0)]
+    with CellEngine(p, cells) as batch:
+        batch.step(230)
+        bs = batch.scalars()
+        for c, d in enumerate(cells):
+            print(f"case cell {c}", flush=True)
+            with CellEngine(p, d) as one:
+                one.step(230)
+                assert np.array_equal(one.scalars()[0], bs[c])
+                for f in ("CI", "CS", "CJSR", "NOU", "LCC"):
+                    assert np.array_equal(one.field(0, f), batch.field(c, f))
+
+
+def test_fp32_production_tracks_fp64():
+    """FP32 concentrations vs FP64: same draws; deviations stay small over a
+    short window (per-output error of the precision pattern)."""
+    p64 = Params.default(stim_offset=1.0)
+    p32 = Params.default(stim_offset=1.0, precision=PREC_F32)
+    d = CellDesc.default(nx=16, ny=8, nz=4, cl=1000.0, seed=9)
+    with CellEngine(p64, d) as a, CellEngine(p32, d) as b:
+        a.step(101)
+        b.step(101)
+        ra, _ = a.diag()
+        rb, _ = b.diag()
+        for col in (1, 2, 5, 6, 7, 8, 21, 22, 41, 43, 44):
+            assert abs(ra[0, col] - rb[0, col]) <= 1e-4 * max(1.0, abs(ra[0, col])), col
+        for f in ("CI", "CNSR", "CP", "CJSR", "CATS"):
+            x, y = a.field(0, f), b.field(0, f)
+            assert np.allclose(x, y, rtol=2e-4, atol=1e-6), f
+
+
+def test_save_load_continues_bit_identically():
+    p = Params.default(stim_offset=2.0)
+    d = CellDesc.default(nx=10, ny=6, nz=3, cl=1000.0, seed=4)
+    with tempfile.TemporaryDirectory() as td, CellEngine(p, d) as eng:
+        path = os.path.join(td, "snap.bin")
+        eng.step(150)
+        eng.save(path)
+        eng.step(120)
+        ref = eng.scalars()[0].copy()
+        ref_ci = eng.field(0, "CI").copy()
+        eng.load(path)
+        eng.step(120)
+        assert np.array_equal(eng.scalars()[0], ref)
+        assert np.array_equal(eng.field(0, "CI"), ref_ci)
+
+
+def test_cli_writes_the_oracle_traces():
+    """bin/ryr_cell with the reference argv writes wholetest/linescan files
+    byte-identical to the oracle's rows formatted the same way."""
+    cli = os.path.join(ROOT, "ryr_hyperactivity_amd", "bin", "ryr_cell")
+    nsteps = 450
+    argv = ["0", "t1", "1.0", "740", "1.0", "3.333", "0.00", "18.", "0.7", "1",
+            "--lattice", "8x6x5", "--cl", "1000", "--nsteps", str(nsteps), "--stim-offset", "3"]
+    with tempfile.TemporaryDirectory() as td:
+        r = subprocess.run([cli] + argv + ["--outdir", td], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        whole = open(os.path.join(td, "wholetest_t1")).read()
+        line = open(os.path.join(td, "linescan_t1")).read()
+        assert os.path.exists(os.path.join(td, "cadist_t1"))
+    p = Params.default(stim_offset=3.0)
+    d = CellDesc.default(nx=8, ny=6, nz=5, cl=1000.0, Ku=3.333, Kb=18.0, seed=1)
+    o = OracleCell(p, d)
+    exp_w, exp_l = [], []
+    done = 0
+    while done < nsteps:
+        o.step(1)
+        row, st = o.diag()
+        if st == done:
+            exp_w.append(format_wholetest_row(row))
+            exp_l.append(format_linescan(o.linescan()))
+        done += 1
+    assert whole == "".join(exp_w)
+    assert line == "".join(exp_l)

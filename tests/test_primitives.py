@@ -1,0 +1,119 @@
+"""The two arithmetic primitives shared by the kernels and the oracle.
+
+Philox4x32-10 is pinned by the published known-answer vectors of its authors'
+Random123 distribution; the deterministic exp/log/cos/pow by glibc (through
+Python's math module) within a few ulp over the ranges the model uses, plus
+their special cases.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import binding as orc
+
+
+# Random123 known-answer vectors for philox4x32 with 10 rounds:
+# (counter[4], key[2]) -> output[4]
+KAT = [
+    ((0x00000000, 0x00000000, 0x00000000, 0x00000000), (0x00000000, 0x00000000),
+     (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+    ((0xffffffff, 0xffffffff, 0xffffffff, 0xffffffff), (0xffffffff, 0xffffffff),
+     (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+    ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+     (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1)),
+]
+
+
+@pytest.mark.parametrize("ctr,key,expect", KAT)
+def test_philox_known_answers(ctr, key, expect):
+    assert orc.philox(ctr, key) == list(expect)
+
+
+def test_uniforms_open_interval_and_extremes():
+    L = orc.lib()
+    assert L.orc_slot_uniform(0, 0) == 2.0 ** -53
+    assert L.orc_slot_uniform(0xFFFFFFFF, 0xFFFFFFFF) == 1.0 - 2.0 ** -53
+    assert L.orc_slot_uniform(0x80000000, 0) == 0.5 + 2.0 ** -53
+
+
+def test_stream_statistics_and_addressing():
+    u, w = orc.stream(1, 17, 99, 200000)
+    assert u.min() > 0 and u.max() < 1
+    assert abs(u.mean() - 0.5) < 4 * math.sqrt(1 / 12 / u.size)
+    assert abs(u.var() - 1 / 12) < 0.002
+    # integer draws are the low word of the slot: fair bits
+    assert abs((w & 1).mean() - 0.5) < 0.01
+    # different CRU / step / seed give unrelated streams; same key repeats
+    assert not np.array_equal(orc.stream(1, 18, 99, 16)[0], u[:16])
+    assert not np.array_equal(orc.stream(1, 17, 100, 16)[0], u[:16])
+    assert not np.array_equal(orc.stream(2, 17, 99, 16)[0], u[:16])
+    assert np.array_equal(orc.stream(1, 17, 99, 16)[0], u[:16])
+    # slot 2k and 2k+1 come from Philox block k of counter (k, cru, step)
+    blk = orc.philox((1, 17, 99, 0), (1, 0))
+    x = (blk[2] << 20) | (blk[3] >> 12)
+    assert u[3] == (x + 0.5) * 2.0 ** -52
+
+
+def _ulps(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    ia = a.view(np.int64).astype(object)
+    ib = b.view(np.int64).astype(object)
+    return max(abs(int(x) - int(y)) for x, y in zip(ia, ib))
+
+
+def test_detmath_exp_accuracy():
+    L = orc.lib()
+    xs = np.concatenate([np.linspace(-745, 709.7, 4001), np.random.default_rng(0).uniform(-50, 50, 20000),
+                         np.linspace(-1e-3, 1e-3, 1001)])
+    got = np.array([L.orc_dm_exp(x) for x in xs])
+    ref = np.array([math.exp(x) for x in xs])
+    normal = ref > 2.3e-308
+    assert _ulps(got[normal], ref[normal]) <= 2
+    assert L.orc_dm_exp(0.0) == 1.0
+    assert L.orc_dm_exp(-800.0) == 0.0
+    assert math.isinf(L.orc_dm_exp(710.0))
+    assert math.isnan(L.orc_dm_exp(float("nan")))
+
+
+def test_detmath_log_accuracy():
+    L = orc.lib()
+    rng = np.random.default_rng(1)
+    xs = np.concatenate([10.0 ** rng.uniform(-300, 300, 20000), rng.uniform(0.5, 2.0, 20000),
+                         [5e-324, 1e-310, 1.0, 2.0, 0.5]])
+    got = np.array([L.orc_dm_log(x) for x in xs])
+    ref = np.array([math.log(x) for x in xs])
+    far = np.abs(ref) > 1e-3
+    assert _ulps(got[far], ref[far]) <= 2
+    near = ~far
+    assert np.all(np.abs(got[near] - ref[near]) <= 4e-16 * np.maximum(np.abs(ref[near]), 1e-300) + 1e-300)
+    assert L.orc_dm_log(1.0) == 0.0
+    assert math.isinf(L.orc_dm_log(0.0)) and L.orc_dm_log(0.0) < 0
+    assert math.isnan(L.orc_dm_log(-1.0))
+
+
+def test_detmath_cos_accuracy():
+    L = orc.lib()
+    xs = np.concatenate([np.linspace(0, 2 * 3.1415926, 20001), np.random.default_rng(2).uniform(-20, 20, 5000)])
+    got = np.array([L.orc_dm_cos(x) for x in xs])
+    ref = np.array([math.cos(x) for x in xs])
+    assert np.max(np.abs(got - ref)) <= 2.5e-16
+
+
+def test_detmath_pow():
+    L = orc.lib()
+    rng = np.random.default_rng(3)
+    # integral exponents by binary powering: pow(x,2) == x*x, pow(x,3) == x*(x*x), exactly
+    for x in rng.uniform(0.01, 100, 1000):
+        assert L.orc_dm_pow(x, 2.0) == x * x
+        assert L.orc_dm_pow(x, 3.0) == x * (x * x)
+        assert L.orc_dm_pow(x, 1.0) == x
+        assert abs(L.orc_dm_pow(x, 23.0) / x ** 23 - 1) < 8 * 2.2e-16
+    # non-integral (SERCA Hill exponent 1.787): exp(y log x) loses |y log x|
+    # ulps of relative accuracy through the rounding of y*log(x)
+    for x in np.concatenate([rng.uniform(0, 50, 2000), 10 ** rng.uniform(-8, 0, 500)]):
+        ref = x ** 1.787
+        bound = (abs(1.787 * math.log(x)) + 3) * 2.3e-16 * ref if x > 0 else 0.0
+        assert abs(L.orc_dm_pow(x, 1.787) - ref) <= bound
+    assert L.orc_dm_pow(0.0, 1.787) == 0.0
