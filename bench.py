@@ -141,7 +141,7 @@ def main():
     batch = None
     if args.workload in ("batch", "both"):
         pb, bcells = make_batch(prec)
-        rb = run_gpu(pb, bcells, args.batch_steps, 100)
+        rb = run_gpu(pb, bcells, args.batch_steps, args.warmup if args.warmup is not None else 100)
         print(f"# batch: {rb}", file=sys.stderr, flush=True)
         batch = {"value": rb["cell_ms_per_s"], "unit": "simulated cell-ms/wall-s", "ncells": len(bcells),
                  "ncru": rb["ncru"], "steps": args.batch_steps, "ms_per_step": rb["ms_per_step"],

@@ -120,6 +120,29 @@ enum ryr_field {
 /* Linescan columns (cuda_Cell_Detailed.cu:974-989), one row per x. */
 #define RYR_LINE_COLS 22
 
+/* Per-beat summary rows (ryr_read_beats).  The reference records none of
+ * these (its only per-interval statistic is the spark flag, :1283-1284,
+ * :901); they are defined here:
+ *   onset     stimulus onset of the beat (ms)
+ *   apd90     time from onset to the first downward crossing of
+ *             Vmax - 0.9 (Vmax - Vrest), linearly interpolated (ms; -1 if none)
+ *   vmax      peak V of the beat; vrest = V at onset (mV)
+ *   ead       1 if, at least 50 ms after onset and once V has repolarised by
+ *             40% of the amplitude (past the notch and dome), V rises
+ *             again by >= 1 mV above its running minimum while above -40 mV
+ *             (dV/dt > 0 during repolarisation, with a 1 mV noise floor)
+ *   ead_t     time of that rise from onset (ms; -1 if none)
+ *   peak_ci   largest whole-cell mean ci over the beat's output rows (uM)
+ *   jsr_load  whole-cell mean cjsr at the output row of the onset (uM)
+ *   sparks    sum over the beat's output rows of the CRUs with a spark onset
+ *   complete  1 for a finished beat, 0 for the beat in progress */
+#define RYR_BEAT_COLS 10
+#define RYR_MAX_BEATS 64
+enum ryr_beat_col {
+    RYR_B_ONSET = 0, RYR_B_APD90, RYR_B_VMAX, RYR_B_VREST, RYR_B_EAD, RYR_B_EAD_T,
+    RYR_B_PEAK_CI, RYR_B_JSR_LOAD, RYR_B_SPARKS, RYR_B_COMPLETE
+};
+
 /* Soft-error counters per cell (ryr_read_counters). */
 enum ryr_counter {
     RYR_C_GAUSS_FALLBACK = 0, /* >200 rejections in a Gaussian draw (subroutine_cuda.c:358-361...) */
@@ -177,6 +200,19 @@ int ryr_read_diag(ryr_handle *h, double *out, int64_t *step_out);
  * RYR_LINE_COLS (:967-999), taken at y = ny/2, z = min(5, nz) in the
  * reference's ghost-inclusive numbering. */
 int ryr_read_linescan(ryr_handle *h, int cell, double *out, int64_t n);
+
+/* Beat summaries of one cell: up to maxbeats rows of RYR_BEAT_COLS, the
+ * completed beats in order followed by the beat in progress (if any);
+ * *nbeats gets the number of rows written.  Beats past RYR_MAX_BEATS are
+ * not recorded. */
+int ryr_read_beats(ryr_handle *h, int cell, double *out, int maxbeats, int *nbeats);
+
+/* Voltage-clamp protocols (cuda_Cell_Detailed.cu:733-773: vclamp,
+ * brownvclamp, apclamp reading one period from v_1000.txt, linearapclamp).
+ * After the membrane ODE of step s the cell's V is replaced by
+ * v[s % n] (periodic != 0) or v[min(s, n-1)]; the gates, currents and [Na]i
+ * still follow the ODE, as in the reference.  n = 0 switches the clamp off. */
+int ryr_set_vclamp(ryr_handle *h, int cell, const double *v, int64_t n, int periodic);
 
 /* Soft-error counters: out[c*RYR_C_COUNT + k] (see enum ryr_counter). */
 int ryr_read_counters(ryr_handle *h, int64_t *out);

@@ -45,6 +45,8 @@ def lib():
             "orc_read_linescan": (i, [vp, _D, i64]),
             "orc_read_counters": (i, [vp, ctypes.POINTER(i64)]),
             "orc_num_threads": (i, []),
+            "orc_set_vclamp": (i, [vp, _D, i64, i]),
+            "orc_read_beats": (i, [vp, _D, i, ctypes.POINTER(i)]),
             "orc_ncx": (d, [d, d, d, _D, d]),
             "orc_uptake": (d, [d, d]),
             "orc_leak": (d, [d, d]),
@@ -136,6 +138,16 @@ class OracleCell:
         out = np.zeros((self.cell.nx, LINE_COLS))
         self.L.orc_read_linescan(self.h, _dp(out), out.size)
         return out
+
+    def beats(self):
+        out = np.zeros((65, 10))
+        n = ctypes.c_int()
+        self.L.orc_read_beats(self.h, _dp(out), 65, ctypes.byref(n))
+        return out[:n.value]
+
+    def set_vclamp(self, v, periodic=False):
+        v = np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(-1))
+        self.L.orc_set_vclamp(self.h, _dp(v), v.size, int(bool(periodic)))
 
     def counters(self):
         out = np.zeros(C_COUNT, dtype=np.int64)

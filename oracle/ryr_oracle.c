@@ -535,6 +535,15 @@ struct orc_cell {
     int64_t diag_step;
     double *line;           /* X * RYR_LINE_COLS */
     double *scratch;        /* N reduction scratch */
+    /* voltage clamp waveform */
+    double *vclamp;
+    int64_t vclamp_n;
+    int vclamp_periodic;
+    /* beat analyser (definitions in include/ryr_cru.h, enum ryr_beat_col) */
+    int64_t beat_on;
+    int beat_n, beat_flags;
+    double b_vrest, b_vmax, b_vmin, b_apd90, b_ead_t, b_peak_ci, b_jsr, b_sparks;
+    double beats[RYR_MAX_BEATS][RYR_BEAT_COLS];
 };
 
 #define POS(c, i, j, k) ((int64_t)(i) + (int64_t)(c)->nx * ((int64_t)(j) + (int64_t)(c)->ny * (int64_t)(k)))
@@ -621,6 +630,7 @@ orc_cell *orc_create(const ryr_params *p, const ryr_cell_desc *d)
     c->xs1 = 0.08433669901; c->xs2 = c->xs1; c->qks = 0.20;
     c->xtos = 0.010; c->ytos = 1.00; c->xtof = 0.020; c->ytof = 0.80;
     c->diag_step = -1;
+    c->beat_on = -1;
     return c;
 }
 
@@ -631,7 +641,7 @@ void orc_destroy(orc_cell *c)
                     c->casar, c->casarh, c->casarj, c->casarhj, c->cs, c->csnext, c->lcc, c->nl,
                     c->nou, c->ncu, c->nob, c->ncb, c->nspark, c->ci, c->cinext, c->cnsr,
                     c->cnsrnext, c->catf, c->cats, c->cacal, c->casr, c->camyo, c->mgmyo,
-                    c->xiup, c->xileak, c->line, c->scratch};
+                    c->xiup, c->xileak, c->line, c->scratch, c->vclamp};
     for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) free(ptrs[i]);
     free(c);
 }
@@ -1034,6 +1044,81 @@ int orc_membrane_probe(double *state13, double sum_icat, double sum_inaca, doubl
     return 0;
 }
 
+/* ---- beat analyser (no reference counterpart; definitions in the header) */
+static void orc_beat_close(orc_cell *c, double complete)
+{
+    if (c->beat_n < RYR_MAX_BEATS) {
+        double *r = c->beats[c->beat_n];
+        r[0] = (double)c->beat_on; r[1] = c->b_apd90; r[2] = c->b_vmax; r[3] = c->b_vrest;
+        r[4] = (c->beat_flags & 2) ? 1.0 : 0.0; r[5] = c->b_ead_t; r[6] = c->b_peak_ci;
+        r[7] = c->b_jsr; r[8] = c->b_sparks; r[9] = complete;
+    }
+    c->beat_n++;
+}
+
+static void orc_beat_track(orc_cell *c, int64_t n, double vo, double vn)
+{
+    const double dt = c->p.dt;
+    int64_t ncl = (int64_t)llround(c->d.cl / dt), noff = (int64_t)llround(c->p.stim_offset / dt);
+    if (ncl > 0 && n >= noff && (n - noff) % ncl == 0) {
+        if (c->beat_on >= 0) orc_beat_close(c, 1.0);
+        c->beat_on = n;
+        c->b_vrest = vo; c->b_vmax = vo; c->b_vmin = vo;
+        c->b_apd90 = -1.0; c->b_ead_t = -1.0; c->beat_flags = 0;
+        c->b_peak_ci = 0.0; c->b_jsr = -1.0; c->b_sparks = 0.0;
+    }
+    if (c->beat_on < 0) return;
+    if (vn > c->b_vmax) c->b_vmax = vn;
+    double amp = c->b_vmax - c->b_vrest;
+    if (!(c->beat_flags & 1)) {
+        if (c->b_vmax > -40.0 && vn < c->b_vmax - 0.4 * amp &&
+            (double)(n + 1) * dt - (double)c->beat_on * dt >= 50.0) { c->beat_flags |= 1; c->b_vmin = vn; }
+    } else {
+        if (vn < c->b_vmin) c->b_vmin = vn;
+        if (!(c->beat_flags & 2) && vn > -40.0 && vn - c->b_vmin >= 1.0) {
+            c->beat_flags |= 2;
+            c->b_ead_t = (double)(n + 1) * dt - (double)c->beat_on * dt;
+        }
+    }
+    if (c->b_apd90 < 0.0 && c->b_vmax > -40.0) {
+        double thr = c->b_vmax - 0.9 * amp;
+        if (vo >= thr && vn < thr)
+            c->b_apd90 = ((double)n + (vo - thr) / (vo - vn)) * dt - (double)c->beat_on * dt;
+    }
+}
+
+int orc_set_vclamp(orc_cell *c, const double *v, int64_t n, int periodic)
+{
+    free(c->vclamp);
+    c->vclamp = NULL;
+    c->vclamp_n = 0;
+    c->vclamp_periodic = periodic ? 1 : 0;
+    if (n > 0) {
+        c->vclamp = (double *)malloc(sizeof(double) * (size_t)n);
+        memcpy(c->vclamp, v, sizeof(double) * (size_t)n);
+        c->vclamp_n = n;
+    }
+    return 0;
+}
+
+int orc_read_beats(const orc_cell *c, double *out, int maxbeats, int *nbeats)
+{
+    int done = c->beat_n < RYR_MAX_BEATS ? c->beat_n : RYR_MAX_BEATS, k = 0;
+    for (; k < done && k < maxbeats; ++k) {
+        memcpy(out + k * RYR_BEAT_COLS, c->beats[k], sizeof(double) * RYR_BEAT_COLS);
+        out[k * RYR_BEAT_COLS] *= c->p.dt;
+    }
+    if (c->beat_on >= 0 && k < maxbeats && c->beat_n < RYR_MAX_BEATS) {
+        double *r = out + k * RYR_BEAT_COLS;
+        r[0] = (double)c->beat_on * c->p.dt; r[1] = c->b_apd90; r[2] = c->b_vmax; r[3] = c->b_vrest;
+        r[4] = (c->beat_flags & 2) ? 1.0 : 0.0; r[5] = c->b_ead_t; r[6] = c->b_peak_ci; r[7] = c->b_jsr;
+        r[8] = c->b_sparks; r[9] = 0.0;
+        ++k;
+    }
+    *nbeats = k;
+    return 0;
+}
+
 /* stimulus of step n: t = n*dt inside [offset, offset+dur) modulo CL
  * (cuda_Cell_Detailed.cu:689-692 as an integer-step test) */
 static double orc_stim(const orc_cell *c, int64_t n)
@@ -1140,6 +1225,11 @@ static void orc_diag(orc_cell *c)
     r[37] = outAncx; r[38] = c->xs1; r[39] = c->qks; r[40] = (double)sparks;
     r[41] = catst; r[42] = csqnbt; r[43] = Total_Ci; r[44] = Total_Ca;
     c->diag_step = c->step;
+    if (c->beat_on >= 0) {                   /* Ca features of the beat */
+        if (c->step == c->beat_on) c->b_jsr = cjsrt;
+        else c->b_sparks += (double)sparks;
+        if (cit > c->b_peak_ci) c->b_peak_ci = cit;
+    }
 
     /* linescan (:967-999): jy = ny/2, jz = 5 in ghost-inclusive numbering,
      * clamped to the interior for lattices with fewer than 5 z-layers */
@@ -1210,7 +1300,10 @@ static void orc_one_step(orc_cell *c)
                      c->xtof, c->ytof, c->xnai};
     double cur[10];
     c->stim = orc_stim(c, n);
+    const double vold = c->v;
     orc_membrane(st, sum_icat, sum_inaca, sum_cs, (double)c->N, c->Cm, c->p.dt, c->svikr, c->stim, cur);
+    if (c->vclamp_n > 0)   /* clamp protocols (cuda_Cell_Detailed.cu:733-773) */
+        st[0] = c->vclamp[c->vclamp_periodic ? n % c->vclamp_n : (n < c->vclamp_n ? n : c->vclamp_n - 1)];
     c->v = st[0]; c->xm = st[1]; c->xh = st[2]; c->xj = st[3]; c->xr = st[4]; c->xs1 = st[5];
     c->xs2 = st[6]; c->qks = st[7]; c->xtos = st[8]; c->ytos = st[9]; c->xtof = st[10];
     c->ytof = st[11]; c->xnai = st[12];
@@ -1218,6 +1311,7 @@ static void orc_one_step(orc_cell *c)
     c->xikr = cur[5]; c->xik1 = cur[6]; c->xinak = cur[7]; c->xitos = cur[8]; c->xitof = cur[9];
     if (!(c->v == c->v) || c->v == __builtin_inf() || c->v == -__builtin_inf())
         c->counters[RYR_C_NONFINITE_V]++;
+    orc_beat_track(c, n, vold, c->v);
 
     /* interval accumulators (:774-776) */
     c->out_ica += c->xicatto;

@@ -34,6 +34,8 @@ int orc_read_diag(const orc_cell *c, double *out, int64_t *step);
 int orc_read_linescan(const orc_cell *c, double *out, int64_t n);
 int orc_read_counters(const orc_cell *c, int64_t *out);
 int orc_num_threads(void);
+int orc_set_vclamp(orc_cell *c, const double *v, int64_t n, int periodic);
+int orc_read_beats(const orc_cell *c, double *out, int maxbeats, int *nbeats);
 
 /* leaf functions (unit tests) */
 double orc_ncx(double cs_mM, double v, double xnai, double *Ancx, double dt);

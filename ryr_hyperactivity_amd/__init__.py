@@ -7,8 +7,9 @@ built library, importing the engine raises.
 """
 from ._abi import (CellDesc, Params, LibraryMissing, load_library, PREC_F32, PREC_F64, SCHEME_A,
                    SCHEME_B, DIAG_NAMES, FIELDS, SCALARS, COUNTERS)
-from .engine import CellEngine, EngineError, gating_probe, format_wholetest_row, format_linescan, run_cell
+from .engine import (CellEngine, EngineError, gating_probe, format_wholetest_row, format_linescan, run_cell,
+                     clamp_waveform)
 
 __all__ = ["CellDesc", "Params", "LibraryMissing", "load_library", "CellEngine", "EngineError",
-           "gating_probe", "format_wholetest_row", "format_linescan", "run_cell", "PREC_F32",
+           "gating_probe", "format_wholetest_row", "format_linescan", "run_cell", "clamp_waveform", "PREC_F32",
            "PREC_F64", "SCHEME_A", "SCHEME_B", "DIAG_NAMES", "FIELDS", "SCALARS", "COUNTERS"]

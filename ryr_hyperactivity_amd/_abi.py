@@ -39,6 +39,10 @@ F = {name: i for i, name in enumerate(FIELDS)}
 PLANES = {name: (8 if name in ("CI", "CNSR", "CATF", "CATS", "CACAL", "CASR", "CAMYO", "MGMYO", "LCC")
                  else 1) for name in FIELDS}
 
+BEAT_COLS = 10
+MAX_BEATS = 64
+BEAT_NAMES = ["onset", "apd90", "vmax", "vrest", "ead", "ead_t", "peak_ci", "jsr_load", "sparks", "complete"]
+
 COUNTERS = ["GAUSS_FALLBACK", "CLAMP_CP", "CLAMP_CS", "CI_RUNAWAY", "NONFINITE_V"]
 C_COUNT = len(COUNTERS)
 
@@ -131,6 +135,8 @@ def load_library(path: str | None = None):
         "ryr_load": (c_int, [vp, ctypes.c_char_p]),
         "ryr_destroy": (None, [vp]),
         "ryr_last_error": (ctypes.c_char_p, [vp]),
+        "ryr_read_beats": (c_int, [vp, c_int, P(c_double), c_int, P(c_int)]),
+        "ryr_set_vclamp": (c_int, [vp, c_int, P(c_double), c_i64, c_int]),
         "ryr_gating_probe": (c_int, [c_int, c_i64, c_u64, P(ctypes.c_uint32), c_u64, c_double, c_double,
                                      c_double, c_double, c_int, P(ctypes.c_int32), P(ctypes.c_int32),
                                      P(c_double), P(ctypes.c_int32), P(ctypes.c_int32),
@@ -152,4 +158,4 @@ EXPORTED = ["ryr_abi_version", "ryr_params_default", "ryr_cell_desc_default", "r
             "ryr_sync", "ryr_read_scalars", "ryr_read_field", "ryr_write_field", "ryr_write_scalar",
             "ryr_read_diag", "ryr_read_linescan", "ryr_read_counters", "ryr_cru_steps",
             "ryr_kernel_time", "ryr_save", "ryr_load", "ryr_destroy", "ryr_last_error",
-            "ryr_gating_probe"]
+            "ryr_read_beats", "ryr_set_vclamp", "ryr_gating_probe"]

@@ -46,7 +46,9 @@ struct RyrCellInfo {
     unsigned long long seed;
     double Ku, Ku2, Kb, svica, svncx, svikr, sviup_tot, Cm, dN;
     long long ncl, noff, ndur;
-    int nlcc, line_j, line_k, pad_;
+    int nlcc, line_j, line_k, vclamp_periodic;
+    const double *vclamp;   /* V protocol (device array) or null */
+    long long vclamp_n;
 };
 
 /* membrane state of a cell plus the V-only factors hoisted for its next step */
@@ -62,6 +64,10 @@ struct RyrMem {
     int ghk_small, pad_;
     /* ncx factors (subroutine_cuda.c:56-67) */
     double ncx_n1, ncx_n2, ncx_t1, ncx_t3a, ncx_den2;
+    /* beat tracker (definitions in the public header, enum ryr_beat_col) */
+    long long beat_on;      /* onset step of the beat in progress, -1 before the first */
+    int beat_n, beat_flags; /* beats completed; bit0 repolarising, bit1 EAD seen */
+    double b_vrest, b_vmax, b_vmin, b_apd90, b_ead_t, b_peak_ci, b_jsr, b_sparks;
 };
 
 struct RyrDD { double hi, lo; };
@@ -87,9 +93,15 @@ struct RyrKArgs {
     long long *diag_step;   /* per cell */
     double *line;           /* per cell x maxX x 22 (offset line_off[c]) */
     const long long *line_off;
+    double *beats;          /* per cell x RYR_MAX_BEATS x RYR_BEAT_COLS */
+    RyrDD *dpartial;        /* output-row chunk sums: per block x 21 */
+    unsigned long long *dcount;   /* output-row integer counts: per cell x 13 */
+    unsigned int *darrive;  /* per cell */
 };
 
 #define RYR_NCNT 5
+#define RYR_MAXB 64
+#define RYR_BCOLS 10
 
 /* ------------------------------------------------------------- reductions */
 RYR_DEV RyrDD dd_add(RyrDD a, RyrDD b)
@@ -230,7 +242,8 @@ RYR_DEV int lcc_step(int i, double cpx, double dt, double ragg, const RyrMem &m,
 RYR_DEV int ryr_transition(ryr_stream *s, int n, double p, int *fails)
 {
     RYR_STRICT_FP
-    double lam = ryr_exp(-(n) * p);
+    /* exp(-0) is exactly 1: skip the call for empty populations or zero rates */
+    double lam = (n == 0 || p == 0.0) ? 1.0 : ryr_exp(-(n) * p);
     int out = -1;
     if (n <= 1 || p < 0.2 || (n <= 5 && p < 0.3)) {
         int kk = 0;
@@ -257,9 +270,21 @@ RYR_DEV int ryr_transition(ryr_stream *s, int n, double p, int *fails)
     return out;
 }
 
-/* subroutine_cuda.c:251-669; counts in/out: nou, ncu, nob, ncb */
+/* CSQN monomer fraction (subroutine_cuda.c:170-171, :328-329): the same
+ * expression feeds the unbound->bound rate aub (divided by taub = 1, exact)
+ * and the luminal buffer factor, so it is evaluated once per CRU-step. */
+RYR_DEV double csqn_mono(double cjsr, double *roo2_out)
+{
+    RYR_STRICT_FP
+    double roo2 = 5000.0 / (1.0 + ryr_pow(850.0 / (cjsr), 23.0));
+    *roo2_out = roo2;
+    return (-1.0 + __builtin_sqrt(1.0 + 8.0 * 460.0 * roo2)) / (4.0 * roo2 * 460.0);
+}
+
+/* subroutine_cuda.c:251-669; counts in/out: nou, ncu, nob, ncb; mono =
+ * csqn_mono(cjsr) */
 RYR_DEV void ryr_gating(ryr_stream *s, double Ku, double Ku2, double Kb, double cp, double cjsr,
-                        double dt, int &nou, int &ncu, int &nob, int &ncb, int *fails)
+                        double mono, double dt, int &nou, int &ncu, int &nob, int &ncb, int *fails)
 {
     RYR_STRICT_FP
     double q1 = (5000 / cjsr) * (5000 / cjsr);
@@ -272,8 +297,7 @@ RYR_DEV void ryr_gating(ryr_stream *s, double Ku, double Ku2, double Kb, double 
     if (pkb * dt > 1.0) pkb = 1.0 / dt;
     if (pkuminus * dt > 1.0) pkuminus = 1.0 / dt;
     if (pkbminus * dt > 1.0) pkbminus = 1.0 / dt;
-    double roo2 = 5000.0 / (1.0 + ryr_pow(850.0 / (cjsr), 23.0));
-    double aub = (-1.0 + __builtin_sqrt(1.0 + 8.0 * 460.0 * roo2)) / (4.0 * roo2 * 460.0) / (1.0);
+    double aub = mono / (1.0);
     double abu = 1.0 / (0.2 * 5 * 220.);
 
     int n_ou_cu = ryr_transition(s, nou, pkuminus * dt, fails);
@@ -466,6 +490,230 @@ RYR_DEV T nb_load(const T *__restrict__ f, long long idx, bool inside, T centre)
     return inside ? f[idx] : centre;
 }
 
+/* ------------------------------------------------------------ beat tracker */
+RYR_DEV void beat_close(RyrMem *M, double *beats, double complete)
+{
+    if (M->beat_n < RYR_MAXB) {
+        double *r = beats + (long long)M->beat_n * RYR_BCOLS;
+        r[0] = (double)M->beat_on; r[1] = M->b_apd90; r[2] = M->b_vmax; r[3] = M->b_vrest;
+        r[4] = (M->beat_flags & 2) ? 1.0 : 0.0; r[5] = M->b_ead_t; r[6] = M->b_peak_ci;
+        r[7] = M->b_jsr; r[8] = M->b_sparks; r[9] = complete;
+    }
+    M->beat_n++;
+}
+
+/* V-based features of the beat, once per step after V is final (vo: V at
+ * the start of step n, vn: after it). Onset column holds the onset step;
+ * the host converts it to ms. */
+RYR_DEV void beat_track(RyrMem *M, const RyrCellInfo &C, long long n, double dt, double vo, double vn,
+                        double *beats)
+{
+    RYR_STRICT_FP
+    const bool onset = C.ncl > 0 && n >= C.noff && ((n - C.noff) % C.ncl) == 0;
+    if (onset) {
+        if (M->beat_on >= 0) beat_close(M, beats, 1.0);
+        M->beat_on = n;
+        M->b_vrest = vo; M->b_vmax = vo; M->b_vmin = vo;
+        M->b_apd90 = -1.0; M->b_ead_t = -1.0; M->beat_flags = 0;
+        M->b_peak_ci = 0.0; M->b_jsr = -1.0; M->b_sparks = 0.0;
+    }
+    if (M->beat_on < 0) return;
+    if (vn > M->b_vmax) M->b_vmax = vn;
+    const double amp = M->b_vmax - M->b_vrest;
+    if (!(M->beat_flags & 1)) {
+        if (M->b_vmax > -40.0 && vn < M->b_vmax - 0.4 * amp &&
+            (double)(n + 1) * dt - (double)M->beat_on * dt >= 50.0) { M->beat_flags |= 1; M->b_vmin = vn; }
+    } else {
+        if (vn < M->b_vmin) M->b_vmin = vn;
+        if (!(M->beat_flags & 2) && vn > -40.0 && vn - M->b_vmin >= 1.0) {
+            M->beat_flags |= 2;
+            M->b_ead_t = (double)(n + 1) * dt - (double)M->beat_on * dt;
+        }
+    }
+    if (M->b_apd90 < 0.0 && M->b_vmax > -40.0) {
+        const double thr = M->b_vmax - 0.9 * amp;
+        if (vo >= thr && vn < thr)
+            M->b_apd90 = ((double)n + (vo - thr) / (vo - vn)) * dt - (double)M->beat_on * dt;
+    }
+}
+
+/* ------------------------------------- FP32 production cytosol and cleft */
+/* buffers.c flux with the saturated on-rate 0.95/dt precomputed (sat) */
+RYR_DEV float fbuf(float b, float c, float dt, float sat, float kon, float koff, float B)
+{
+    const float on = kon * c;
+    return (on * dt > 1.0f) ? sat * (B - b) - koff * b : on * (B - b) - koff * b;
+}
+
+/* The cytosol/NSR update (cuda_Cell_Detailed.cu:1305-1393, :1557-1586) and
+ * the cleft/sub-membrane sub-steps (:1396-1464 scheme A, :1479-1553 scheme B)
+ * of the FP32 production path: the same equations with every division by a
+ * constant replaced by a product with its reciprocal, float arithmetic,
+ * and the slow pools (cnsr, slow troponin) accumulated in double storage. */
+template <typename TF, typename TS>
+RYR_DEV void fast_cytosol_and_cleft(const RyrKArgs &a, long long g, long long P, int i, int j, int k, int X, int Y,
+                                    int Z, long long sy, long long sz, const TF *cil, const TS *cnl,
+                                    const TF *__restrict__ ci0, const TS *__restrict__ cn0, TF *__restrict__ ci1,
+                                    TS *__restrict__ cn1, float dt, float sviup, TS diffjn0, TS diffjn1, TF &cpv,
+                                    const TF csv, const TF cs_up, const TF cs_dn, const TF icat, const TF xinaca,
+                                    const TF xire, TF &v_casar, TF &v_casarh, TF &v_casarj, TF &v_casarhj, TF &csn,
+                                    double &up_acc, double &leak_acc)
+{
+    TF *catf = (TF *)a.catf, *cacal = (TF *)a.cacal, *casr = (TF *)a.casr;
+    TF *camyo = (TF *)a.camyo, *mgmyo = (TF *)a.mgmyo;
+    TS *cats = (TS *)a.cats;
+    const float rIT = (float)(1.0 / ((0.33 * 2.) * 0.7)), rIL = (float)(1.0 / ((0.7 * 2.) * 0.7));
+    const float rNT = (float)(1.0 / (1.8 * 0.7)), rNL = (float)(1.0 / (6. * 0.7));
+    const float sat = 0.95f / dt;
+    const float cpf0 = (float)cpv, csf0 = (float)csv;
+    const float cif0 = (float)cil[0], cif1 = (float)cil[1];
+    const float Vi_nsr = (float)((0.5 / 8.) / (0.025 / 8.)), Vjsr_nsr = (float)((0.02 * 1) / (0.025 / 8.));
+    const float Vs_i = (float)((0.025) / (0.5 / 8.)), Vp_i = (float)((0.00126) / (0.5 / 8.));
+    const float Vp_s = (float)((0.00126) / (0.025));
+    const float r_tausi2 = (float)(1.0 / 0.04 / 2.), r_taupi2 = (float)(1.0 / 0.1 / 2.);
+    const float r_taups = (float)(1.0 / 0.0283), r_taust = (float)(1.0 / 1.42);
+    const bool schemeA = a.scheme == 0;
+    float dotci01[2] = {0.0f, 0.0f};
+    float up_a = 0.0f, leak_a = 0.0f;
+#pragma unroll
+    for (int ii = 0; ii < 8; ++ii) {
+        const long long pl = ii * P;
+        const float cc = (float)cil[ii];
+        const TS cn = cnl[ii];
+        const float cnf = (float)cn;
+        const float xiup = sviup * uptake_f(cc, cnf);
+        const float cn2 = cnf * cnf;
+        const float xileak = 1.035e-5f * (cnf - cc) * cn2 * __builtin_amdgcn_rcpf(cn2 + 250000.0f);
+        up_a += xiup;
+        leak_a += xileak;
+        float inN, inS, inE, inW, inT, inB;
+        TS nnN, nnS, nnE, nnW, nnT, nnB;
+        if (ii & 1) {
+            const bool in = k + 1 < Z;
+            inN = in ? (float)ci0[(ii - 1) * P + g + sz] : cc;
+            nnN = in ? cn0[(ii - 1) * P + g + sz] : cn;
+            inS = (float)cil[ii - 1]; nnS = cnl[ii - 1];
+        } else {
+            const bool in = k > 0;
+            inN = (float)cil[ii + 1]; nnN = cnl[ii + 1];
+            inS = in ? (float)ci0[(ii + 1) * P + g - sz] : cc;
+            nnS = in ? cn0[(ii + 1) * P + g - sz] : cn;
+        }
+        if (ii & 2) {
+            const bool in = j + 1 < Y;
+            inE = in ? (float)ci0[(ii - 2) * P + g + sy] : cc;
+            nnE = in ? cn0[(ii - 2) * P + g + sy] : cn;
+            inW = (float)cil[ii - 2]; nnW = cnl[ii - 2];
+        } else {
+            const bool in = j > 0;
+            inE = (float)cil[ii + 2]; nnE = cnl[ii + 2];
+            inW = in ? (float)ci0[(ii + 2) * P + g - sy] : cc;
+            nnW = in ? cn0[(ii + 2) * P + g - sy] : cn;
+        }
+        if (ii & 4) {
+            const bool in = i + 1 < X;
+            inT = in ? (float)ci0[(ii - 4) * P + g + 1] : cc;
+            nnT = in ? cn0[(ii - 4) * P + g + 1] : cn;
+            inB = (float)cil[ii - 4]; nnB = cnl[ii - 4];
+        } else {
+            const bool in = i > 0;
+            inT = (float)cil[ii + 4]; nnT = cnl[ii + 4];
+            inB = in ? (float)ci0[(ii + 4) * P + g - 1] : cc;
+            nnB = in ? cn0[(ii + 4) * P + g - 1] : cn;
+        }
+        const float xicoupi = ((inN - cc) + (inS - cc) + (inE - cc) + (inW - cc)) * rIT +
+                              ((inT - cc) + (inB - cc)) * rIL;
+        const float xicoupn = ((float)(nnN - cn) + (float)(nnS - cn) + (float)(nnE - cn) + (float)(nnW - cn)) * rNT +
+                              ((float)(nnT - cn) + (float)(nnB - cn)) * rNL;
+        float dotcnsr = (xiup - xileak) * Vi_nsr + xicoupn;
+        const float v_catf = (float)catf[pl + g], v_cacal = (float)cacal[pl + g], v_casr = (float)casr[pl + g];
+        const float v_camyo = (float)camyo[pl + g], v_mgmyo = (float)mgmyo[pl + g];
+        const TS v_cats = cats[pl + g];
+        const float f_tf = fbuf(v_catf, cc, dt, sat, 0.0327f, 0.0196f, 70.0f);
+        const float f_ts = fbuf((float)v_cats, cc, dt, sat, 0.00254f, 0.000033f, 134.0f);
+        const float f_cal = fbuf(v_cacal, cc, dt, sat, 0.0543f, 0.238f, 24.0f);
+        const float f_sr = fbuf(v_casr, cc, dt, sat, 0.1f, 0.06f, 19.0f);
+        const float freec = 140.0f - v_camyo - v_mgmyo;
+        const float on_my = 13.8f * cc;
+        const float f_myo = (on_my * dt > 1.0f ? sat : on_my) * freec - 0.46f * v_camyo;
+        float dotci = (xileak - xiup) - (f_tf + f_ts + f_cal + f_sr + f_myo) + xicoupi;
+        if (ii < 2) {
+            dotcnsr += (float)(ii == 0 ? diffjn0 : diffjn1) * Vjsr_nsr;
+            const float cif = ii == 0 ? cif0 : cif1;
+            if (schemeA) dotci += (csf0 - cif) * r_tausi2 * Vs_i + (cpf0 - cif) * r_taupi2 * Vp_i;
+            dotci01[ii] = dotci;
+        }
+        if (ii >= 2 || schemeA) ci1[pl + g] = (TF)fmaxf(cc + dotci * dt, 0.0f);
+        cn1[pl + g] = cn + (TS)(dotcnsr * dt);
+        const float n_camyo = fmaxf(v_camyo + f_myo * dt, 0.0f);
+        const float on_mg = 0.0157f * 500.0f;
+        const float f_mg = (on_mg * dt > 1.0f ? sat : on_mg) * (140.0f - n_camyo - v_mgmyo) - 0.057f * v_mgmyo;
+        catf[pl + g] = (TF)fmaxf(v_catf + f_tf * dt, 0.0f);
+        TS n_cats = v_cats + (TS)(f_ts * dt);
+        cats[pl + g] = n_cats < TS(0) ? TS(0) : n_cats;
+        cacal[pl + g] = (TF)fmaxf(v_cacal + f_cal * dt, 0.0f);
+        casr[pl + g] = (TF)fmaxf(v_casr + f_sr * dt, 0.0f);
+        camyo[pl + g] = (TF)n_camyo;
+        mgmyo[pl + g] = (TF)fmaxf(v_mgmyo + f_mg * dt, 0.0f);
+    }
+    up_acc = (double)up_a * 0.125;
+    leak_acc = (double)leak_a * 0.125;
+
+    const float dth = dt * 0.2f;
+    const float sath = 0.95f / dth;
+    const float ica = (float)icat, xna = (float)xinaca, xre = (float)xire;
+    const float csu = (float)cs_up, csd = (float)cs_dn;
+    float cp = cpf0, cs = csf0;
+    float sar = (float)v_casar, sarh = (float)v_casarh, sarj = (float)v_casarj, sarhj = (float)v_casarhj;
+    if (schemeA) {
+        const float csdiff = (csu + csd - 2.0f * csf0) * r_taust;
+        const float to_ci = ((csf0 - cif0) + (csf0 - cif1)) * r_tausi2;
+        const float cp_ci = ((cif0 - cpf0) + (cif1 - cpf0)) * r_taupi2;
+        const float ica_s = 0.5f * ica * Vp_s;
+#pragma unroll
+        for (int it = 0; it < 5; ++it) {
+            const float fsar = fbuf(sar, cs, dth, sath, 0.1f, 1.3f, (float)RYR_BSAR);
+            const float fsarh = fbuf(sarh, cs, dth, sath, 0.1f, 0.03f, (float)RYR_BSARH);
+            const float dotcs = Vp_s * (cp - cs) * r_taups - ica_s + xna - to_ci + csdiff - (fsar + fsarh);
+            const float cpbuff = fbuf(sarj, cp, dth, sath, 0.1f, 1.3f, (float)RYR_BSAR);
+            const float cpbuffh = fbuf(sarhj, cp, dth, sath, 0.1f, 0.03f, (float)RYR_BSARH);
+            const float cpdot = (cs - cp) * r_taups + cp_ci + xre - ica - cpbuff - cpbuffh;
+            sarj = fmaxf(sarj + cpbuff * dth, 0.0f);
+            sarhj = fmaxf(sarhj + cpbuffh * dth, 0.0f);
+            cp += cpdot * dth;
+            sar = fmaxf(sar + fsar * dth, 0.0f);
+            sarh = fmaxf(sarh + fsarh * dth, 0.0f);
+            cs += dotcs * dth;
+        }
+    } else {
+        const float csum = csu + csd;
+#pragma unroll
+        for (int it = 0; it < 5; ++it) {
+            dotci01[0] += ((cs - cif0) * r_tausi2 * Vs_i + (cp - cif0) * r_taupi2 * Vp_i) * 0.2f;
+            dotci01[1] += ((cs - cif1) * r_tausi2 * Vs_i + (cp - cif1) * r_taupi2 * Vp_i) * 0.2f;
+            const float fsar = fbuf(sar, cs, dth, sath, 0.1f, 1.3f, (float)RYR_BSAR);
+            const float fsarh = fbuf(sarh, cs, dth, sath, 0.1f, 0.03f, (float)RYR_BSARH);
+            const float dotcs = Vp_s * (cp - 0.5f * 0.0283f * ica - cs) * r_taups + xna -
+                                ((cs - cif0) + (cs - cif1)) * r_tausi2 + (csum - 2.0f * cs) * r_taust - (fsar + fsarh);
+            const float cpbuff = fbuf(sarj, cp, dth, sath, 0.1f, 1.3f, (float)RYR_BSAR);
+            const float cpbuffh = fbuf(sarhj, cp, dth, sath, 0.1f, 0.03f, (float)RYR_BSARH);
+            const float cpdot = (csf0 - cp) * r_taups + ((cif0 - cp) + (cif1 - cp)) * r_taupi2 + xre - ica -
+                                cpbuff - cpbuffh;
+            sar = fmaxf(sar + fsar * dth, 0.0f);
+            sarh = fmaxf(sarh + fsarh * dth, 0.0f);
+            cs += dotcs * dth;
+            sarj = fmaxf(sarj + cpbuff * dth, 0.0f);
+            sarhj = fmaxf(sarhj + cpbuffh * dth, 0.0f);
+            cp += cpdot * dth;
+        }
+        ci1[g] = (TF)fmaxf(cif0 + dotci01[0] * dt, 0.0f);
+        ci1[P + g] = (TF)fmaxf(cif1 + dotci01[1] * dt, 0.0f);
+    }
+    cpv = (TF)cp;
+    csn = (TF)cs;
+    v_casar = (TF)sar; v_casarh = (TF)sarh; v_casarj = (TF)sarj; v_casarhj = (TF)sarhj;
+}
+
 /* ------------------------------------------------------------ step kernel */
 template <typename TF, typename TS>
 __global__ __launch_bounds__(RYR_BLOCK) void k_cru_step(RyrKArgs a, double stim_amp)
@@ -550,7 +798,9 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_cru_step(RyrKArgs a, double stim_
         int nou = (int)(rw & 255u), ncu = (int)((rw >> 8) & 255u), nob = (int)((rw >> 16) & 255u),
             ncb = (int)(rw >> 24);
         const int prevsp = nou + nob;
-        ryr_gating(&rs, C.Ku, C.Ku2, C.Kb, cp_d, (double)cjsrv, dt, nou, ncu, nob, ncb, &nfail);
+        double roo2;
+        const double mono = csqn_mono((double)cjsrv, &roo2);
+        ryr_gating(&rs, C.Ku, C.Ku2, C.Kb, cp_d, (double)cjsrv, mono, dt, nou, ncu, nob, ncb, &nfail);
         if (prevsp < 5 && (nou + nob) >= 5) nspark = 1;
         double xire;
         {
@@ -561,10 +811,23 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_cru_step(RyrKArgs a, double stim_
             if (xire < 0) xire = 0;
         }
 
-        /* NCX (:1294-1304, subroutine_cuda.c:47-75), FP64 */
+        /* NCX (:1294-1304, subroutine_cuda.c:47-75): FP64 in the exact path,
+         * float with reciprocals in the production path (the per-CRU current
+         * still enters an FP64 compensated sum) */
         double xinaca;
         TF ancx_new;
-        {
+        if constexpr (sizeof(TF) != 8) {
+            const float csm = (float)cs_d * 1e-3f;
+            const float Ka = (float)ancx[g];
+            const float t2 = 669921.875f * csm * (1.0f + csm * (float)(1.0 / 0.00359));
+            const float t3 = (float)M->ncx_t3a + csm * 2744000.0f;
+            const float Inaca = Ka * 21.0f * ((float)M->ncx_n1 - (float)M->ncx_n2 * csm) *
+                                __builtin_amdgcn_rcpf(((float)M->ncx_t1 + t2 + t3) * (float)M->ncx_den2);
+            const float r = 3e-4f * __builtin_amdgcn_rcpf(csm);
+            const float Ancxdot = (__builtin_amdgcn_rcpf(1.0f + r * r * r) - Ka) * (1.0f / 150.0f);
+            ancx_new = (TF)(Ka + Ancxdot * (float)dt);
+            xinaca = (double)((float)C.svncx * Inaca);
+        } else {
             RYR_STRICT_FP
             double cs = cs_d / 1000.0;
             double Ka = (double)ancx[g];
@@ -582,18 +845,18 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_cru_step(RyrKArgs a, double stim_
         /* exchange fluxes and JSR (:1309-1318) */
         const TF cpT = cpv, csT = csv;
         const TF dtT = (TF)dt;
-        const TF diffpi0 = (cpT - cil[0]) / TF(0.1) / TF(2.);
-        const TF diffpi1 = (cpT - cil[1]) / TF(0.1) / TF(2.);
-        const TF diffsi0 = (csT - cil[0]) / TF(0.04) / TF(2.);
-        const TF diffsi1 = (csT - cil[1]) / TF(0.04) / TF(2.);
-        const TS diffjn0 = (cjsrv - cnl[0]) / TS(25.0 * 1.0 / 4.) / TS(2.);
-        const TS diffjn1 = (cjsrv - cnl[1]) / TS(25.0 * 1.0 / 4.) / TS(2.);
+        TS diffjn0, diffjn1;
+        if constexpr (sizeof(TF) == 8) {
+            diffjn0 = (cjsrv - cnl[0]) / TS(25.0 * 1.0 / 4.) / TS(2.);
+            diffjn1 = (cjsrv - cnl[1]) / TS(25.0 * 1.0 / 4.) / TS(2.);
+        } else {
+            diffjn0 = (cjsrv - cnl[0]) * TS(1.0 / (25.0 * 1.0 / 4.) / 2.);
+            diffjn1 = (cjsrv - cnl[1]) * TS(1.0 / (25.0 * 1.0 / 4.) / 2.);
+        }
         TS dotcjsr, cjsrb_new;
         {
             RYR_STRICT_FP
             const double cj = (double)cjsrv;
-            double roo2 = 5000.0 / (1.0 + ryr_pow(850.0 / cj, 23.0));
-            double mono = (-1.0 + __builtin_sqrt(1.0 + 8.0 * 460.0 * roo2)) / (4.0 * roo2 * 460.0);
             double ene = mono * 20 + (1.0 - mono) * 30;
             double enedot = (20 - 30) * (-mono + 1.0 / (4.0 * 460.0 * mono * roo2 + 1.0)) * (23.0 / cj)
                             * (1.0 - roo2 / 5000.0);
@@ -604,10 +867,25 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_cru_step(RyrKArgs a, double stim_
             cjsrb_new = (TS)((double)cjsrbp[g] + (1 - beta) * flux * dt);
         }
 
-        /* cytosol/NSR diffusion, buffers, updates (:1305-1393 / :1557-1586) */
+        /* cytosol/NSR diffusion, buffers, updates (:1305-1393 / :1557-1586) and
+         * the cleft/sub-membrane sub-steps (:1356-1473 / :1475-1597) */
+        TF v_casar = casar[g], v_casarh = casarh[g], v_casarj = casarj[g], v_casarhj = casarhj[g];
+        const TF cs_up = (k + 1 < Z) ? cs0[g + sz] : csv;
+        const TF cs_dn = (k > 0) ? cs0[g - sz] : csv;
+        const TF icatT = (TF)icat1, xinacaT = (TF)xinaca, xireT = (TF)xire;
+        TF csn = csv;
+        double up_acc = 0.0, leak_acc = 0.0;
+        if constexpr (sizeof(TF) != 8) {
+            fast_cytosol_and_cleft(a, g, P, i, j, k, X, Y, Z, sy, sz, cil, cnl, ci0, cn0, ci1, cn1, (float)dt,
+                                   (float)C.sviup_tot, diffjn0, diffjn1, cpv, csv, cs_up, cs_dn, icatT, xinacaT,
+                                   xireT, v_casar, v_casarh, v_casarj, v_casarhj, csn, up_acc, leak_acc);
+        } else {
+        const TF diffpi0 = (cpT - cil[0]) / TF(0.1) / TF(2.);
+        const TF diffpi1 = (cpT - cil[1]) / TF(0.1) / TF(2.);
+        const TF diffsi0 = (csT - cil[0]) / TF(0.04) / TF(2.);
+        const TF diffsi1 = (csT - cil[1]) / TF(0.04) / TF(2.);
         const TF sviup = (TF)C.sviup_tot;
         TF dotci01[2];
-        double up_acc = 0.0, leak_acc = 0.0;
 #pragma unroll
         for (int ii = 0; ii < 8; ++ii) {
             const long long pl = ii * P;
@@ -703,12 +981,7 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_cru_step(RyrKArgs a, double stim_
         }
 
         /* cleft / sub-membrane sub-steps */
-        TF v_casar = casar[g], v_casarh = casarh[g], v_casarj = casarj[g], v_casarhj = casarhj[g];
         const TF dth = dtT / TF(5);
-        const TF cs_up = (k + 1 < Z) ? cs0[g + sz] : csv;
-        const TF cs_dn = (k > 0) ? cs0[g - sz] : csv;
-        const TF icatT = (TF)icat1, xinacaT = (TF)xinaca, xireT = (TF)xire;
-        TF csn = csv;
         if (a.scheme == 0) {
             /* scheme A (:1396-1464) */
             const TF cpstart = cpT;
@@ -788,6 +1061,7 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_cru_step(RyrKArgs a, double stim_
                 ci1[ii * P + g] = cin;
             }
         }
+        }   /* exact path */
         if (cpv < TF(0)) { cpv = TF(0); ++nclampcp; }
         if (csn < TF(0)) { csn = TF(1e-6); ++nclampcs; }
         casar[g] = v_casar; casarh[g] = v_casarh; casarj[g] = v_casarj; casarhj[g] = v_casarhj;
@@ -850,12 +1124,19 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_cru_step(RyrKArgs a, double stim_
             s2 = dd_add(s2, pp[3 * b + 2]);
         }
         const double stim = stim_of(C, nstep, stim_amp);
+        const double vold = M->m[0];
         membrane_step(M, C, s0.hi, s1.hi, s2.hi, dt, stim);
+        if (C.vclamp_n > 0) {
+            /* clamp protocols (cuda_Cell_Detailed.cu:733-773) */
+            long long q = C.vclamp_periodic ? nstep % C.vclamp_n : (nstep < C.vclamp_n ? nstep : C.vclamp_n - 1);
+            M->m[0] = C.vclamp[q];
+        }
         M->stim = stim;
         M->out_ica += M->cur[1];
         M->out_ncx += M->cur[2];
         M->out_ina += M->cur[3];
         const double vnew = M->m[0];
+        beat_track(M, C, nstep, dt, vold, vnew, a.beats + (long long)c * RYR_MAXB * RYR_BCOLS);
         if (!(vnew == vnew) || vnew == __builtin_inf() || vnew == -__builtin_inf())
             atomicAdd(a.counters + (long long)c * RYR_NCNT + 4, 1ull);
         if (diag) {
@@ -878,15 +1159,21 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_cru_step(RyrKArgs a, double stim_
 }
 
 /* ------------------------------------------------------- output-row kernel */
-/* One workgroup per cell; the sums run chunk by chunk in the same order as
- * the step reduction (pairwise tree in a chunk, chunks in order). */
+/* One workgroup per 256-CRU chunk (the step kernel's chunks): each forms the
+ * pairwise double-double sums of its chunk and the integer counts; the last
+ * workgroup of a cell adds the chunk sums in chunk order (the same order as
+ * the step reduction and the oracle) and writes the row
+ * (cuda_Cell_Detailed.cu:839-962, :1031-1072). */
 #define RYR_DIAG_Q 21
+#define RYR_DIAG_I 13
 template <typename TF, typename TS>
 __global__ __launch_bounds__(RYR_BLOCK) void k_diag(RyrKArgs a)
 {
     __shared__ RyrDD s_red[RYR_DIAG_Q][RYR_BLOCK / 64];
-    __shared__ unsigned long long s_cnt[16];
-    const int c = blockIdx.x;
+    __shared__ unsigned long long s_cnt[RYR_DIAG_I];
+    __shared__ int s_last;
+    const int b = blockIdx.x;
+    const int c = a.block_cell[b];
     const RyrCellInfo C = a.cells[c];
     RyrMem *M = a.mem + c;
     if (a.diag_step[c] != M->step - 1) return;   /* not an output step */
@@ -903,88 +1190,113 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_diag(RyrKArgs a)
     const TF *casarj = (const TF *)a.casarj, *casarhj = (const TF *)a.casarhj;
     const TS *cjsr = (const TS *)a.cjsr, *cjsrb = (const TS *)a.cjsrb;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    if (threadIdx.x < 16) s_cnt[threadIdx.x] = 0;
+    if (threadIdx.x < RYR_DIAG_I) s_cnt[threadIdx.x] = 0;
     __syncthreads();
 
-    RyrDD acc[RYR_DIAG_Q];
+    unsigned long long ic[RYR_DIAG_I] = {0};  /* tnou tnob tncu tncb tn1..tn7 sparks runaway */
+    const int n = (b - C.first_block) * RYR_BLOCK + (int)threadIdx.x;
+    double val[RYR_DIAG_Q];
 #pragma unroll
-    for (int q = 0; q < RYR_DIAG_Q; ++q) { acc[q].hi = 0.0; acc[q].lo = 0.0; }
-    unsigned long long ic[13] = {0};  /* tnou tnob tncu tncb tn1..tn7 sparks runaway */
-
-    for (int base = 0; base < C.N; base += RYR_BLOCK) {
-        const int n = base + (int)threadIdx.x;
-        double val[RYR_DIAG_Q];
-#pragma unroll
-        for (int q = 0; q < RYR_DIAG_Q; ++q) val[q] = 0.0;
-        if (n < C.N) {
-            RYR_STRICT_FP
-            const long long g = C.off + n;
-            const uint32_t rw = a.ryrw[g], lw = a.lccw[g], ax = a.aux[g];
-            const int nou = rw & 255, ncu = (rw >> 8) & 255, nob = (rw >> 16) & 255, ncb = rw >> 24;
-            double xin = a.d_xinaca[g];
-            val[0] = xin < 0 ? xin : 0.0;
-            val[1] = (double)ancx[g];
-            val[2] = (double)cp[g];
-            val[3] = (double)cjsr[g];
-            val[4] = (double)cjsrb[g];
-            val[5] = (double)(nou + nob) / (double)(RYR_NRYR);
-            val[6] = (double)ncb;
-            val[7] = (double)casar[g];
-            val[8] = (double)casarh[g];
-            val[9] = (double)casarj[g];
-            val[10] = (double)casarhj[g];
-            double s_ci = 0, s_cn = 0, s_tf = 0, s_ts = 0, s_sr = 0, s_my = 0, s_cal = 0;
-            for (int ii = 0; ii < 8; ++ii) {
-                const long long e = ii * P + g;
-                s_ci = s_ci + (double)ci[e] / 8.;
-                s_cn = s_cn + (double)cn[e] / 8.;
-                s_tf = s_tf + (double)catf[e] / 8.;
-                s_ts = s_ts + (double)cats[e] / 8.;
-                s_sr = s_sr + (double)casr[e] / 8.;
-                s_my = s_my + (double)camyo[e] / 8.;
-                s_cal = s_cal + (double)cacal[e] / 8.;
-                if ((double)ci[e] > 1000) ic[12]++;
-            }
-            val[11] = s_ci; val[12] = s_cn; val[13] = s_tf; val[14] = s_ts; val[15] = s_sr;
-            val[16] = s_my; val[17] = s_cal;
-            val[18] = a.d_leak[g];
-            val[19] = a.d_up[g];
-            val[20] = a.d_xire[g];
-            ic[0] += nou; ic[1] += nob; ic[2] += ncu; ic[3] += ncb;
-            ic[11] += (ax >> 16) & 1u;
-            for (int jj = 0; jj < C.nlcc; ++jj) {
-                const int st = (lw >> (4 * jj)) & 15;
-                /* histogram of cuda_Cell_Detailed.cu:904-923 */
-                if (st == 0) { ic[10]++; ic[9]++; continue; }
-                if (st & 1) ic[4]++;
-                if (st & 2) ic[5]++;
-                if (st & 4) ic[6]++;
-                if (st & 8) ic[7]++;
-                if ((st & 4) && !(st & 8)) ic[8]++;
-                if (st == 1) ic[9]++;
-            }
+    for (int q = 0; q < RYR_DIAG_Q; ++q) val[q] = 0.0;
+    if (n < C.N) {
+        RYR_STRICT_FP
+        const long long g = C.off + n;
+        const uint32_t rw = a.ryrw[g], lw = a.lccw[g], ax = a.aux[g];
+        const int nou = rw & 255, ncu = (rw >> 8) & 255, nob = (rw >> 16) & 255, ncb = rw >> 24;
+        double xin = a.d_xinaca[g];
+        val[0] = xin < 0 ? xin : 0.0;
+        val[1] = (double)ancx[g];
+        val[2] = (double)cp[g];
+        val[3] = (double)cjsr[g];
+        val[4] = (double)cjsrb[g];
+        val[5] = (double)(nou + nob) / (double)(RYR_NRYR);
+        val[6] = (double)ncb;
+        val[7] = (double)casar[g];
+        val[8] = (double)casarh[g];
+        val[9] = (double)casarj[g];
+        val[10] = (double)casarhj[g];
+        double s_ci = 0, s_cn = 0, s_tf = 0, s_ts = 0, s_sr = 0, s_my = 0, s_cal = 0;
+        for (int ii = 0; ii < 8; ++ii) {
+            const long long e = ii * P + g;
+            s_ci = s_ci + (double)ci[e] / 8.;
+            s_cn = s_cn + (double)cn[e] / 8.;
+            s_tf = s_tf + (double)catf[e] / 8.;
+            s_ts = s_ts + (double)cats[e] / 8.;
+            s_sr = s_sr + (double)casr[e] / 8.;
+            s_my = s_my + (double)camyo[e] / 8.;
+            s_cal = s_cal + (double)cacal[e] / 8.;
+            if ((double)ci[e] > 1000) ic[12]++;
         }
-#pragma unroll
-        for (int q = 0; q < RYR_DIAG_Q; ++q) {
-            RyrDD t = {val[q], 0.0};
-            t = wave_tree(t);
-            if (lane == 0) s_red[q][wv] = t;
+        val[11] = s_ci; val[12] = s_cn; val[13] = s_tf; val[14] = s_ts; val[15] = s_sr;
+        val[16] = s_my; val[17] = s_cal;
+        val[18] = a.d_leak[g];
+        val[19] = a.d_up[g];
+        val[20] = a.d_xire[g];
+        ic[0] += nou; ic[1] += nob; ic[2] += ncu; ic[3] += ncb;
+        ic[11] += (ax >> 16) & 1u;
+        for (int jj = 0; jj < C.nlcc; ++jj) {
+            const int st = (lw >> (4 * jj)) & 15;
+            /* histogram of cuda_Cell_Detailed.cu:904-923 */
+            if (st == 0) { ic[10]++; ic[9]++; continue; }
+            if (st & 1) ic[4]++;
+            if (st & 2) ic[5]++;
+            if (st & 4) ic[6]++;
+            if (st & 8) ic[7]++;
+            if ((st & 4) && !(st & 8)) ic[8]++;
+            if (st == 1) ic[9]++;
         }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-#pragma unroll
-            for (int q = 0; q < RYR_DIAG_Q; ++q)
-                acc[q] = dd_add(acc[q], dd_add(dd_add(s_red[q][0], s_red[q][1]), dd_add(s_red[q][2], s_red[q][3])));
+        /* linescan row at y = line_j, z = line_k (interior indices) */
+        const int row0 = C.X * (C.line_j + C.Y * C.line_k);
+        if (n >= row0 && n < row0 + C.X) {
+            const int x = n - row0;
+            double *o = a.line + a.line_off[c] + (long long)x * 22;
+            o[0] = (double)(M->step - 1) * a.dt; o[1] = (double)(x + 1); o[2] = (double)ci[g];
+            o[3] = (double)nob; o[4] = (double)nou; o[5] = (double)ncb;
+            o[6] = (double)ncu; o[7] = (double)(ax & 0xffff); o[8] = (double)cs[g];
+            o[9] = (double)cp[g]; o[10] = (double)casar[g]; o[11] = (double)casarh[g];
+            o[12] = (double)camyo[g]; o[13] = (double)((const TF *)a.mgmyo)[g]; o[14] = (double)casr[g];
+            o[15] = (double)cacal[g]; o[16] = (double)cats[g]; o[17] = (double)catf[g];
+            o[18] = a.d_xire[g]; o[19] = (double)cjsr[g]; o[20] = (double)ancx[g]; o[21] = a.d_xinaca[g];
         }
-        __syncthreads();
     }
 #pragma unroll
-    for (int q = 0; q < 13; ++q)
+    for (int q = 0; q < RYR_DIAG_Q; ++q) {
+        RyrDD t = {val[q], 0.0};
+        t = wave_tree(t);
+        if (lane == 0) s_red[q][wv] = t;
+    }
+#pragma unroll
+    for (int q = 0; q < RYR_DIAG_I; ++q)
         if (ic[q]) atomicAdd(&s_cnt[q], ic[q]);
     __syncthreads();
-
     if (threadIdx.x == 0) {
+        RyrDD *pp = a.dpartial + (long long)b * RYR_DIAG_Q;
+#pragma unroll
+        for (int q = 0; q < RYR_DIAG_Q; ++q)
+            pp[q] = dd_add(dd_add(s_red[q][0], s_red[q][1]), dd_add(s_red[q][2], s_red[q][3]));
+        unsigned long long *cc = a.dcount + (long long)c * RYR_DIAG_I;
+        for (int q = 0; q < RYR_DIAG_I; ++q)
+            if (s_cnt[q]) atomicAdd(cc + q, s_cnt[q]);
+        __threadfence();
+        unsigned int prev = atomicAdd(a.darrive + c, 1u);
+        s_last = (prev == (unsigned int)(C.nblocks - 1));
+    }
+    __syncthreads();
+    if (!s_last || threadIdx.x != 0) return;
+
+    {
         RYR_STRICT_FP
+        __threadfence();
+        RyrDD acc[RYR_DIAG_Q];
+#pragma unroll
+        for (int q = 0; q < RYR_DIAG_Q; ++q) { acc[q].hi = 0.0; acc[q].lo = 0.0; }
+        const RyrDD *pp = a.dpartial + (long long)C.first_block * RYR_DIAG_Q;
+        for (int bb = 0; bb < C.nblocks; ++bb)
+#pragma unroll
+            for (int q = 0; q < RYR_DIAG_Q; ++q) acc[q] = dd_add(acc[q], pp[(long long)bb * RYR_DIAG_Q + q]);
+        unsigned long long *cc = a.dcount + (long long)c * RYR_DIAG_I;
+        unsigned long long cnt[RYR_DIAG_I];
+        for (int q = 0; q < RYR_DIAG_I; ++q) { cnt[q] = __hip_atomic_load(cc + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); cc[q] = 0ull; }
         const double dN = C.dN;
         double *r = a.diag + (long long)c * 45;
         double ncxfwd = acc[0].hi / dN, outAncx = acc[1].hi / dN, cproxit = acc[2].hi / dN;
@@ -1008,26 +1320,18 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_diag(RyrKArgs a)
         r[1] = cit; r[5] = cproxit; r[7] = cjsrt; r[8] = cnsrt; r[9] = poto;
         r[19] = ncxfwd * ((0.025) / (0.00126)); r[20] = pbcto; r[21] = cacalt; r[22] = catft;
         r[23] = leakt; r[24] = upt; r[25] = ire;
-        r[26] = (double)s_cnt[0]; r[27] = (double)s_cnt[1]; r[28] = (double)s_cnt[2]; r[29] = (double)s_cnt[3];
-        for (int q = 0; q < 7; ++q) r[30 + q] = (double)s_cnt[4 + q];
-        r[37] = outAncx; r[40] = (double)s_cnt[11]; r[41] = catst; r[42] = csqnbt;
+        r[26] = (double)cnt[0]; r[27] = (double)cnt[1]; r[28] = (double)cnt[2]; r[29] = (double)cnt[3];
+        for (int q = 0; q < 7; ++q) r[30 + q] = (double)cnt[4 + q];
+        r[37] = outAncx; r[40] = (double)cnt[11]; r[41] = catst; r[42] = csqnbt;
         r[43] = Total_Ci; r[44] = Total_Ca;
-        if (s_cnt[12]) a.counters[(long long)c * RYR_NCNT + 3] += s_cnt[12];
-    }
-    /* linescan row at y = line_j, z = line_k (interior indices) */
-    double *L = a.line + a.line_off[c];
-    for (int x = threadIdx.x; x < C.X; x += RYR_BLOCK) {
-        const int n = x + C.X * (C.line_j + C.Y * C.line_k);
-        const long long g = C.off + n;
-        const uint32_t rw = a.ryrw[g], ax = a.aux[g];
-        double *o = L + (long long)x * 22;
-        o[0] = (double)(M->step - 1) * a.dt; o[1] = (double)(x + 1); o[2] = (double)ci[g];
-        o[3] = (double)((rw >> 16) & 255); o[4] = (double)(rw & 255); o[5] = (double)(rw >> 24);
-        o[6] = (double)((rw >> 8) & 255); o[7] = (double)(ax & 0xffff); o[8] = (double)cs[g];
-        o[9] = (double)cp[g]; o[10] = (double)casar[g]; o[11] = (double)casarh[g];
-        o[12] = (double)camyo[g]; o[13] = (double)((const TF *)a.mgmyo)[g]; o[14] = (double)casr[g];
-        o[15] = (double)cacal[g]; o[16] = (double)cats[g]; o[17] = (double)catf[g];
-        o[18] = a.d_xire[g]; o[19] = (double)cjsr[g]; o[20] = (double)ancx[g]; o[21] = a.d_xinaca[g];
+        if (cnt[12]) a.counters[(long long)c * RYR_NCNT + 3] += cnt[12];
+        if (M->beat_on >= 0) {
+            if (M->step - 1 == M->beat_on) M->b_jsr = cjsrt;
+            else M->b_sparks += (double)cnt[11];
+            if (cit > M->b_peak_ci) M->b_peak_ci = cit;
+        }
+        a.darrive[c] = 0u;
+        __threadfence();
     }
 }
 
@@ -1168,7 +1472,8 @@ __global__ void k_export(RyrKArgs a, int cell, int field, int planes, double *ou
     const long long n = e % C.N;
     const long long g = C.off + n;
     const int cur = (int)(a.mem[cell].step & 1);
-    out[e] = field_get<TF, TS>(a, field, cur, (long long)p * a.P + g, g, p);
+    /* slots past the cell's channel count are absent: state 16 (:1157-1162) */
+    out[e] = (field == 21 && p >= C.nlcc) ? 16.0 : field_get<TF, TS>(a, field, cur, (long long)p * a.P + g, g, p);
 }
 
 template <typename TF, typename TS>
@@ -1213,7 +1518,9 @@ __global__ void k_gating_probe(long long nq, unsigned long long seed, const uint
     }
     int nou = counts[4 * q], ncu = counts[4 * q + 1], nob = counts[4 * q + 2], ncb = counts[4 * q + 3];
     int f = 0;
-    ryr_gating(&rs, Ku, Ku2, Kb, cp, cjsr, dt, nou, ncu, nob, ncb, &f);
+    double roo2;
+    const double mono = csqn_mono(cjsr, &roo2);
+    ryr_gating(&rs, Ku, Ku2, Kb, cp, cjsr, mono, dt, nou, ncu, nob, ncb, &f);
     counts[4 * q] = nou; counts[4 * q + 1] = ncu; counts[4 * q + 2] = nob; counts[4 * q + 3] = ncb;
     draws[q] = (int32_t)rs.slot;
     nopen[q] = no;

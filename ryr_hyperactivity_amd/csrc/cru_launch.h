@@ -27,7 +27,7 @@ RYR_DECLARE_LAUNCHERS(f32)
     }                                                                                           \
     hipError_t ryr_launch_diag_##sfx(const RyrKArgs &a, int ncells, hipStream_t s)              \
     {                                                                                           \
-        hipLaunchKernelGGL((k_diag<TF, TS>), dim3(ncells), dim3(RYR_BLOCK), 0, s, a);           \
+        hipLaunchKernelGGL((k_diag<TF, TS>), dim3(ncells), dim3(RYR_BLOCK), 0, s, a);   /* ncells = workgroups */           \
         return hipGetLastError();                                                               \
     }                                                                                           \
     hipError_t ryr_launch_init_##sfx(const RyrKArgs &a, const RyrInit *init, int nblocks,       \

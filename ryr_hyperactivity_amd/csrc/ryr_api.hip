@@ -38,6 +38,7 @@ struct ryr_handle {
     bool timing_open = false;
     long long launches = 0;
     std::string err;
+    std::vector<double *> vclamp;   /* per cell clamp waveform (device) */
 };
 
 static std::string g_create_err;
@@ -248,7 +249,12 @@ extern "C" int ryr_create(int device, const ryr_params *p, const ryr_cell_desc *
     ALLOC(d_part, sizeof(RyrDD) * 3 * h->nblocks);
     ALLOC(d_arr, sizeof(unsigned int) * ncells);
     ALLOC(d_loff, sizeof(long long) * ncells);
+    ALLOC(a.beats, sizeof(double) * ncells * RYR_MAXB * RYR_BCOLS);
+    ALLOC(a.dpartial, sizeof(RyrDD) * RYR_DIAG_Q * h->nblocks);
+    ALLOC(a.dcount, sizeof(unsigned long long) * RYR_DIAG_I * ncells);
+    ALLOC(a.darrive, sizeof(unsigned int) * ncells);
 #undef ALLOC
+    h->vclamp.assign(ncells, nullptr);
     a.cells = d_cells; a.mem = d_mem; a.block_cell = d_bc; a.partial = d_part; a.arrive = d_arr;
     a.counters = d_cnt; a.diag = d_diag; a.diag_step = d_dstep; a.line = d_line; a.line_off = d_loff;
 
@@ -299,6 +305,7 @@ extern "C" int ryr_create(int device, const ryr_params *p, const ryr_cell_desc *
         M.m[10] = 0.020; M.m[11] = 0.80;
         M.m[12] = 6.1 * ((9000) / (cells[c].cl + 5000));
         M.step = 0;
+        M.beat_on = -1;
     }
     RyrInit *d_init = nullptr;
     if (hipMalloc((void **)&d_init, sizeof(RyrInit) * ncells) != hipSuccess ||
@@ -331,7 +338,7 @@ static hipError_t launch_one(ryr_handle *h, long long s)
                        : ryr_launch_step_f64(h->a, h->p.stim_amp, h->nblocks, h->stream);
     if (e != hipSuccess) return e;
     if (s % h->p.steps_per_output == 0)
-        e = f32 ? ryr_launch_diag_f32(h->a, h->ncells, h->stream) : ryr_launch_diag_f64(h->a, h->ncells, h->stream);
+        e = f32 ? ryr_launch_diag_f32(h->a, h->nblocks, h->stream) : ryr_launch_diag_f64(h->a, h->nblocks, h->stream);
     return e;
 }
 
@@ -535,6 +542,53 @@ extern "C" int ryr_read_counters(ryr_handle *h, int64_t *out)
     return RYR_OK;
 }
 
+extern "C" int ryr_read_beats(ryr_handle *h, int cell, double *out, int maxbeats, int *nbeats)
+{
+    if (!h || !out || !nbeats || cell < 0 || cell >= h->ncells || maxbeats < 0) return RYR_E_ARG;
+    RYR_HIPCHECK(h, hipSetDevice(h->device));
+    RYR_HIPCHECK(h, hipStreamSynchronize(h->stream));
+    RyrMem M;
+    RYR_HIPCHECK(h, hipMemcpy(&M, h->a.mem + cell, sizeof M, hipMemcpyDeviceToHost));
+    std::vector<double> rec((size_t)RYR_MAXB * RYR_BCOLS);
+    RYR_HIPCHECK(h, hipMemcpy(rec.data(), h->a.beats + (size_t)cell * RYR_MAXB * RYR_BCOLS,
+                              sizeof(double) * rec.size(), hipMemcpyDeviceToHost));
+    int done = M.beat_n < RYR_MAXB ? M.beat_n : RYR_MAXB;
+    int k = 0;
+    for (; k < done && k < maxbeats; ++k) {
+        for (int q = 0; q < RYR_BCOLS; ++q) out[k * RYR_BCOLS + q] = rec[(size_t)k * RYR_BCOLS + q];
+        out[k * RYR_BCOLS] *= h->p.dt;
+    }
+    if (M.beat_on >= 0 && k < maxbeats && M.beat_n < RYR_MAXB) {
+        double *r = out + (size_t)k * RYR_BCOLS;
+        r[0] = (double)M.beat_on * h->p.dt; r[1] = M.b_apd90; r[2] = M.b_vmax; r[3] = M.b_vrest;
+        r[4] = (M.beat_flags & 2) ? 1.0 : 0.0; r[5] = M.b_ead_t; r[6] = M.b_peak_ci; r[7] = M.b_jsr;
+        r[8] = M.b_sparks; r[9] = 0.0;
+        ++k;
+    }
+    *nbeats = k;
+    return RYR_OK;
+}
+
+extern "C" int ryr_set_vclamp(ryr_handle *h, int cell, const double *v, int64_t n, int periodic)
+{
+    if (!h || cell < 0 || cell >= h->ncells || n < 0 || (n > 0 && !v)) return RYR_E_ARG;
+    RYR_HIPCHECK(h, hipSetDevice(h->device));
+    RYR_HIPCHECK(h, hipStreamSynchronize(h->stream));
+    if (h->vclamp[cell]) { RYR_HIPCHECK(h, hipFree(h->vclamp[cell])); h->vclamp[cell] = nullptr; }
+    RyrCellInfo &I = h->info[cell];
+    I.vclamp = nullptr;
+    I.vclamp_n = 0;
+    I.vclamp_periodic = periodic ? 1 : 0;
+    if (n > 0) {
+        RYR_HIPCHECK(h, hipMalloc((void **)&h->vclamp[cell], sizeof(double) * n));
+        RYR_HIPCHECK(h, hipMemcpy(h->vclamp[cell], v, sizeof(double) * n, hipMemcpyHostToDevice));
+        I.vclamp = h->vclamp[cell];
+        I.vclamp_n = n;
+    }
+    RYR_HIPCHECK(h, hipMemcpy((RyrCellInfo *)h->a.cells + cell, &I, sizeof I, hipMemcpyHostToDevice));
+    return RYR_OK;
+}
+
 extern "C" int ryr_cru_steps(ryr_handle *h, int64_t *out)
 {
     if (!h || !out) return RYR_E_ARG;
@@ -612,6 +666,7 @@ extern "C" void ryr_destroy(ryr_handle *h)
     if (h->graph) hipGraphExecDestroy(h->graph);
     for (auto &al : h->allocs) hipFree(al.first);
     if (h->stage) hipFree(h->stage);
+    for (double *q : h->vclamp) if (q) hipFree(q);
     if (h->ev0) hipEventDestroy(h->ev0);
     if (h->ev1) hipEventDestroy(h->ev1);
     if (h->stream) hipStreamDestroy(h->stream);

@@ -123,6 +123,19 @@ class CellEngine:
                     "ryr_read_counters")
         return out
 
+    def beats(self, cell: int = 0) -> np.ndarray:
+        """Per-beat summary rows (columns _abi.BEAT_NAMES)."""
+        out = np.zeros((_abi.MAX_BEATS + 1, _abi.BEAT_COLS))
+        n = ctypes.c_int()
+        self._check(self.lib.ryr_read_beats(self.h, cell, _dp(out), _abi.MAX_BEATS + 1, ctypes.byref(n)),
+                    "ryr_read_beats")
+        return out[:n.value]
+
+    def set_vclamp(self, cell: int, v, periodic: bool = False):
+        """Replace V after each step by a waveform (see clamp_waveform)."""
+        v = np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(-1))
+        self._check(self.lib.ryr_set_vclamp(self.h, cell, _dp(v), v.size, int(bool(periodic))), "ryr_set_vclamp")
+
     def save(self, path: str):
         self._check(self.lib.ryr_save(self.h, path.encode()), "ryr_save")
 
@@ -152,6 +165,27 @@ def gating_probe(seed, cru, step, Ku, Ku2, Kb, dt, iso2, counts, lcc, conc, devi
     if rc != _abi.RYR_OK:
         raise EngineError(f"ryr_gating_probe failed ({rc}): {lib.ryr_last_error(None).decode()}")
     return counts, lcc, draws, nopen, fails
+
+
+def clamp_waveform(name: str, nsteps: int, dt: float = 0.025, clamp_voltage: float = 0.0):
+    """V after step n (n = 0..nsteps-1) for the reference's clamp protocols,
+    with t = n*dt the time at the start of the step:
+      vclamp        -86 mV, then clamp_voltage (argv[3]) once t > 100 ms (:733-736)
+      brownvclamp   -86 / -10 / 10 / -30 / -86 mV with edges at 4000, 4200,
+                    6200, 7200 ms (:741-752)
+      linearapclamp piecewise-linear AP from 100 ms (:758-773)
+    apclamp (:754-757) is a periodic waveform read from a file: pass the
+    array itself to CellEngine.set_vclamp(..., periodic=True)."""
+    t = np.arange(nsteps) * dt
+    if name == "vclamp":
+        return np.where(t > 100, clamp_voltage, -86.0)
+    if name == "brownvclamp":
+        return np.select([t < 4000, t < 4200, t < 6200, t < 7200], [-86.0, -10.0, 10.0, -30.0], -86.0)
+    if name == "linearapclamp":
+        tt = t - 100
+        return np.select([t < 100, t < 101.865, t < 119.1, t < 138.9, t < 364.43, t < 395.43],
+                         [-86.0, 68 * tt - 80, -tt + 48, 27 + 0.1 * tt, 43 - tt / 3.2, -1.5 * tt + 357], -86.0)
+    raise ValueError(f"unknown clamp protocol {name}")
 
 
 # -- output files (cuda_Cell_Detailed.cu:974-997, :1041-1072) --------------
