@@ -135,21 +135,33 @@ static orc_dd dd_add(orc_dd a, orc_dd b)
     return r;
 }
 
-/* Pairwise double-double tree over chunks of 256 consecutive values
- * (zero-padded), chunk results accumulated in chunk order. */
+static orc_dd dd_tree256(orc_dd *v)
+{
+    for (int w = 1; w < 256; w *= 2)
+        for (int l = 0; l < 256; l += 2 * w)
+            v[l] = dd_add(v[l], v[l + w]);
+    return v[0];
+}
+
+/* Two-level pairwise double-double tree: values in chunks of 256
+ * consecutive entries (zero-padded) give chunk sums; chunk sums in groups of
+ * 256 (zero-padded) give group sums; group sums are added in order. */
 double orc_canonical_sum(const double *x, int64_t n)
 {
     orc_dd acc = {0.0, 0.0};
-    orc_dd v[256];
+    orc_dd v[256], grp[256];
+    int gl = 0;
     for (int64_t c0 = 0; c0 < n; c0 += 256) {
         for (int l = 0; l < 256; ++l) {
             v[l].hi = (c0 + l < n) ? x[c0 + l] : 0.0;
             v[l].lo = 0.0;
         }
-        for (int w = 1; w < 256; w *= 2)
-            for (int l = 0; l < 256; l += 2 * w)
-                v[l] = dd_add(v[l], v[l + w]);
-        acc = dd_add(acc, v[0]);
+        grp[gl++] = dd_tree256(v);
+        if (gl == 256 || c0 + 256 >= n) {
+            for (int l = gl; l < 256; ++l) { grp[l].hi = 0.0; grp[l].lo = 0.0; }
+            acc = dd_add(acc, dd_tree256(grp));
+            gl = 0;
+        }
     }
     return acc.hi;
 }

@@ -81,13 +81,14 @@ struct RyrKArgs {
     void *ci[2], *cnsr[2], *cs[2];
     void *catf, *cats, *cacal, *casr, *camyo, *mgmyo;
     void *cp, *cjsr, *cjsrb, *Ancx, *casar, *casarh, *casarj, *casarhj;
+    void *sb;               /* scheme B: partial cytosol rates of sub-voxels 0,1 (2 planes) */
     uint32_t *lccw, *ryrw, *aux;          /* aux: nl (bits 0-15) | nspark (bit 16) */
-    double *d_xire, *d_xinaca, *d_up, *d_leak;   /* per CRU, written on output steps */
+    double *d_xire, *d_xinaca;   /* per CRU, written on output steps */
+    double *d_up, *d_leak;       /* per sub-voxel (8 planes), written on output steps */
     const RyrCellInfo *cells;
     RyrMem *mem;
     const int *block_cell;
     RyrDD *partial;         /* 3 per block */
-    unsigned int *arrive;   /* per cell */
     unsigned long long *counters;  /* per cell x RYR_C_COUNT */
     double *diag;           /* per cell x 45 */
     long long *diag_step;   /* per cell */
@@ -343,127 +344,247 @@ RYR_DEV void ryr_gating(ryr_stream *s, double Ku, double Ku2, double Kb, double 
     ncb = ncb < 0 ? 0 : (ncb > RYR_NRYR ? RYR_NRYR : ncb);
 }
 
-/* -------------------------------------------- membrane ODE (FP64, one lane) */
+/* ------------------------------------------ membrane ODE (FP64, one wave) */
 /* cuda_Cell_Detailed.cu:447-739: whole-cell currents from the three sums,
- * gates by Rush-Larsen, [Na]i and V by forward Euler. */
-RYR_DEV void membrane_step(RyrMem *M, const RyrCellInfo &C, double sum_icat, double sum_inaca,
-                           double sum_cs, double dt, double stim)
+ * gates by Rush-Larsen, [Na]i and V by forward Euler.  One wavefront
+ * evaluates it: the exponentials and logarithms whose arguments depend only
+ * on V, [Na]i and constants are spread over the lanes and computed in one
+ * SIMD pass, the Rush-Larsen factors exp(-dt/tau) in a second pass; every
+ * lane then repeats the same scalar arithmetic on the broadcast values.  Each
+ * argument is the reference's expression and the arithmetic keeps its
+ * operand order, so the result does not depend on which lane computed what.
+ * out[0..12]: new v xm xh xj xr xs1 xs2 qks xtos ytos xtof ytof xnai;
+ * cur[0..9]: csubt xicatto xinacato xina xiks xikr xik1 xinak xitos xitof */
+RYR_DEV double lanev(double x, int src) { return __shfl(x, src, 64); }
+
+RYR_DEV void membrane_wave(const double *s, double svikr, double Cm, double dN, double sum_icat,
+                           double sum_inaca, double sum_cs, double dt, double stim, int lane, double *out,
+                           double *cur)
 {
     RYR_STRICT_FP
-    double *s = M->m;
     double v = s[0], xm = s[1], xh = s[2], xj = s[3], xr = s[4], xs1 = s[5], xs2 = s[6];
     double qks = s[7], xtos = s[8], ytos = s[9], xtof = s[10], ytof = s[11], xnai = s[12];
     const double xki = 140.0, xko = 5.40;
     const double frt = (96.485 / 8.314 / 308.0);
+    const double ek = (1.00 / frt) * ryr_log(xko / xki);
+    const double rt1 = -(v + 3.0) / 15.00;
+    const double rt2 = (v + 33.5) / 10.00;
+    const double rt3 = (v + 60.00) / 10.00;
+    const double rt4 = -(v / 30.00) * (v / 30.00);
+    const double rt5 = (v + 33.50) / 10.00;
 
-    double csubt = sum_cs / C.dN;
-    double xicatto = sum_icat / C.Cm * 0.0965 * (0.00126) * 2.0 * ((0.5) + 1.);
-    double xinacato = sum_inaca / C.Cm * 0.0965 * (0.025);
+    /* pass 1: exponentials of V, [Na]i and constants; logs of ENa and EKs */
+    double arg = 0.0;
+    switch (lane) {
+    case 0: arg = -0.1 * (v + 47.13); break;
+    case 1: arg = -v / 11.0; break;
+    case 2: arg = (80.0 + v) / (-6.8); break;
+    case 3: arg = 0.079 * v; break;
+    case 4: arg = 0.35 * v; break;
+    case 5: arg = 0.2444 * v; break;
+    case 6: arg = -0.04391 * v; break;
+    case 7: arg = 0.311 * (v + 79.23); break;
+    case 8: arg = -0.01052 * v; break;
+    case 9: arg = -0.1378 * (v + 40.14); break;
+    case 10: arg = (v + 10.66) / (-11.10); break;
+    case 11: arg = -0.00000025350 * v; break;
+    case 12: arg = -0.10 * (v + 32.00); break;
+    case 13: arg = -0.123 * (v + 7.00); break;
+    case 14: arg = 0.1450 * (v + 10.00); break;
+    case 15: arg = -(v + 50.00) / 7.50; break;
+    case 16: arg = (v + 33.00) / 22.40; break;
+    case 17: arg = -(v - 1.500) / 16.700; break;
+    case 18: arg = -0.1480 * (v + 30.0); break;
+    case 19: arg = 0.06870 * (v + 30.00); break;
+    case 20: arg = 0.2385 * (v - ek - 59.215); break;
+    case 21: arg = 0.08032 * (v - ek + 5.476); break;
+    case 22: arg = 0.061750 * (v - ek - 594.31); break;
+    case 23: arg = -0.5143 * (v - ek + 4.753); break;
+    case 24: arg = rt1; break;
+    case 25: arg = rt2; break;
+    case 26: arg = -rt1; break;
+    case 27: arg = rt3; break;
+    case 28: arg = rt4; break;
+    case 29: arg = rt5; break;
+    case 30: arg = 140.0 / 67.30; break;
+    case 31: arg = -0.1 * v * frt; break;
+    case 32: arg = -v * frt; break;
+    default: break;
+    }
+    const double ex = ryr_exp(arg);
+    double larg = 1.0;
+    if (lane == 0) larg = 140.0 / xnai;
+    else if (lane == 1) larg = (xko + 0.0183300 * 140.0) / (xki + 0.0183300 * xnai);
+    const double lg = ryr_log(larg);
 
-    double ena = (1.0 / frt) * ryr_log(140.0 / xnai);
-    double am = 0.32 * (v + 47.13) / (1.0 - ryr_exp(-0.1 * (v + 47.13)));
-    double bm = 0.08 * ryr_exp(-v / 11.0);
+    const double csubt = sum_cs / dN;
+    const double xicatto = sum_icat / Cm * 0.0965 * (0.00126) * 2.0 * ((0.5) + 1.);
+    const double xinacato = sum_inaca / Cm * 0.0965 * (0.025);
+
+    /* I_Na (:460-502) */
+    const double ena = (1.0 / frt) * lanev(lg, 0);
+    const double am = 0.32 * (v + 47.13) / (1.0 - lanev(ex, 0));
+    const double bm = 0.08 * lanev(ex, 1);
     double ah, bh, aj, bj;
     if (v < -40.0) {
-        ah = 0.135 * ryr_exp((80.0 + v) / (-6.8));
-        bh = 3.56 * ryr_exp(0.079 * v) + 310000.0 * ryr_exp(0.35 * v);
-        aj = (-127140.0 * ryr_exp(0.2444 * v) - 0.00003474 * ryr_exp(-0.04391 * v))
-             * ((v + 37.78) / (1.0 + ryr_exp(0.311 * (v + 79.23))));
-        bj = (0.1212 * ryr_exp(-0.01052 * v)) / (1.0 + ryr_exp(-0.1378 * (v + 40.14)));
+        ah = 0.135 * lanev(ex, 2);
+        bh = 3.56 * lanev(ex, 3) + 310000.0 * lanev(ex, 4);
+        aj = (-127140.0 * lanev(ex, 5) - 0.00003474 * lanev(ex, 6)) * ((v + 37.78) / (1.0 + lanev(ex, 7)));
+        bj = (0.1212 * lanev(ex, 8)) / (1.0 + lanev(ex, 9));
     } else {
         ah = 0.00;
-        bh = 1.00 / (0.130 * (1.00 + ryr_exp((v + 10.66) / (-11.10))));
+        bh = 1.00 / (0.130 * (1.00 + lanev(ex, 10)));
         aj = 0.00;
-        bj = (0.3 * ryr_exp(-0.00000025350 * v)) / (1.0 + ryr_exp(-0.10 * (v + 32.00)));
+        bj = (0.3 * lanev(ex, 11)) / (1.0 + lanev(ex, 12));
     }
-    double tauh = 1.00 / (ah + bh);
-    double tauj = 1.00 / (aj + bj);
-    double taum = 1.00 / (am + bm);
-    double gna = 15;
-    double gnaleak = 0.3e-3 * 5;
-    double xina = gna * xh * xj * xm * xm * xm * (v - ena) + gnaleak * (v - ena);
-    xh = ah / (ah + bh) - ((ah / (ah + bh)) - xh) * ryr_exp(-dt / tauh);
-    xj = aj / (aj + bj) - ((aj / (aj + bj)) - xj) * ryr_exp(-dt / tauj);
-    xm = am / (am + bm) - ((am / (am + bm)) - xm) * ryr_exp(-dt / taum);
+    const double tauh = 1.00 / (ah + bh);
+    const double tauj = 1.00 / (aj + bj);
+    const double taum = 1.00 / (am + bm);
+    const double gna = 15;
+    const double gnaleak = 0.3e-3 * 5;
+    const double xina = gna * xh * xj * xm * xm * xm * (v - ena) + gnaleak * (v - ena);
 
-    double ek = (1.00 / frt) * ryr_log(xko / xki);
-    double gss = __builtin_sqrt(xko / 5.40);
-    double xkrv1 = 0.001380 * (v + 7.00) / (1.0 - ryr_exp(-0.123 * (v + 7.00)));
-    double xkrv2 = 0.000610 * (v + 10.00) / (ryr_exp(0.1450 * (v + 10.00)) - 1.00);
-    double taukr = 1.00 / (xkrv1 + xkrv2);
-    double xkrinf = 1.00 / (1.00 + ryr_exp(-(v + 50.00) / 7.50));
-    double rg = 1.00 / (1.00 + ryr_exp((v + 33.00) / 22.40));
-    double gkr = 0.0078360;
-    double xikr = C.svikr * gkr * gss * xr * rg * (v - ek);
-    xr = xkrinf - (xkrinf - xr) * ryr_exp(-dt / taukr);
+    /* I_Kr (:507-529) */
+    const double gss = __builtin_sqrt(xko / 5.40);
+    const double xkrv1 = 0.001380 * (v + 7.00) / (1.0 - lanev(ex, 13));
+    const double xkrv2 = 0.000610 * (v + 10.00) / (lanev(ex, 14) - 1.00);
+    const double taukr = 1.00 / (xkrv1 + xkrv2);
+    const double xkrinf = 1.00 / (1.00 + lanev(ex, 15));
+    const double rg = 1.00 / (1.00 + lanev(ex, 16));
+    const double gkr = 0.0078360;
+    const double xikr = svikr * gkr * gss * xr * rg * (v - ek);
 
-    double prnak = 0.0183300;
-    double qks_inf = 0.2 * (1 + 0.8 / (1 + ryr_pow((0.28 / csubt), 3)));
-    double tauqks = 1000.00;
-    double eks = (1.00 / frt) * ryr_log((xko + prnak * 140.0) / (xki + prnak * xnai));
-    double xs1ss = 1.0 / (1.0 + ryr_exp(-(v - 1.500) / 16.700));
-    double tauxs = 1.00 / (0.0000719 * (v + 30.00) / (1.00 - ryr_exp(-0.1480 * (v + 30.0)))
-                           + 0.0001310 * (v + 30.00) / (ryr_exp(0.06870 * (v + 30.00)) - 1.00));
+    /* I_Ks (:534-559) */
+    const double prnak = 0.0183300;
+    (void)prnak;
+    const double qks_inf = 0.2 * (1 + 0.8 / (1 + ryr_pow((0.28 / csubt), 3)));
+    const double tauqks = 1000.00;
+    const double eks = (1.00 / frt) * lanev(lg, 1);
+    const double xs1ss = 1.0 / (1.0 + lanev(ex, 17));
+    double tauxs = 1.00 / (0.0000719 * (v + 30.00) / (1.00 - lanev(ex, 18))
+                           + 0.0001310 * (v + 30.00) / (lanev(ex, 19) - 1.00));
     tauxs = 0.95 * tauxs;
-    double gksx = 0.2000;
-    double sviks_iso = 1.;
-    double xiks = 1.0 * sviks_iso * gksx * qks * xs1 * xs2 * (v - eks);
-    xs1 = xs1ss - (xs1ss - xs1) * ryr_exp(-dt / tauxs);
-    xs2 = xs1ss - (xs1ss - xs2) * ryr_exp(-dt / tauxs);
+    const double gksx = 0.2000;
+    const double sviks_iso = 1.;
+    const double xiks = 1.0 * sviks_iso * gksx * qks * xs1 * xs2 * (v - eks);
+
+    /* I_K1 (:590-600) */
+    const double gkix = 0.600;
+    const double gki = gkix * (__builtin_sqrt(xko / 5.4));
+    const double aki = 1.02 / (1.0 + lanev(ex, 20));
+    const double bki = (0.49124 * lanev(ex, 21) + lanev(ex, 22)) / (1.0 + lanev(ex, 23));
+    const double xkin = aki / (aki + bki);
+    const double xik1 = (1.0) * gki * xkin * (v - ek);
+
+    /* I_to,s and I_to,f (:604-659) */
+    const double xtos_inf = 1.00 / (1.0 + lanev(ex, 24));
+    const double ytos_inf = 1.00 / (1.00 + lanev(ex, 25));
+    const double rs_inf = 1.00 / (1.00 + lanev(ex, 25));
+    const double txs = 9.00 / (1.00 + lanev(ex, 26)) + 0.50;
+    const double tys = 3000.00 / (1.0 + lanev(ex, 27)) + 30.00;
+    const double gtos = 0.040;
+    const double xitos = 1.0 * gtos * xtos * (ytos + 0.50 * rs_inf) * (v - ek);
+    const double xtof_inf = xtos_inf;
+    const double ytof_inf = ytos_inf;
+    const double txf = 3.50 * lanev(ex, 28) + 1.50;
+    const double tyf = 20.0 / (1.0 + lanev(ex, 29)) + 20.00;
+    const double gtof = 0.10;
+    const double xitof = 1.0 * gtof * xtof * ytof * (v - ek);
+
+    /* I_NaK (:665-676) */
+    const double xkmko = 1.50, xkmnai = 12.00, xibarnak = 1.5000, hh = 1.00;
+    const double sigma = (lanev(ex, 30) - 1.00) / 7.00;
+    const double fnak = 1.00 / (1.0 + 0.1245 * lanev(ex, 31) + 0.0365 * sigma * lanev(ex, 32));
+    const double xinak = 1.0 * xibarnak * fnak * (1.0 / (1.0 + ryr_pow((xkmnai / xnai), hh))) * xko / (xko + xkmko);
+
+    /* pass 2: Rush-Larsen factors */
+    double arg2 = 0.0;
+    switch (lane) {
+    case 0: arg2 = -dt / tauh; break;
+    case 1: arg2 = -dt / tauj; break;
+    case 2: arg2 = -dt / taum; break;
+    case 3: arg2 = -dt / taukr; break;
+    case 4: arg2 = -dt / tauxs; break;
+    case 5: arg2 = -dt / txs; break;
+    case 6: arg2 = -dt / tys; break;
+    case 7: arg2 = -dt / txf; break;
+    case 8: arg2 = -dt / tyf; break;
+    default: break;
+    }
+    const double ex2 = ryr_exp(arg2);
+    xh = ah / (ah + bh) - ((ah / (ah + bh)) - xh) * lanev(ex2, 0);
+    xj = aj / (aj + bj) - ((aj / (aj + bj)) - xj) * lanev(ex2, 1);
+    xm = am / (am + bm) - ((am / (am + bm)) - xm) * lanev(ex2, 2);
+    xr = xkrinf - (xkrinf - xr) * lanev(ex2, 3);
+    xs1 = xs1ss - (xs1ss - xs1) * lanev(ex2, 4);
+    xs2 = xs1ss - (xs1ss - xs2) * lanev(ex2, 4);
     qks = qks + dt * (qks_inf - qks) / tauqks;
+    xtos = xtos_inf - (xtos_inf - xtos) * lanev(ex2, 5);
+    ytos = ytos_inf - (ytos_inf - ytos) * lanev(ex2, 6);
+    xtof = xtof_inf - (xtof_inf - xtof) * lanev(ex2, 7);
+    ytof = ytof_inf - (ytof_inf - ytof) * lanev(ex2, 8);
 
-    double gkix = 0.600;
-    double gki = gkix * (__builtin_sqrt(xko / 5.4));
-    double aki = 1.02 / (1.0 + ryr_exp(0.2385 * (v - ek - 59.215)));
-    double bki = (0.49124 * ryr_exp(0.08032 * (v - ek + 5.476)) + ryr_exp(0.061750 * (v - ek - 594.31)))
-                 / (1.0 + ryr_exp(-0.5143 * (v - ek + 4.753)));
-    double xkin = aki / (aki + bki);
-    double xik1 = (1.0) * gki * xkin * (v - ek);
-
-    double rt1 = -(v + 3.0) / 15.00;
-    double rt2 = (v + 33.5) / 10.00;
-    double rt3 = (v + 60.00) / 10.00;
-    double xtos_inf = 1.00 / (1.0 + ryr_exp(rt1));
-    double ytos_inf = 1.00 / (1.00 + ryr_exp(rt2));
-    double rs_inf = 1.00 / (1.00 + ryr_exp(rt2));
-    double txs = 9.00 / (1.00 + ryr_exp(-rt1)) + 0.50;
-    double tys = 3000.00 / (1.0 + ryr_exp(rt3)) + 30.00;
-    double gtos = 0.040;
-    double xitos = 1.0 * gtos * xtos * (ytos + 0.50 * rs_inf) * (v - ek);
-    xtos = xtos_inf - (xtos_inf - xtos) * ryr_exp(-dt / txs);
-    ytos = ytos_inf - (ytos_inf - ytos) * ryr_exp(-dt / tys);
-
-    double xtof_inf = xtos_inf;
-    double ytof_inf = ytos_inf;
-    double rt4 = -(v / 30.00) * (v / 30.00);
-    double rt5 = (v + 33.50) / 10.00;
-    double txf = 3.50 * ryr_exp(rt4) + 1.50;
-    double tyf = 20.0 / (1.0 + ryr_exp(rt5)) + 20.00;
-    double gtof = 0.10;
-    double xitof = 1.0 * gtof * xtof * ytof * (v - ek);
-    xtof = xtof_inf - (xtof_inf - xtof) * ryr_exp(-dt / txf);
-    ytof = ytof_inf - (ytof_inf - ytof) * ryr_exp(-dt / tyf);
-
-    double xkmko = 1.50, xkmnai = 12.00, xibarnak = 1.5000, hh = 1.00;
-    double sigma = (ryr_exp(140.0 / 67.30) - 1.00) / 7.00;
-    double fnak = 1.00 / (1.0 + 0.1245 * ryr_exp(-0.1 * v * frt) + 0.0365 * sigma * ryr_exp(-v * frt));
-    double xinak = 1.0 * xibarnak * fnak * (1.0 / (1.0 + ryr_pow((xkmnai / xnai), hh))) * xko / (xko + xkmko);
-
-    double wcainv = 1.0 / 50.0;
-    double conv = 0.18 * 12500;
-    double xinaca = xinacato;
-    double trick = 1.0;
-    double xrr = trick * (1.0 / wcainv / conv) / 1000.0;
-    double dnai = -xrr * (xina + 3.0 * xinak + 3.0 * xinaca);
+    /* [Na]i and V (:701-738) */
+    const double wcainv = 1.0 / 50.0;
+    const double conv = 0.18 * 12500;
+    const double xinaca = xinacato;
+    const double trick = 1.0;
+    const double xrr = trick * (1.0 / wcainv / conv) / 1000.0;
+    const double dnai = -xrr * (xina + 3.0 * xinak + 3.0 * xinaca);
     xnai += dnai * dt;
-    double dvh = -(xina + xik1 + 1 * xikr + 1 * xiks + xitos + 1 * xitof + 1. * xinacato + 1 * xicatto + xinak) + stim;
+    const double dvh = -(xina + xik1 + 1 * xikr + 1 * xiks + xitos + 1 * xitof + 1. * xinacato + 1 * xicatto + xinak) + stim;
     v += dvh * dt;
 
-    s[0] = v; s[1] = xm; s[2] = xh; s[3] = xj; s[4] = xr; s[5] = xs1; s[6] = xs2;
-    s[7] = qks; s[8] = xtos; s[9] = ytos; s[10] = xtof; s[11] = ytof; s[12] = xnai;
-    double *cur = M->cur;
+    out[0] = v; out[1] = xm; out[2] = xh; out[3] = xj; out[4] = xr; out[5] = xs1; out[6] = xs2;
+    out[7] = qks; out[8] = xtos; out[9] = ytos; out[10] = xtof; out[11] = ytof; out[12] = xnai;
     cur[0] = csubt; cur[1] = xicatto; cur[2] = xinacato; cur[3] = xina; cur[4] = xiks;
     cur[5] = xikr; cur[6] = xik1; cur[7] = xinak; cur[8] = xitos; cur[9] = xitof;
+}
+
+/* The V-only factors of the next step (vfactors) with the exponentials
+ * spread over the lanes; lane 0 stores them. */
+RYR_DEV void vfactors_wave(RyrMem *M, int iso2, int lane, const double v, const double xnai)
+{
+    RYR_STRICT_FP
+    double dv5 = 5, dvk = 8, fv5 = -22.8, fvk = 9.1;
+    if (iso2) { dv5 = 0; fv5 = -28; fvk = 8.5; }
+    const double q = 0.0337 * (v + 10.5);
+    const double za = v * 96.485 / 8.314 / 308;
+    double arg = 0.0;
+    switch (lane) {
+    case 0: arg = -(v - dv5) / dvk; break;
+    case 1: arg = -(v - fv5) / fvk; break;
+    case 2: arg = (v - 60) / 12.; break;
+    case 3: arg = -(q * q); break;
+    case 4: arg = 2.0 * za; break;
+    case 5: arg = 0.35 * za; break;
+    case 6: arg = (0.35 - 1.0) * za; break;
+    default: break;
+    }
+    const double ex = ryr_exp(arg);
+    const double e0 = lanev(ex, 0), e1 = lanev(ex, 1), e2 = lanev(ex, 2), e3 = lanev(ex, 3);
+    const double e4 = lanev(ex, 4), e5 = lanev(ex, 5), e6 = lanev(ex, 6);
+    if (lane != 0) return;
+    double dinf = 1.0 / (1.0 + e0);
+    double taudin = 1.0 / ((1.0 - e0) / (0.035 * (v - dv5)) * dinf);
+    if ((v < 0.0001) && (v > -0.0001)) taudin = 0.035 * dvk / dinf;
+    double finf = 1. - 1.0 / (1.0 + e1) / (1. + e2);
+    double taufin = (0.02 - 0.007 * e3);
+    M->alphad = dinf * taudin;
+    M->betad = (1 - dinf) * taudin;
+    M->alphaf = (finf) * taufin;
+    M->betaf = (1 - finf) * taufin;
+    M->ghk_small = __builtin_fabs(za) < 0.001;
+    M->ghk_e2 = e4;
+    M->ghk_A = 4.0 * 11.9 * za * 96.485 * 0.341;
+    M->ghk_D = M->ghk_e2 - 1.0;
+    double r = xnai / 12.3;
+    M->ncx_t1 = 0.00359 * ((140.0) * (140.0) * (140.0)) * (1.0 + r * r * r);
+    double p3 = xnai * xnai * xnai;
+    M->ncx_t3a = (1.3 + (1 * 1.8)) * p3;
+    M->ncx_n1 = e5 * p3 * (1 * 1.8);
+    M->ncx_n2 = e6 * ((140.0) * (140.0) * (140.0));
+    M->ncx_den2 = 1.0 + 0.27 * e6;
 }
 
 RYR_DEV double stim_of(const RyrCellInfo &C, long long n, double amp)
@@ -537,7 +658,7 @@ RYR_DEV void beat_track(RyrMem *M, const RyrCellInfo &C, long long n, double dt,
     }
 }
 
-/* ------------------------------------- FP32 production cytosol and cleft */
+/* ------------------------------------------------------ cytosol/NSR kernel */
 /* buffers.c flux with the saturated on-rate 0.95/dt precomputed (sat) */
 RYR_DEV float fbuf(float b, float c, float dt, float sat, float kon, float koff, float B)
 {
@@ -545,126 +666,197 @@ RYR_DEV float fbuf(float b, float c, float dt, float sat, float kon, float koff,
     return (on * dt > 1.0f) ? sat * (B - b) - koff * b : on * (B - b) - koff * b;
 }
 
-/* The cytosol/NSR update (cuda_Cell_Detailed.cu:1305-1393, :1557-1586) and
- * the cleft/sub-membrane sub-steps (:1396-1464 scheme A, :1479-1553 scheme B)
- * of the FP32 production path: the same equations with every division by a
- * constant replaced by a product with its reciprocal, float arithmetic,
- * and the slow pools (cnsr, slow troponin) accumulated in double storage. */
+/* One lane per sub-voxel of the 2x-refined cytosol/NSR grid: a workgroup is
+ * one plane ii of one 256-CRU chunk.  SERCA and leak, the 7-point stencils of
+ * ci and cnsr, the five cytosolic buffers and the ci/cnsr/buffer updates
+ * (cuda_Cell_Detailed.cu:1305-1308, :1320-1393 / :1557-1586), plus the
+ * JSR->NSR and sub-membrane/cleft->cytosol exchange terms of sub-voxels 0 and 1
+ * (:1309-1314, :1356-1361).  Under scheme B those two sub-voxels receive
+ * their cleft terms during the sub-steps, so their partial rate is left in
+ * `sb` for the CRU kernel.  Runs before the CRU kernel of the same step
+ * (it reads the step-start cp, cs, cjsr). */
 template <typename TF, typename TS>
-RYR_DEV void fast_cytosol_and_cleft(const RyrKArgs &a, long long g, long long P, int i, int j, int k, int X, int Y,
-                                    int Z, long long sy, long long sz, const TF *cil, const TS *cnl,
-                                    const TF *__restrict__ ci0, const TS *__restrict__ cn0, TF *__restrict__ ci1,
-                                    TS *__restrict__ cn1, float dt, float sviup, TS diffjn0, TS diffjn1, TF &cpv,
-                                    const TF csv, const TF cs_up, const TF cs_dn, const TF icat, const TF xinaca,
-                                    const TF xire, TF &v_casar, TF &v_casarh, TF &v_casarj, TF &v_casarhj, TF &csn,
-                                    double &up_acc, double &leak_acc)
+__global__ __launch_bounds__(RYR_BLOCK) void k_cyto(RyrKArgs a)
 {
+    const int b = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int ii = (int)blockIdx.y;
+    const int c = a.block_cell[b];
+    const RyrCellInfo C = a.cells[c];
+    const long long nstep = a.mem[c].step;
+    const int n = (b - C.first_block) * RYR_BLOCK + (int)threadIdx.x;
+    if (n >= C.N) return;
+    const int par = (int)(nstep & 1);
+    const bool diag = (nstep % a.spo) == 0;
+    const long long P = a.P, g = C.off + n, e = ii * P + g;
+    const int X = C.X, Y = C.Y, Z = C.Z;
+    const int i = n % X, j = (n / X) % Y, k = n / (X * Y);
+    const long long sy = X, sz = (long long)X * Y;
+    const TF *__restrict__ ci0 = (const TF *)a.ci[par];
+    TF *__restrict__ ci1 = (TF *)a.ci[par ^ 1];
+    const TS *__restrict__ cn0 = (const TS *)a.cnsr[par];
+    TS *__restrict__ cn1 = (TS *)a.cnsr[par ^ 1];
     TF *catf = (TF *)a.catf, *cacal = (TF *)a.cacal, *casr = (TF *)a.casr;
     TF *camyo = (TF *)a.camyo, *mgmyo = (TF *)a.mgmyo;
     TS *cats = (TS *)a.cats;
-    const float rIT = (float)(1.0 / ((0.33 * 2.) * 0.7)), rIL = (float)(1.0 / ((0.7 * 2.) * 0.7));
-    const float rNT = (float)(1.0 / (1.8 * 0.7)), rNL = (float)(1.0 / (6. * 0.7));
-    const float sat = 0.95f / dt;
-    const float cpf0 = (float)cpv, csf0 = (float)csv;
-    const float cif0 = (float)cil[0], cif1 = (float)cil[1];
-    const float Vi_nsr = (float)((0.5 / 8.) / (0.025 / 8.)), Vjsr_nsr = (float)((0.02 * 1) / (0.025 / 8.));
+    const bool schemeA = a.scheme == 0;
+
+    /* neighbours on the fine grid (bit0 z, bit1 y, bit2 x); outside the
+     * interior the mirrored ghost equals the centre value */
+    const TF cc = ci0[e];
+    const TS cn = cn0[e];
+    TF inN, inS, inE, inW, inT, inB;
+    TS nnN, nnS, nnE, nnW, nnT, nnB;
+    if (ii & 1) {
+        const bool in = k + 1 < Z;
+        inN = in ? ci0[(ii - 1) * P + g + sz] : cc; nnN = in ? cn0[(ii - 1) * P + g + sz] : cn;
+        inS = ci0[(ii - 1) * P + g]; nnS = cn0[(ii - 1) * P + g];
+    } else {
+        const bool in = k > 0;
+        inN = ci0[(ii + 1) * P + g]; nnN = cn0[(ii + 1) * P + g];
+        inS = in ? ci0[(ii + 1) * P + g - sz] : cc; nnS = in ? cn0[(ii + 1) * P + g - sz] : cn;
+    }
+    if (ii & 2) {
+        const bool in = j + 1 < Y;
+        inE = in ? ci0[(ii - 2) * P + g + sy] : cc; nnE = in ? cn0[(ii - 2) * P + g + sy] : cn;
+        inW = ci0[(ii - 2) * P + g]; nnW = cn0[(ii - 2) * P + g];
+    } else {
+        const bool in = j > 0;
+        inE = ci0[(ii + 2) * P + g]; nnE = cn0[(ii + 2) * P + g];
+        inW = in ? ci0[(ii + 2) * P + g - sy] : cc; nnW = in ? cn0[(ii + 2) * P + g - sy] : cn;
+    }
+    if (ii & 4) {
+        const bool in = i + 1 < X;
+        inT = in ? ci0[(ii - 4) * P + g + 1] : cc; nnT = in ? cn0[(ii - 4) * P + g + 1] : cn;
+        inB = ci0[(ii - 4) * P + g]; nnB = cn0[(ii - 4) * P + g];
+    } else {
+        const bool in = i > 0;
+        inT = ci0[(ii + 4) * P + g]; nnT = cn0[(ii + 4) * P + g];
+        inB = in ? ci0[(ii + 4) * P + g - 1] : cc; nnB = in ? cn0[(ii + 4) * P + g - 1] : cn;
+    }
+
+    if constexpr (sizeof(TF) == 8) {
+        /* exact path: the reference's operand order throughout */
+        const double dt = a.dt;
+        const double xiup = C.sviup_tot * uptake_d(cc, cn);
+        const double xileak = 1.0 * leak_t<double>(cn, cc);
+        const double xicoupn = ((nnN - cn) / (1.8 * 0.7) + (nnS - cn) / (1.8 * 0.7) +
+                                (nnE - cn) / (1.8 * 0.7) + (nnW - cn) / (1.8 * 0.7) +
+                                (nnT - cn) / (6. * 0.7) + (nnB - cn) / (6. * 0.7));
+        const double xicoupi = ((inN - cc) / ((0.33 * 2.) * 0.7) + (inS - cc) / ((0.33 * 2.) * 0.7) +
+                                (inE - cc) / ((0.33 * 2.) * 0.7) + (inW - cc) / ((0.33 * 2.) * 0.7) +
+                                (inT - cc) / ((0.7 * 2.) * 0.7) + (inB - cc) / ((0.7 * 2.) * 0.7));
+        double dotcnsr = (xiup - xileak) * (0.5 / 8.) / (0.025 / 8.) + xicoupn;
+        const double v_catf = catf[e], v_cats = cats[e], v_cacal = cacal[e], v_casr = casr[e];
+        const double v_camyo = camyo[e], v_mgmyo = mgmyo[e];
+        const double f_tf = b_tropf<double>(v_catf, cc, dt);
+        const double f_ts = b_trops<double>(v_cats, cc, dt);
+        const double f_cal = b_bucal<double>(v_cacal, cc, dt);
+        const double f_sr = b_busr<double>(v_casr, cc, dt);
+        const double f_myo = b_myoca<double>(v_camyo, v_mgmyo, cc, dt);
+        const double buffers = (f_tf + f_ts + f_cal + f_sr + f_myo);
+        double dotci = (-xiup + xileak) - buffers + xicoupi;
+        if (ii < 2) {
+            const double cj = ((const TS *)a.cjsr)[g];
+            const double diffjn = (cj - cn) / (25.0 * 1.0 / 4.) / 2.;
+            dotcnsr += diffjn * (0.02 * 1) / (0.025 / 8.);
+            if (schemeA) {
+                const double csv = ((const TF *)a.cs[par])[g], cpv = ((const TF *)a.cp)[g];
+                dotci += (csv - cc) / (0.04) / 2. * ((0.025) / (0.5 / 8.));
+                dotci += (cpv - cc) / (0.1) / 2. * ((0.00126) / (0.5 / 8.));
+            } else {
+                ((TF *)a.sb)[e] = dotci;
+            }
+        }
+        if (ii >= 2 || schemeA) {
+            double cin = cc + dotci * dt;
+            if (cin < 0) cin = 0;
+            ci1[e] = cin;
+        }
+        cn1[e] = cn + dotcnsr * dt;
+        double n_catf = v_catf + f_tf * dt;
+        if (n_catf < 0) n_catf = 0;
+        double n_cats = v_cats + f_ts * dt;
+        if (n_cats < 0) n_cats = 0;
+        double n_cacal = v_cacal + f_cal * dt;
+        if (n_cacal < 0) n_cacal = 0;
+        double n_casr = v_casr + f_sr * dt;
+        if (n_casr < 0) n_casr = 0;
+        double n_camyo = v_camyo + f_myo * dt;
+        if (n_camyo < 0) n_camyo = 0;
+        double n_mgmyo = v_mgmyo + b_myomg<double>(n_camyo, v_mgmyo, dt) * dt;
+        if (n_mgmyo < 0) n_mgmyo = 0;
+        catf[e] = n_catf; cats[e] = n_cats; cacal[e] = n_cacal;
+        casr[e] = n_casr; camyo[e] = n_camyo; mgmyo[e] = n_mgmyo;
+        if (diag) { a.d_up[e] = xiup; a.d_leak[e] = xileak; }
+    } else {
+        /* FP32 production path: the same equations with divisions by
+         * constants replaced by reciprocal products; cnsr and slow troponin
+         * keep double storage */
+        const float dt = (float)a.dt;
+        const float sat = 0.95f / dt;
+        const float rIT = (float)(1.0 / ((0.33 * 2.) * 0.7)), rIL = (float)(1.0 / ((0.7 * 2.) * 0.7));
+        const float rNT = (float)(1.0 / (1.8 * 0.7)), rNL = (float)(1.0 / (6. * 0.7));
+        const float cf = (float)cc, cnf = (float)cn;
+        const float xiup = (float)C.sviup_tot * uptake_f(cf, cnf);
+        const float cn2 = cnf * cnf;
+        const float xileak = 1.035e-5f * (cnf - cf) * cn2 * __builtin_amdgcn_rcpf(cn2 + 250000.0f);
+        const float xicoupi = (((float)inN - cf) + ((float)inS - cf) + ((float)inE - cf) + ((float)inW - cf)) * rIT +
+                              (((float)inT - cf) + ((float)inB - cf)) * rIL;
+        const float xicoupn = ((float)(nnN - cn) + (float)(nnS - cn) + (float)(nnE - cn) + (float)(nnW - cn)) * rNT +
+                              ((float)(nnT - cn) + (float)(nnB - cn)) * rNL;
+        float dotcnsr = (xiup - xileak) * (float)((0.5 / 8.) / (0.025 / 8.)) + xicoupn;
+        const float v_catf = (float)catf[e], v_cacal = (float)cacal[e], v_casr = (float)casr[e];
+        const float v_camyo = (float)camyo[e], v_mgmyo = (float)mgmyo[e];
+        const TS v_cats = cats[e];
+        const float f_tf = fbuf(v_catf, cf, dt, sat, 0.0327f, 0.0196f, 70.0f);
+        const float f_ts = fbuf((float)v_cats, cf, dt, sat, 0.00254f, 0.000033f, 134.0f);
+        const float f_cal = fbuf(v_cacal, cf, dt, sat, 0.0543f, 0.238f, 24.0f);
+        const float f_sr = fbuf(v_casr, cf, dt, sat, 0.1f, 0.06f, 19.0f);
+        const float on_my = 13.8f * cf;
+        const float f_myo = (on_my * dt > 1.0f ? sat : on_my) * (140.0f - v_camyo - v_mgmyo) - 0.46f * v_camyo;
+        float dotci = (xileak - xiup) - (f_tf + f_ts + f_cal + f_sr + f_myo) + xicoupi;
+        if (ii < 2) {
+            const TS cj = ((const TS *)a.cjsr)[g];
+            dotcnsr += (float)((cj - cn) * TS(1.0 / (25.0 * 1.0 / 4.) / 2.)) * (float)((0.02 * 1) / (0.025 / 8.));
+            if (schemeA) {
+                const float csv = (float)((const TF *)a.cs[par])[g], cpv = (float)((const TF *)a.cp)[g];
+                dotci += (csv - cf) * (float)(1.0 / 0.04 / 2. * ((0.025) / (0.5 / 8.))) +
+                         (cpv - cf) * (float)(1.0 / 0.1 / 2. * ((0.00126) / (0.5 / 8.)));
+            } else {
+                ((TF *)a.sb)[e] = (TF)dotci;
+            }
+        }
+        if (ii >= 2 || schemeA) ci1[e] = (TF)fmaxf(cf + dotci * dt, 0.0f);
+        cn1[e] = cn + (TS)(dotcnsr * dt);
+        const float n_camyo = fmaxf(v_camyo + f_myo * dt, 0.0f);
+        const float on_mg = 0.0157f * 500.0f;
+        const float f_mg = (on_mg * dt > 1.0f ? sat : on_mg) * (140.0f - n_camyo - v_mgmyo) - 0.057f * v_mgmyo;
+        catf[e] = (TF)fmaxf(v_catf + f_tf * dt, 0.0f);
+        const TS n_cats = v_cats + (TS)(f_ts * dt);
+        cats[e] = n_cats < TS(0) ? TS(0) : n_cats;
+        cacal[e] = (TF)fmaxf(v_cacal + f_cal * dt, 0.0f);
+        casr[e] = (TF)fmaxf(v_casr + f_sr * dt, 0.0f);
+        camyo[e] = (TF)n_camyo;
+        mgmyo[e] = (TF)fmaxf(v_mgmyo + f_mg * dt, 0.0f);
+        if (diag) { a.d_up[e] = (double)xiup; a.d_leak[e] = (double)xileak; }
+    }
+}
+
+/* Cleft/sub-membrane sub-steps of the FP32 production path (scheme A
+ * cuda_Cell_Detailed.cu:1396-1464, scheme B :1479-1553), reciprocal
+ * constants and float arithmetic.  Under scheme B dotci01 enters with the
+ * cytosol kernel's partial rates of sub-voxels 0 and 1. */
+RYR_DEV void fast_cleft(bool schemeA, float dt, float cif0, float cif1, float csf0, float csu, float csd,
+                        float ica, float xna, float xre, float &cp, float &cs, float &sar, float &sarh, float &sarj,
+                        float &sarhj, float &dci0, float &dci1)
+{
     const float Vs_i = (float)((0.025) / (0.5 / 8.)), Vp_i = (float)((0.00126) / (0.5 / 8.));
     const float Vp_s = (float)((0.00126) / (0.025));
     const float r_tausi2 = (float)(1.0 / 0.04 / 2.), r_taupi2 = (float)(1.0 / 0.1 / 2.);
     const float r_taups = (float)(1.0 / 0.0283), r_taust = (float)(1.0 / 1.42);
-    const bool schemeA = a.scheme == 0;
-    float dotci01[2] = {0.0f, 0.0f};
-    float up_a = 0.0f, leak_a = 0.0f;
-#pragma unroll
-    for (int ii = 0; ii < 8; ++ii) {
-        const long long pl = ii * P;
-        const float cc = (float)cil[ii];
-        const TS cn = cnl[ii];
-        const float cnf = (float)cn;
-        const float xiup = sviup * uptake_f(cc, cnf);
-        const float cn2 = cnf * cnf;
-        const float xileak = 1.035e-5f * (cnf - cc) * cn2 * __builtin_amdgcn_rcpf(cn2 + 250000.0f);
-        up_a += xiup;
-        leak_a += xileak;
-        float inN, inS, inE, inW, inT, inB;
-        TS nnN, nnS, nnE, nnW, nnT, nnB;
-        if (ii & 1) {
-            const bool in = k + 1 < Z;
-            inN = in ? (float)ci0[(ii - 1) * P + g + sz] : cc;
-            nnN = in ? cn0[(ii - 1) * P + g + sz] : cn;
-            inS = (float)cil[ii - 1]; nnS = cnl[ii - 1];
-        } else {
-            const bool in = k > 0;
-            inN = (float)cil[ii + 1]; nnN = cnl[ii + 1];
-            inS = in ? (float)ci0[(ii + 1) * P + g - sz] : cc;
-            nnS = in ? cn0[(ii + 1) * P + g - sz] : cn;
-        }
-        if (ii & 2) {
-            const bool in = j + 1 < Y;
-            inE = in ? (float)ci0[(ii - 2) * P + g + sy] : cc;
-            nnE = in ? cn0[(ii - 2) * P + g + sy] : cn;
-            inW = (float)cil[ii - 2]; nnW = cnl[ii - 2];
-        } else {
-            const bool in = j > 0;
-            inE = (float)cil[ii + 2]; nnE = cnl[ii + 2];
-            inW = in ? (float)ci0[(ii + 2) * P + g - sy] : cc;
-            nnW = in ? cn0[(ii + 2) * P + g - sy] : cn;
-        }
-        if (ii & 4) {
-            const bool in = i + 1 < X;
-            inT = in ? (float)ci0[(ii - 4) * P + g + 1] : cc;
-            nnT = in ? cn0[(ii - 4) * P + g + 1] : cn;
-            inB = (float)cil[ii - 4]; nnB = cnl[ii - 4];
-        } else {
-            const bool in = i > 0;
-            inT = (float)cil[ii + 4]; nnT = cnl[ii + 4];
-            inB = in ? (float)ci0[(ii + 4) * P + g - 1] : cc;
-            nnB = in ? cn0[(ii + 4) * P + g - 1] : cn;
-        }
-        const float xicoupi = ((inN - cc) + (inS - cc) + (inE - cc) + (inW - cc)) * rIT +
-                              ((inT - cc) + (inB - cc)) * rIL;
-        const float xicoupn = ((float)(nnN - cn) + (float)(nnS - cn) + (float)(nnE - cn) + (float)(nnW - cn)) * rNT +
-                              ((float)(nnT - cn) + (float)(nnB - cn)) * rNL;
-        float dotcnsr = (xiup - xileak) * Vi_nsr + xicoupn;
-        const float v_catf = (float)catf[pl + g], v_cacal = (float)cacal[pl + g], v_casr = (float)casr[pl + g];
-        const float v_camyo = (float)camyo[pl + g], v_mgmyo = (float)mgmyo[pl + g];
-        const TS v_cats = cats[pl + g];
-        const float f_tf = fbuf(v_catf, cc, dt, sat, 0.0327f, 0.0196f, 70.0f);
-        const float f_ts = fbuf((float)v_cats, cc, dt, sat, 0.00254f, 0.000033f, 134.0f);
-        const float f_cal = fbuf(v_cacal, cc, dt, sat, 0.0543f, 0.238f, 24.0f);
-        const float f_sr = fbuf(v_casr, cc, dt, sat, 0.1f, 0.06f, 19.0f);
-        const float freec = 140.0f - v_camyo - v_mgmyo;
-        const float on_my = 13.8f * cc;
-        const float f_myo = (on_my * dt > 1.0f ? sat : on_my) * freec - 0.46f * v_camyo;
-        float dotci = (xileak - xiup) - (f_tf + f_ts + f_cal + f_sr + f_myo) + xicoupi;
-        if (ii < 2) {
-            dotcnsr += (float)(ii == 0 ? diffjn0 : diffjn1) * Vjsr_nsr;
-            const float cif = ii == 0 ? cif0 : cif1;
-            if (schemeA) dotci += (csf0 - cif) * r_tausi2 * Vs_i + (cpf0 - cif) * r_taupi2 * Vp_i;
-            dotci01[ii] = dotci;
-        }
-        if (ii >= 2 || schemeA) ci1[pl + g] = (TF)fmaxf(cc + dotci * dt, 0.0f);
-        cn1[pl + g] = cn + (TS)(dotcnsr * dt);
-        const float n_camyo = fmaxf(v_camyo + f_myo * dt, 0.0f);
-        const float on_mg = 0.0157f * 500.0f;
-        const float f_mg = (on_mg * dt > 1.0f ? sat : on_mg) * (140.0f - n_camyo - v_mgmyo) - 0.057f * v_mgmyo;
-        catf[pl + g] = (TF)fmaxf(v_catf + f_tf * dt, 0.0f);
-        TS n_cats = v_cats + (TS)(f_ts * dt);
-        cats[pl + g] = n_cats < TS(0) ? TS(0) : n_cats;
-        cacal[pl + g] = (TF)fmaxf(v_cacal + f_cal * dt, 0.0f);
-        casr[pl + g] = (TF)fmaxf(v_casr + f_sr * dt, 0.0f);
-        camyo[pl + g] = (TF)n_camyo;
-        mgmyo[pl + g] = (TF)fmaxf(v_mgmyo + f_mg * dt, 0.0f);
-    }
-    up_acc = (double)up_a * 0.125;
-    leak_acc = (double)leak_a * 0.125;
-
     const float dth = dt * 0.2f;
     const float sath = 0.95f / dth;
-    const float ica = (float)icat, xna = (float)xinaca, xre = (float)xire;
-    const float csu = (float)cs_up, csd = (float)cs_dn;
-    float cp = cpf0, cs = csf0;
-    float sar = (float)v_casar, sarh = (float)v_casarh, sarj = (float)v_casarj, sarhj = (float)v_casarhj;
+    const float cpf0 = cp;
     if (schemeA) {
         const float csdiff = (csu + csd - 2.0f * csf0) * r_taust;
         const float to_ci = ((csf0 - cif0) + (csf0 - cif1)) * r_tausi2;
@@ -689,8 +881,8 @@ RYR_DEV void fast_cytosol_and_cleft(const RyrKArgs &a, long long g, long long P,
         const float csum = csu + csd;
 #pragma unroll
         for (int it = 0; it < 5; ++it) {
-            dotci01[0] += ((cs - cif0) * r_tausi2 * Vs_i + (cp - cif0) * r_taupi2 * Vp_i) * 0.2f;
-            dotci01[1] += ((cs - cif1) * r_tausi2 * Vs_i + (cp - cif1) * r_taupi2 * Vp_i) * 0.2f;
+            dci0 += ((cs - cif0) * r_tausi2 * Vs_i + (cp - cif0) * r_taupi2 * Vp_i) * 0.2f;
+            dci1 += ((cs - cif1) * r_tausi2 * Vs_i + (cp - cif1) * r_taupi2 * Vp_i) * 0.2f;
             const float fsar = fbuf(sar, cs, dth, sath, 0.1f, 1.3f, (float)RYR_BSAR);
             const float fsarh = fbuf(sarh, cs, dth, sath, 0.1f, 0.03f, (float)RYR_BSARH);
             const float dotcs = Vp_s * (cp - 0.5f * 0.0283f * ica - cs) * r_taups + xna -
@@ -706,12 +898,7 @@ RYR_DEV void fast_cytosol_and_cleft(const RyrKArgs &a, long long g, long long P,
             sarhj = fmaxf(sarhj + cpbuffh * dth, 0.0f);
             cp += cpdot * dth;
         }
-        ci1[g] = (TF)fmaxf(cif0 + dotci01[0] * dt, 0.0f);
-        ci1[P + g] = (TF)fmaxf(cif1 + dotci01[1] * dt, 0.0f);
     }
-    cpv = (TF)cp;
-    csn = (TF)cs;
-    v_casar = (TF)sar; v_casarh = (TF)sarh; v_casarj = (TF)sarj; v_casarhj = (TF)sarhj;
 }
 
 /* ------------------------------------------------------------ step kernel */
@@ -719,7 +906,6 @@ template <typename TF, typename TS>
 __global__ __launch_bounds__(RYR_BLOCK) void k_cru_step(RyrKArgs a, double stim_amp)
 {
     __shared__ RyrDD s_red[3][RYR_BLOCK / 64];
-    __shared__ int s_last;
 
     const int bid = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     const int c = a.block_cell[bid];
@@ -756,10 +942,10 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_cru_step(RyrKArgs a, double stim_
         TS *cjsrp = (TS *)a.cjsr, *cjsrbp = (TS *)a.cjsrb;
 
         const double v = M->m[0];
-        TF cil[8];
-        TS cnl[8];
+        TF cil[2];
+        TS cnl[2];
 #pragma unroll
-        for (int ii = 0; ii < 8; ++ii) { cil[ii] = ci0[ii * P + g]; cnl[ii] = cn0[ii * P + g]; }
+        for (int ii = 0; ii < 2; ++ii) { cil[ii] = ci0[ii * P + g]; cnl[ii] = cn0[ii * P + g]; }
         const TF csv = cs0[g];
         TF cpv = cpp[g];
         const TS cjsrv = cjsrp[g];
@@ -867,120 +1053,30 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_cru_step(RyrKArgs a, double stim_
             cjsrb_new = (TS)((double)cjsrbp[g] + (1 - beta) * flux * dt);
         }
 
-        /* cytosol/NSR diffusion, buffers, updates (:1305-1393 / :1557-1586) and
-         * the cleft/sub-membrane sub-steps (:1356-1473 / :1475-1597) */
+        /* cleft/sub-membrane sub-steps (:1356-1473 / :1475-1597); the
+         * cytosol/NSR sub-voxels were advanced by k_cyto */
         TF v_casar = casar[g], v_casarh = casarh[g], v_casarj = casarj[g], v_casarhj = casarhj[g];
         const TF cs_up = (k + 1 < Z) ? cs0[g + sz] : csv;
         const TF cs_dn = (k > 0) ? cs0[g - sz] : csv;
         const TF icatT = (TF)icat1, xinacaT = (TF)xinaca, xireT = (TF)xire;
         TF csn = csv;
-        double up_acc = 0.0, leak_acc = 0.0;
+        const TF *sbp = (const TF *)a.sb;
         if constexpr (sizeof(TF) != 8) {
-            fast_cytosol_and_cleft(a, g, P, i, j, k, X, Y, Z, sy, sz, cil, cnl, ci0, cn0, ci1, cn1, (float)dt,
-                                   (float)C.sviup_tot, diffjn0, diffjn1, cpv, csv, cs_up, cs_dn, icatT, xinacaT,
-                                   xireT, v_casar, v_casarh, v_casarj, v_casarhj, csn, up_acc, leak_acc);
+            float dci[2] = {0.0f, 0.0f};
+            if (a.scheme != 0) { dci[0] = (float)sbp[g]; dci[1] = (float)sbp[P + g]; }
+            float cpf = (float)cpv, csf = (float)csv;
+            float sar = (float)v_casar, sarh = (float)v_casarh, sarj = (float)v_casarj, sarhj = (float)v_casarhj;
+            fast_cleft(a.scheme == 0, (float)dt, (float)cil[0], (float)cil[1], (float)csv, (float)cs_up,
+                       (float)cs_dn, (float)icat1, (float)xinaca, (float)xire, cpf, csf, sar, sarh, sarj, sarhj, dci[0], dci[1]);
+            if (a.scheme != 0) {
+                ci1[g] = (TF)fmaxf((float)cil[0] + dci[0] * (float)dt, 0.0f);
+                ci1[P + g] = (TF)fmaxf((float)cil[1] + dci[1] * (float)dt, 0.0f);
+            }
+            cpv = (TF)cpf; csn = (TF)csf;
+            v_casar = (TF)sar; v_casarh = (TF)sarh; v_casarj = (TF)sarj; v_casarhj = (TF)sarhj;
         } else {
-        const TF diffpi0 = (cpT - cil[0]) / TF(0.1) / TF(2.);
-        const TF diffpi1 = (cpT - cil[1]) / TF(0.1) / TF(2.);
-        const TF diffsi0 = (csT - cil[0]) / TF(0.04) / TF(2.);
-        const TF diffsi1 = (csT - cil[1]) / TF(0.04) / TF(2.);
-        const TF sviup = (TF)C.sviup_tot;
-        TF dotci01[2];
-#pragma unroll
-        for (int ii = 0; ii < 8; ++ii) {
-            const long long pl = ii * P;
-            TF xiup;
-            if (sizeof(TF) == 8) xiup = (TF)((double)sviup * uptake_d((double)cil[ii], (double)cnl[ii]));
-            else xiup = sviup * (TF)uptake_f((float)cil[ii], (float)cnl[ii]);
-            const TF xileak = TF(1.0) * leak_t<TF>((TF)cnl[ii], cil[ii]);
-            up_acc = up_acc + (double)xiup / 8.;
-            leak_acc = leak_acc + (double)xileak / 8.;
-            /* neighbours on the fine grid (bit0 z, bit1 y, bit2 x) */
-            TF inN, inS, inE, inW, inT, inB;
-            TS nnN, nnS, nnE, nnW, nnT, nnB;
-            if (ii & 1) {
-                bool in = k + 1 < Z;
-                inN = nb_load<TF>(ci0, (ii - 1) * P + g + sz, in, cil[ii]);
-                nnN = nb_load<TS>(cn0, (ii - 1) * P + g + sz, in, cnl[ii]);
-                inS = cil[ii - 1]; nnS = cnl[ii - 1];
-            } else {
-                bool in = k > 0;
-                inN = cil[ii + 1]; nnN = cnl[ii + 1];
-                inS = nb_load<TF>(ci0, (ii + 1) * P + g - sz, in, cil[ii]);
-                nnS = nb_load<TS>(cn0, (ii + 1) * P + g - sz, in, cnl[ii]);
-            }
-            if (ii & 2) {
-                bool in = j + 1 < Y;
-                inE = nb_load<TF>(ci0, (ii - 2) * P + g + sy, in, cil[ii]);
-                nnE = nb_load<TS>(cn0, (ii - 2) * P + g + sy, in, cnl[ii]);
-                inW = cil[ii - 2]; nnW = cnl[ii - 2];
-            } else {
-                bool in = j > 0;
-                inE = cil[ii + 2]; nnE = cnl[ii + 2];
-                inW = nb_load<TF>(ci0, (ii + 2) * P + g - sy, in, cil[ii]);
-                nnW = nb_load<TS>(cn0, (ii + 2) * P + g - sy, in, cnl[ii]);
-            }
-            if (ii & 4) {
-                bool in = i + 1 < X;
-                inT = nb_load<TF>(ci0, (ii - 4) * P + g + 1, in, cil[ii]);
-                nnT = nb_load<TS>(cn0, (ii - 4) * P + g + 1, in, cnl[ii]);
-                inB = cil[ii - 4]; nnB = cnl[ii - 4];
-            } else {
-                bool in = i > 0;
-                inT = cil[ii + 4]; nnT = cnl[ii + 4];
-                inB = nb_load<TF>(ci0, (ii + 4) * P + g - 1, in, cil[ii]);
-                nnB = nb_load<TS>(cn0, (ii + 4) * P + g - 1, in, cnl[ii]);
-            }
-            const TF cc = cil[ii];
-            const TS cn = cnl[ii];
-            const TS xicoupn = ((nnN - cn) / TS(1.8 * 0.7) + (nnS - cn) / TS(1.8 * 0.7) +
-                                (nnE - cn) / TS(1.8 * 0.7) + (nnW - cn) / TS(1.8 * 0.7) +
-                                (nnT - cn) / TS(6. * 0.7) + (nnB - cn) / TS(6. * 0.7));
-            const TF xicoupi = ((inN - cc) / TF((0.33 * 2.) * 0.7) + (inS - cc) / TF((0.33 * 2.) * 0.7) +
-                                (inE - cc) / TF((0.33 * 2.) * 0.7) + (inW - cc) / TF((0.33 * 2.) * 0.7) +
-                                (inT - cc) / TF((0.7 * 2.) * 0.7) + (inB - cc) / TF((0.7 * 2.) * 0.7));
-            TS dotcnsr = (TS)(xiup - xileak) * TS(0.5 / 8.) / TS(0.025 / 8.) + xicoupn;
-            const TF v_catf = catf[pl + g], v_cacal = cacal[pl + g], v_casr = casr[pl + g];
-            const TF v_camyo = camyo[pl + g], v_mgmyo = mgmyo[pl + g];
-            const TS v_cats = cats[pl + g];
-            const TF f_tf = b_tropf<TF>(v_catf, cc, dtT);
-            const TS f_ts = b_trops<TS>(v_cats, (TS)cc, (TS)dtT);
-            const TF f_cal = b_bucal<TF>(v_cacal, cc, dtT);
-            const TF f_sr = b_busr<TF>(v_casr, cc, dtT);
-            const TF f_myo = b_myoca<TF>(v_camyo, v_mgmyo, cc, dtT);
-            const TF buffers = (f_tf + (TF)f_ts + f_cal + f_sr + f_myo);
-            TF dotci = (-xiup + xileak) - buffers + xicoupi;
-            if (ii < 2) {
-                dotcnsr += (ii == 0 ? diffjn0 : diffjn1) * TS(0.02 * 1) / TS(0.025 / 8.);
-                if (a.scheme == 0) {
-                    dotci += (ii == 0 ? diffsi0 : diffsi1) * TF((0.025) / (0.5 / 8.));
-                    dotci += (ii == 0 ? diffpi0 : diffpi1) * TF((0.00126) / (0.5 / 8.));
-                }
-                dotci01[ii] = dotci;
-            }
-            if (ii >= 2 || a.scheme == 0) {
-                TF cin = cc + dotci * dtT;
-                if (cin < TF(0)) cin = TF(0);
-                ci1[pl + g] = cin;
-            }
-            cn1[pl + g] = cn + dotcnsr * (TS)dtT;
-            TF n_catf = v_catf + f_tf * dtT;
-            if (n_catf < TF(0)) n_catf = TF(0);
-            TS n_cats = v_cats + f_ts * (TS)dtT;
-            if (n_cats < TS(0)) n_cats = TS(0);
-            TF n_cacal = v_cacal + f_cal * dtT;
-            if (n_cacal < TF(0)) n_cacal = TF(0);
-            TF n_casr = v_casr + f_sr * dtT;
-            if (n_casr < TF(0)) n_casr = TF(0);
-            TF n_camyo = v_camyo + f_myo * dtT;
-            if (n_camyo < TF(0)) n_camyo = TF(0);
-            TF n_mgmyo = v_mgmyo + b_myomg<TF>(n_camyo, v_mgmyo, dtT) * dtT;
-            if (n_mgmyo < TF(0)) n_mgmyo = TF(0);
-            catf[pl + g] = n_catf; cats[pl + g] = n_cats; cacal[pl + g] = n_cacal;
-            casr[pl + g] = n_casr; camyo[pl + g] = n_camyo; mgmyo[pl + g] = n_mgmyo;
-        }
-
-        /* cleft / sub-membrane sub-steps */
+        TF dotci01[2] = {TF(0), TF(0)};
+        if (a.scheme != 0) { dotci01[0] = sbp[g]; dotci01[1] = sbp[P + g]; }
         const TF dth = dtT / TF(5);
         if (a.scheme == 0) {
             /* scheme A (:1396-1464) */
@@ -1075,8 +1171,6 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_cru_step(RyrKArgs a, double stim_
         if (diag) {
             a.d_xire[g] = xire;
             a.d_xinaca[g] = xinaca;
-            a.d_up[g] = up_acc;
-            a.d_leak[g] = leak_acc;
         }
         if (nfail | nclampcp | nclampcs) {
             unsigned long long *cnt = a.counters + (long long)c * RYR_NCNT;
@@ -1090,7 +1184,7 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_cru_step(RyrKArgs a, double stim_
     }
 
     /* whole-cell sums: pairwise double-double tree over this block's 256
-     * CRUs (index order), one partial per block */
+     * CRUs (index order), one partial per block; k_membrane adds them */
     {
         const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
         RyrDD t0 = {r_icat, 0.0}, t1 = {r_inaca, 0.0}, t2 = {r_cs, 0.0};
@@ -1104,38 +1198,67 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_cru_step(RyrKArgs a, double stim_
 #pragma unroll
             for (int q = 0; q < 3; ++q)
                 pp[q] = dd_add(dd_add(s_red[q][0], s_red[q][1]), dd_add(s_red[q][2], s_red[q][3]));
-            __threadfence();
-            unsigned int prev = atomicAdd(a.arrive + c, 1u);
-            s_last = (prev == (unsigned int)(C.nblocks - 1));
         }
+    }
+}
+
+/* ------------------------------------------------------------ membrane kernel */
+/* One workgroup per cell, after k_cru_step: the chunk partials of the three
+ * whole-cell sums (pairwise trees over groups of 256 chunks, groups in order),
+ * then the membrane ODE on wavefront 0, the clamp protocol, the beat tracker,
+ * the membrane columns of an output row and the V-only factors of the next
+ * step (cuda_Cell_Detailed.cu:430-739). */
+template <typename TF, typename TS>
+__global__ __launch_bounds__(RYR_BLOCK) void k_membrane(RyrKArgs a, double stim_amp)
+{
+    __shared__ RyrDD s_red[3][RYR_BLOCK / 64];
+    const int c = blockIdx.x;
+    const RyrCellInfo C = a.cells[c];
+    RyrMem *M = a.mem + c;
+    const long long nstep = M->step;
+    const int spo = a.spo;
+    const bool diag = (nstep % spo) == 0;
+    const double dt = a.dt;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    RyrDD acc[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    const RyrDD *pp = a.partial + 3LL * C.first_block;
+    for (int g0 = 0; g0 < C.nblocks; g0 += RYR_BLOCK) {
+        const int bb = g0 + (int)threadIdx.x;
+        RyrDD t[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            if (bb < C.nblocks) t[q] = pp[3LL * bb + q];
+            else { t[q].hi = 0.0; t[q].lo = 0.0; }
+            t[q] = wave_tree(t[q]);
+        }
+        if (lane == 0) { s_red[0][wv] = t[0]; s_red[1][wv] = t[1]; s_red[2][wv] = t[2]; }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+            acc[q] = dd_add(acc[q], dd_add(dd_add(s_red[q][0], s_red[q][1]), dd_add(s_red[q][2], s_red[q][3])));
         __syncthreads();
     }
-    if (!s_last) return;
+    if (wv != 0) return;
 
-    /* last block of the cell: chunk partials in order, membrane ODE, factors
-     * for the next step (cuda_Cell_Detailed.cu:430-739) */
-    if (threadIdx.x == 0) {
-        __threadfence();
-        RyrDD s0 = {0.0, 0.0}, s1 = {0.0, 0.0}, s2 = {0.0, 0.0};
-        const RyrDD *pp = a.partial + 3LL * C.first_block;
-        for (int b = 0; b < C.nblocks; ++b) {
-            s0 = dd_add(s0, pp[3 * b + 0]);
-            s1 = dd_add(s1, pp[3 * b + 1]);
-            s2 = dd_add(s2, pp[3 * b + 2]);
-        }
-        const double stim = stim_of(C, nstep, stim_amp);
+    const double stim = stim_of(C, nstep, stim_amp);
+    double mo[13], cu[10];
+    membrane_wave(M->m, C.svikr, C.Cm, C.dN, acc[0].hi, acc[1].hi, acc[2].hi, dt, stim, lane, mo, cu);
+    double vnew = mo[0];
+    if (C.vclamp_n > 0) {
+        /* clamp protocols (cuda_Cell_Detailed.cu:733-773) */
+        long long q = C.vclamp_periodic ? nstep % C.vclamp_n : (nstep < C.vclamp_n ? nstep : C.vclamp_n - 1);
+        vnew = C.vclamp[q];
+    }
+    const double xnai_new = mo[12];
+    if (lane == 0) {
         const double vold = M->m[0];
-        membrane_step(M, C, s0.hi, s1.hi, s2.hi, dt, stim);
-        if (C.vclamp_n > 0) {
-            /* clamp protocols (cuda_Cell_Detailed.cu:733-773) */
-            long long q = C.vclamp_periodic ? nstep % C.vclamp_n : (nstep < C.vclamp_n ? nstep : C.vclamp_n - 1);
-            M->m[0] = C.vclamp[q];
-        }
+        for (int q = 0; q < 13; ++q) M->m[q] = mo[q];
+        M->m[0] = vnew;
+        for (int q = 0; q < 10; ++q) M->cur[q] = cu[q];
         M->stim = stim;
         M->out_ica += M->cur[1];
         M->out_ncx += M->cur[2];
         M->out_ina += M->cur[3];
-        const double vnew = M->m[0];
         beat_track(M, C, nstep, dt, vold, vnew, a.beats + (long long)c * RYR_MAXB * RYR_BCOLS);
         if (!(vnew == vnew) || vnew == __builtin_inf() || vnew == -__builtin_inf())
             atomicAdd(a.counters + (long long)c * RYR_NCNT + 4, 1ull);
@@ -1151,11 +1274,9 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_cru_step(RyrKArgs a, double stim_
             M->out_ica = 0; M->out_ncx = 0; M->out_ina = 0;
             a.diag_step[c] = nstep;
         }
-        vfactors(M, a.iso2);
-        M->step = nstep + 1;
-        a.arrive[c] = 0u;
-        __threadfence();
     }
+    vfactors_wave(M, a.iso2, lane, vnew, xnai_new);
+    if (lane == 0) M->step = nstep + 1;
 }
 
 /* ------------------------------------------------------- output-row kernel */
@@ -1215,7 +1336,7 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_diag(RyrKArgs a)
         val[8] = (double)casarh[g];
         val[9] = (double)casarj[g];
         val[10] = (double)casarhj[g];
-        double s_ci = 0, s_cn = 0, s_tf = 0, s_ts = 0, s_sr = 0, s_my = 0, s_cal = 0;
+        double s_ci = 0, s_cn = 0, s_tf = 0, s_ts = 0, s_sr = 0, s_my = 0, s_cal = 0, s_lk = 0, s_up = 0;
         for (int ii = 0; ii < 8; ++ii) {
             const long long e = ii * P + g;
             s_ci = s_ci + (double)ci[e] / 8.;
@@ -1225,12 +1346,14 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_diag(RyrKArgs a)
             s_sr = s_sr + (double)casr[e] / 8.;
             s_my = s_my + (double)camyo[e] / 8.;
             s_cal = s_cal + (double)cacal[e] / 8.;
+            s_lk = s_lk + a.d_leak[e] / 8.;
+            s_up = s_up + a.d_up[e] / 8.;
             if ((double)ci[e] > 1000) ic[12]++;
         }
         val[11] = s_ci; val[12] = s_cn; val[13] = s_tf; val[14] = s_ts; val[15] = s_sr;
         val[16] = s_my; val[17] = s_cal;
-        val[18] = a.d_leak[g];
-        val[19] = a.d_up[g];
+        val[18] = s_lk;
+        val[19] = s_up;
         val[20] = a.d_xire[g];
         ic[0] += nou; ic[1] += nob; ic[2] += ncu; ic[3] += ncb;
         ic[11] += (ax >> 16) & 1u;
@@ -1282,18 +1405,32 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_diag(RyrKArgs a)
         s_last = (prev == (unsigned int)(C.nblocks - 1));
     }
     __syncthreads();
-    if (!s_last || threadIdx.x != 0) return;
-
+    if (!s_last) return;
+    __threadfence();
+    /* chunk sums: pairwise trees over groups of 256 chunks, groups in order */
+    RyrDD acc[RYR_DIAG_Q];
+#pragma unroll
+    for (int q = 0; q < RYR_DIAG_Q; ++q) { acc[q].hi = 0.0; acc[q].lo = 0.0; }
+    for (int g0 = 0; g0 < C.nblocks; g0 += RYR_BLOCK) {
+        const int bb = g0 + (int)threadIdx.x;
+        const RyrDD *pp = a.dpartial + (long long)(C.first_block + bb) * RYR_DIAG_Q;
+#pragma unroll
+        for (int q = 0; q < RYR_DIAG_Q; ++q) {
+            RyrDD t;
+            if (bb < C.nblocks) t = pp[q];
+            else { t.hi = 0.0; t.lo = 0.0; }
+            t = wave_tree(t);
+            if (lane == 0) s_red[q][wv] = t;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < RYR_DIAG_Q; ++q)
+            acc[q] = dd_add(acc[q], dd_add(dd_add(s_red[q][0], s_red[q][1]), dd_add(s_red[q][2], s_red[q][3])));
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
     {
         RYR_STRICT_FP
-        __threadfence();
-        RyrDD acc[RYR_DIAG_Q];
-#pragma unroll
-        for (int q = 0; q < RYR_DIAG_Q; ++q) { acc[q].hi = 0.0; acc[q].lo = 0.0; }
-        const RyrDD *pp = a.dpartial + (long long)C.first_block * RYR_DIAG_Q;
-        for (int bb = 0; bb < C.nblocks; ++bb)
-#pragma unroll
-            for (int q = 0; q < RYR_DIAG_Q; ++q) acc[q] = dd_add(acc[q], pp[(long long)bb * RYR_DIAG_Q + q]);
         unsigned long long *cc = a.dcount + (long long)c * RYR_DIAG_I;
         unsigned long long cnt[RYR_DIAG_I];
         for (int q = 0; q < RYR_DIAG_I; ++q) { cnt[q] = __hip_atomic_load(cc + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); cc[q] = 0ull; }
@@ -1383,7 +1520,7 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_init(RyrKArgs a, const RyrInit *_
     a.lccw[g] = I.lcc0;
     a.ryrw[g] = (uint32_t)0 | ((uint32_t)I.ncu0 << 8) | ((uint32_t)0 << 16) | ((uint32_t)I.ncb0 << 24);
     a.aux[g] = 0u;
-    a.d_xire[g] = 0.0; a.d_xinaca[g] = 0.0; a.d_up[g] = 0.0; a.d_leak[g] = 0.0;
+    a.d_xire[g] = 0.0; a.d_xinaca[g] = 0.0;
 }
 
 /* Non-template kernels: defined in exactly one unit (the API). */

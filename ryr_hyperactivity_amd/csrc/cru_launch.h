@@ -7,7 +7,7 @@
 #include "cru_device.h"
 
 #define RYR_DECLARE_LAUNCHERS(sfx)                                                              \
-    hipError_t ryr_launch_step_##sfx(const RyrKArgs &a, double amp, int nblocks, hipStream_t s); \
+    hipError_t ryr_launch_step_##sfx(const RyrKArgs &a, double amp, int nblocks, int ncells, hipStream_t s); \
     hipError_t ryr_launch_diag_##sfx(const RyrKArgs &a, int ncells, hipStream_t s);              \
     hipError_t ryr_launch_init_##sfx(const RyrKArgs &a, const RyrInit *init, int nblocks,       \
                                      hipStream_t s);                                            \
@@ -20,9 +20,12 @@ RYR_DECLARE_LAUNCHERS(f64)
 RYR_DECLARE_LAUNCHERS(f32)
 
 #define RYR_DEFINE_LAUNCHERS(sfx, TF, TS)                                                       \
-    hipError_t ryr_launch_step_##sfx(const RyrKArgs &a, double amp, int nblocks, hipStream_t s) \
+    hipError_t ryr_launch_step_##sfx(const RyrKArgs &a, double amp, int nblocks, int ncells,    \
+                                     hipStream_t s)                                             \
     {                                                                                           \
+        hipLaunchKernelGGL((k_cyto<TF, TS>), dim3(nblocks, 8), dim3(RYR_BLOCK), 0, s, a);         \
         hipLaunchKernelGGL((k_cru_step<TF, TS>), dim3(nblocks), dim3(RYR_BLOCK), 0, s, a, amp); \
+        hipLaunchKernelGGL((k_membrane<TF, TS>), dim3(ncells), dim3(RYR_BLOCK), 0, s, a, amp);  \
         return hipGetLastError();                                                               \
     }                                                                                           \
     hipError_t ryr_launch_diag_##sfx(const RyrKArgs &a, int ncells, hipStream_t s)              \

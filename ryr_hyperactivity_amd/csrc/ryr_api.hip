@@ -236,8 +236,9 @@ extern "C" int ryr_create(int device, const ryr_params *p, const ryr_cell_desc *
     ALLOC(a.cp, P * tf); ALLOC(a.cjsr, P * ts); ALLOC(a.cjsrb, P * ts); ALLOC(a.Ancx, P * tf);
     ALLOC(a.casar, P * tf); ALLOC(a.casarh, P * tf); ALLOC(a.casarj, P * tf); ALLOC(a.casarhj, P * tf);
     ALLOC(a.lccw, P * 4); ALLOC(a.ryrw, P * 4); ALLOC(a.aux, P * 4);
-    ALLOC(a.d_xire, P * 8); ALLOC(a.d_xinaca, P * 8); ALLOC(a.d_up, P * 8); ALLOC(a.d_leak, P * 8);
-    RyrCellInfo *d_cells; RyrMem *d_mem; int *d_bc; RyrDD *d_part; unsigned int *d_arr;
+    ALLOC(a.d_xire, P * 8); ALLOC(a.d_xinaca, P * 8); ALLOC(a.d_up, 8 * P * 8); ALLOC(a.d_leak, 8 * P * 8);
+    ALLOC(a.sb, 2 * P * tf);
+    RyrCellInfo *d_cells; RyrMem *d_mem; int *d_bc; RyrDD *d_part;
     unsigned long long *d_cnt; double *d_diag; long long *d_dstep; double *d_line; long long *d_loff;
     ALLOC(d_mem, sizeof(RyrMem) * ncells);
     ALLOC(d_cnt, sizeof(unsigned long long) * ncells * RYR_NCNT);
@@ -247,7 +248,6 @@ extern "C" int ryr_create(int device, const ryr_params *p, const ryr_cell_desc *
     ALLOC(d_cells, sizeof(RyrCellInfo) * ncells);
     ALLOC(d_bc, sizeof(int) * h->nblocks);
     ALLOC(d_part, sizeof(RyrDD) * 3 * h->nblocks);
-    ALLOC(d_arr, sizeof(unsigned int) * ncells);
     ALLOC(d_loff, sizeof(long long) * ncells);
     ALLOC(a.beats, sizeof(double) * ncells * RYR_MAXB * RYR_BCOLS);
     ALLOC(a.dpartial, sizeof(RyrDD) * RYR_DIAG_Q * h->nblocks);
@@ -255,7 +255,7 @@ extern "C" int ryr_create(int device, const ryr_params *p, const ryr_cell_desc *
     ALLOC(a.darrive, sizeof(unsigned int) * ncells);
 #undef ALLOC
     h->vclamp.assign(ncells, nullptr);
-    a.cells = d_cells; a.mem = d_mem; a.block_cell = d_bc; a.partial = d_part; a.arrive = d_arr;
+    a.cells = d_cells; a.mem = d_mem; a.block_cell = d_bc; a.partial = d_part;
     a.counters = d_cnt; a.diag = d_diag; a.diag_step = d_dstep; a.line = d_line; a.line_off = d_loff;
 
     auto H2D = [&](void *dst, const void *src, size_t n) {
@@ -334,8 +334,8 @@ extern "C" int ryr_create(int device, const ryr_params *p, const ryr_cell_desc *
 static hipError_t launch_one(ryr_handle *h, long long s)
 {
     const bool f32 = h->p.precision == RYR_PREC_F32;
-    hipError_t e = f32 ? ryr_launch_step_f32(h->a, h->p.stim_amp, h->nblocks, h->stream)
-                       : ryr_launch_step_f64(h->a, h->p.stim_amp, h->nblocks, h->stream);
+    hipError_t e = f32 ? ryr_launch_step_f32(h->a, h->p.stim_amp, h->nblocks, h->ncells, h->stream)
+                       : ryr_launch_step_f64(h->a, h->p.stim_amp, h->nblocks, h->ncells, h->stream);
     if (e != hipSuccess) return e;
     if (s % h->p.steps_per_output == 0)
         e = f32 ? ryr_launch_diag_f32(h->a, h->nblocks, h->stream) : ryr_launch_diag_f64(h->a, h->nblocks, h->stream);
