@@ -31,15 +31,32 @@ HBM_PEAK_GBS = 8000.0
 
 
 def alg_bytes_per_cru_step(precision):
-    """Algorithmic HBM bytes of one interior CRU-step: every live state word
-    read once and written once (neighbour re-reads, ghost cells and the
-    output-step extras excluded).  FP32 mode: 55 float words (8 ci, 5x8 fast
-    buffers, cs, cp, Ancx, 4 sarcolemmal buffers) + 18 double words (8 cnsr,
-    8 slow troponin, cjsr, CSQN-bound) + 3 packed 32-bit gating/flag words.
-    FP64 mode: 73 double words + the 3 packed words."""
+    """Algorithmic HBM bytes of one interior CRU-step of the whole step: every
+    live state word read once and written once (neighbour re-reads, ghost
+    cells and the output-step extras excluded).  FP32 mode: 55 float words
+    (8 ci, 5x8 fast buffers, cs, cp, Ancx, 4 sarcolemmal buffers) + 18 double
+    words (8 cnsr, 8 slow troponin, cjsr, CSQN-bound) + 3 packed 32-bit
+    gating/flag words.  FP64 mode: 73 double words + the 3 packed words."""
     if precision == 1:
         return 2 * (55 * 4 + 18 * 8 + 3 * 4)
     return 2 * (73 * 8 + 3 * 4)
+
+
+def alg_bytes_per_kernel(precision):
+    """Algorithmic HBM bytes per interior CRU of each step kernel (scheme A),
+    every word the kernel must touch counted once.
+    k_cyto (sub-voxel kernel): reads 8 ci, 8 cnsr, 5x8 fast buffers, 8 slow
+    troponin, cjsr, cs, cp; writes 8 ci, 8 cnsr, 5x8 fast buffers, 8 slow
+    troponin.  FP32: 336 + 320 = 656 B; FP64: 536 + 512 = 1048 B.
+    k_cru_step (CRU kernel): reads ci/cnsr of sub-voxels 0-1, cs, cp, cjsr,
+    cjsrb, Ancx, 4 sarcolemmal buffers, 3 packed words; writes Ancx, the 4
+    buffers, cp, cs, cjsr, cjsrb, the packed words.  FP32: 80 + 56 = 136 B;
+    FP64: 116 + 84 = 200 B.  The two add up to the whole-step figure plus the
+    40 (FP64: 56) bytes both kernels read.
+    k_membrane: per cell, not per CRU (a few hundred bytes): 0 here."""
+    if precision == 1:
+        return {"k_cyto": 656, "k_cru_step": 136, "k_membrane": 0}
+    return {"k_cyto": 1048, "k_cru_step": 200, "k_membrane": 0}
 
 
 def make_cell(prec):
@@ -62,7 +79,11 @@ def make_batch(prec, ncells=256, seed=2024):
     return Params.default(precision=prec), cells
 
 
-def run_gpu(p, cells, steps, warmup):
+def run_gpu(p, cells, steps, warmup, prof_steps):
+    """Timed region: `steps` graph-replayed steps, bracketed by a device
+    synchronisation on both sides (wall clock) and by HIP events on the
+    engine's stream.  Then `prof_steps` more steps with every kernel stamped
+    by its own start/end events (the per-kernel durations of the roofline)."""
     from ryr_hyperactivity_amd import CellEngine
     ncru = sum(c.ncru for c in cells)
     with CellEngine(p, cells, device=0) as eng:
@@ -76,20 +97,46 @@ def run_gpu(p, cells, steps, warmup):
         ev_ms, launches = eng.kernel_time()
         v = eng.scalars()[:, 1]
         assert np.all(np.isfinite(v)), "non-finite membrane potential"
+        shape = eng.launch_shape()
+        prof = eng.profile_kernels(prof_steps)
     sim_ms = len(cells) * steps * p.dt
-    per_launch_s = ev_ms * 1e-3 / max(launches, 1)
-    bpl = ncru * alg_bytes_per_cru_step(p.precision)
+    step_s = ev_ms * 1e-3 / max(launches, 1)
+    bpc = alg_bytes_per_kernel(p.precision)
+    kernels = {}
+    for name, (us, cnt) in prof.items():
+        d = {"avg_us": us, "launches": cnt}
+        if name in bpc and bpc[name] and us > 0:
+            d["bytes_per_launch"] = ncru * bpc[name]
+            d["achieved_GBs"] = ncru * bpc[name] / (us * 1e-6) / 1e9
+        kernels[name] = d
+    # dominant kernel: largest share of the step time (k_diag runs once per output row)
+    spo = p.steps_per_output
+    share = {k: v["avg_us"] / (spo if k == "k_diag" else 1) for k, v in kernels.items()}
+    dom = max(share, key=share.get)
     return {
         "cell_ms_per_s": sim_ms / wall,
         "wall_s": wall,
         "ms_per_step": 1e3 * wall / steps,
-        "event_ms": ev_ms,
-        "launches": launches,
-        "kernel_us": per_launch_s * 1e6,
+        "event_ms_per_step": 1e3 * step_s,
         "ncru": ncru,
-        "bytes_per_launch": bpl,
-        "achieved_GBs": bpl / per_launch_s / 1e9,
+        "step_bytes": ncru * alg_bytes_per_cru_step(p.precision),
+        "step_GBs": ncru * alg_bytes_per_cru_step(p.precision) / step_s / 1e9,
+        "launch_shape": {"k_cyto": shape[0], "k_cru_step": shape[1]},
+        "kernels": kernels,
+        "dominant": dom,
     }
+
+
+def roofline(r, workload):
+    k = r["dominant"]
+    kd = r["kernels"][k]
+    return {"bound": "hbm", "achieved": kd.get("achieved_GBs"), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": (kd.get("achieved_GBs") or 0.0) / HBM_PEAK_GBS,
+            "traffic": traffic_from_profiles(k, workload),
+            "kernel": k, "kernel_us": kd["avg_us"], "bytes_per_launch": kd.get("bytes_per_launch"),
+            "step_GBs": r["step_GBs"], "step_frac": r["step_GBs"] / HBM_PEAK_GBS,
+            "kernels": {n: {"us": v["avg_us"], "GBs": v.get("achieved_GBs")} for n, v in r["kernels"].items()},
+            "launch_shape": r["launch_shape"]}
 
 
 def cpu_baseline(p, cell, nsteps):
@@ -107,23 +154,29 @@ def cpu_baseline(p, cell, nsteps):
             "wall_s": wall}
 
 
-def traffic_from_profiles(workload):
-    path = os.path.join(ROOT, "profiles", f"traffic_k_cru_step_{workload}.json")
+def traffic_from_profiles(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` from the committed PMC
+    summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the factor
+    the same counter shows on a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("hbm_bytes_per_launch")
+        t = json.load(f)
+    return t.get(f"{kernel}/{workload}", {}).get("hbm_bytes_per_launch")
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None, help="timed time steps (default 8000 cell / 1200 batch)")
+    ap.add_argument("--steps", type=int, default=None, help="timed time steps of the 1-cell workload (default 8000)")
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--precision", choices=["f32", "f64"], default="f32")
     ap.add_argument("--workload", choices=["cell", "batch", "both"], default="both")
     ap.add_argument("--batch-steps", type=int, default=1200)
-    ap.add_argument("--cpu-steps", type=int, default=1000)
+    ap.add_argument("--prof-steps", type=int, default=None,
+                    help="steps with per-kernel event stamps after the timed region (default 400 cell / 100 batch)")
+    ap.add_argument("--cpu-steps", type=int, default=4000)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -135,23 +188,21 @@ def main():
     warmup = args.warmup if args.warmup is not None else 400
 
     p, cells = make_cell(prec)
+    r = None
     if args.workload in ("cell", "both"):
-        r = run_gpu(p, cells, steps, warmup)
-        print(f"# cell: {r}", file=sys.stderr, flush=True)
+        r = run_gpu(p, cells, steps, warmup, args.prof_steps or 400)
+        print(f"# cell: {json.dumps(r)}", file=sys.stderr, flush=True)
     batch = None
     if args.workload in ("batch", "both"):
         pb, bcells = make_batch(prec)
-        rb = run_gpu(pb, bcells, args.batch_steps, args.warmup if args.warmup is not None else 100)
-        print(f"# batch: {rb}", file=sys.stderr, flush=True)
+        bw = args.warmup if args.warmup is not None else 100
+        rb = run_gpu(pb, bcells, args.batch_steps, bw, args.prof_steps or 100)
+        print(f"# batch: {json.dumps(rb)}", file=sys.stderr, flush=True)
         batch = {"value": rb["cell_ms_per_s"], "unit": "simulated cell-ms/wall-s", "ncells": len(bcells),
-                 "ncru": rb["ncru"], "steps": args.batch_steps, "ms_per_step": rb["ms_per_step"],
-                 "roofline": {"bound": "hbm", "achieved": rb["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                              "frac": rb["achieved_GBs"] / HBM_PEAK_GBS,
-                              "traffic": traffic_from_profiles("batch"),
-                              "kernel_us": rb["kernel_us"], "bytes_per_launch": rb["bytes_per_launch"]}}
-        if args.workload == "batch":
-            r = rb
-            steps = args.batch_steps
+                 "ncru": rb["ncru"], "steps": args.batch_steps, "warmup": bw, "ms_per_step": rb["ms_per_step"],
+                 "roofline": roofline(rb, "batch")}
+        if r is None:
+            r, steps, warmup = rb, args.batch_steps, bw
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(make_cell(0)[0], cells[0], args.cpu_steps)
@@ -173,10 +224,7 @@ def main():
         "config": {"workload": "1 cell 100x20x10 CRUs, Ku2=2.5, I_Kr=0, CL 2000 ms" if wl == "cell"
                    else "256-cell uneven batch", "dt_ms": p.dt, "precision": args.precision,
                    "ncru": r["ncru"]},
-        "roofline": {"bound": "hbm", "achieved": r["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": r["achieved_GBs"] / HBM_PEAK_GBS, "traffic": traffic_from_profiles(wl),
-                     "kernel": "k_cru_step", "kernel_us": r["kernel_us"],
-                     "bytes_per_launch": r["bytes_per_launch"]},
+        "roofline": roofline(r, wl),
         "cpu_baseline": cpu,
         "batch": batch,
     }

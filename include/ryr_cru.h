@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define RYR_ABI_VERSION 1
+#define RYR_ABI_VERSION 2
 
 #define RYR_OK            0
 #define RYR_E_ARG        -1   /* invalid argument */
@@ -224,6 +224,24 @@ int ryr_cru_steps(ryr_handle *h, int64_t *out);
  * handle's stream since the last call; *launches gets how many step kernels
  * ran.  Used by the benchmark for the per-launch kernel duration. */
 int ryr_kernel_time(ryr_handle *h, double *ms, int64_t *launches);
+
+/* Workgroup sizes (64, 128 or 256 lanes) of the two CRU-parallel kernels of a
+ * step: the sub-voxel (cytosol/NSR) kernel and the CRU kernel.  No reference
+ * counterpart (the reference hard-codes a 48x448 launch, :1701).  A new
+ * handle tunes them itself: its first 9 steps alternate the sizes with each
+ * kernel timed by its own events, then the fastest median is kept.  Every size
+ * computes identical values, so tuning never changes a result.  Passing
+ * (0, 0) re-arms the tuning; the getter reports 0 while tuning is pending. */
+int ryr_set_launch_shape(ryr_handle *h, int wg_cyto, int wg_cru);
+int ryr_get_launch_shape(const ryr_handle *h, int *wg_cyto, int *wg_cru);
+
+/* Advances nsteps steps with every kernel launched on its own (no graph) and
+ * stamped by HIP events at its start and end; avg_us[k] / count[k] receive the
+ * mean duration (microseconds) and launch count of kernel k: 0 sub-voxel,
+ * 1 CRU, 2 membrane, 3 output row.  The steps are ordinary simulation steps.
+ * Used by the benchmark for per-kernel roofline figures. */
+#define RYR_NPROF 4
+int ryr_profile_kernels(ryr_handle *h, int64_t nsteps, double *avg_us, int64_t *count);
 
 /* Snapshot/restore of the whole device state (all cells, all fields,
  * membrane scalars, step counter).  The reference has none; with a

@@ -26,10 +26,19 @@
  *    output step and the per-interval counter reset are integer tests on n
  *    (the reference's fmod on an accumulated t can jitter by one step);
  *  - whole-cell and diagnostic sums: a fixed compensated order (pairwise
- *    double-double inside chunks of 256 CRUs, chunks accumulated in order)
- *    instead of the serial loop of :435-445/:839-929;
+ *    double-double trees inside chunks of 256 CRUs, then over groups of 256
+ *    chunk sums, groups accumulated in order) instead of the serial loop of
+ *    :435-445/:839-929;
  *  - the dead dye buffer (cadye, never printed) is not carried;
  *  - Cm follows ryr_params.cm_mode (density-preserving by default).
+ *
+ * Pinning: the tests check each part against closed forms, invariants,
+ * brute force or published known answers (Philox).  Parity unpinned:
+ *  - trajectories against the reference's own runs (cuRAND and CUDA libm are
+ *    absent and the reference ships no outputs), so agreement with it is
+ *    statistical;
+ *  - the scheme B sub-step orchestration (:1475-1597), which conserves no
+ *    quantity; only its uniform fixed point is checked.
  */
 #include <stdlib.h>
 #include <string.h>

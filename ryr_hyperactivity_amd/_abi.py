@@ -131,6 +131,9 @@ def load_library(path: str | None = None):
         "ryr_read_counters": (c_int, [vp, P(c_i64)]),
         "ryr_cru_steps": (c_int, [vp, P(c_i64)]),
         "ryr_kernel_time": (c_int, [vp, P(c_double), P(c_i64)]),
+        "ryr_set_launch_shape": (c_int, [vp, c_int, c_int]),
+        "ryr_get_launch_shape": (c_int, [vp, P(c_int), P(c_int)]),
+        "ryr_profile_kernels": (c_int, [vp, c_i64, P(c_double), P(c_i64)]),
         "ryr_save": (c_int, [vp, ctypes.c_char_p]),
         "ryr_load": (c_int, [vp, ctypes.c_char_p]),
         "ryr_destroy": (None, [vp]),
@@ -146,7 +149,7 @@ def load_library(path: str | None = None):
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    if lib.ryr_abi_version() != 1:
+    if lib.ryr_abi_version() != 2:
         raise LibraryMissing(f"{p}: ABI version mismatch")
     if path is None:
         _lib = lib
@@ -158,4 +161,5 @@ EXPORTED = ["ryr_abi_version", "ryr_params_default", "ryr_cell_desc_default", "r
             "ryr_sync", "ryr_read_scalars", "ryr_read_field", "ryr_write_field", "ryr_write_scalar",
             "ryr_read_diag", "ryr_read_linescan", "ryr_read_counters", "ryr_cru_steps",
             "ryr_kernel_time", "ryr_save", "ryr_load", "ryr_destroy", "ryr_last_error",
-            "ryr_read_beats", "ryr_set_vclamp", "ryr_gating_probe"]
+            "ryr_read_beats", "ryr_set_vclamp", "ryr_gating_probe", "ryr_set_launch_shape",
+            "ryr_get_launch_shape", "ryr_profile_kernels"]

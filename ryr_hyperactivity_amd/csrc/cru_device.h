@@ -88,7 +88,7 @@ struct RyrKArgs {
     const RyrCellInfo *cells;
     RyrMem *mem;
     const int *block_cell;
-    RyrDD *partial;         /* 3 per block */
+    RyrDD *partial;         /* whole-cell sums: 3 per wave, 4 waves per chunk */
     unsigned long long *counters;  /* per cell x RYR_C_COUNT */
     double *diag;           /* per cell x 45 */
     long long *diag_step;   /* per cell */
@@ -666,24 +666,31 @@ RYR_DEV float fbuf(float b, float c, float dt, float sat, float kon, float koff,
     return (on * dt > 1.0f) ? sat * (B - b) - koff * b : on * (B - b) - koff * b;
 }
 
-/* One lane per sub-voxel of the 2x-refined cytosol/NSR grid: a workgroup is
- * one plane ii of one 256-CRU chunk.  SERCA and leak, the 7-point stencils of
- * ci and cnsr, the five cytosolic buffers and the ci/cnsr/buffer updates
- * (cuda_Cell_Detailed.cu:1305-1308, :1320-1393 / :1557-1586), plus the
- * JSR->NSR and sub-membrane/cleft->cytosol exchange terms of sub-voxels 0 and 1
- * (:1309-1314, :1356-1361).  Under scheme B those two sub-voxels receive
- * their cleft terms during the sub-steps, so their partial rate is left in
- * `sb` for the CRU kernel.  Runs before the CRU kernel of the same step
- * (it reads the step-start cp, cs, cjsr). */
-template <typename TF, typename TS>
-__global__ __launch_bounds__(RYR_BLOCK) void k_cyto(RyrKArgs a)
+/* One lane per sub-voxel of the 2x-refined cytosol/NSR grid: a workgroup of
+ * WG lanes is one part of one plane ii of one 256-CRU chunk.  SERCA and leak,
+ * the 7-point stencils of ci and cnsr, the five cytosolic buffers and the
+ * ci/cnsr/buffer updates (cuda_Cell_Detailed.cu:1305-1308, :1320-1393 /
+ * :1557-1586), plus the JSR->NSR and sub-membrane/cleft->cytosol exchange
+ * terms of sub-voxels 0 and 1 (:1309-1314, :1356-1361).  Under scheme B those
+ * two sub-voxels receive their cleft terms during the sub-steps, so their
+ * partial rate is left in `sb` for the CRU kernel.  Runs before the CRU kernel
+ * of the same step (it reads the step-start cp, cs, cjsr).
+ * Grid: nchunks * 8 * (256/WG) workgroups, planes fastest, so the eight planes
+ * of a chunk (which read each other as stencil partners) are consecutive
+ * logical workgroups on one XCD and meet in its L2. */
+template <typename TF, typename TS, int WG>
+__global__ __launch_bounds__(WG) void k_cyto(RyrKArgs a)
 {
-    const int b = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-    const int ii = (int)blockIdx.y;
+    constexpr int SUB = RYR_BLOCK / WG;
+    const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int b = L / (8 * SUB);
+    const int r = L - b * (8 * SUB);
+    const int ii = r & 7;
+    const int part = r >> 3;
     const int c = a.block_cell[b];
     const RyrCellInfo C = a.cells[c];
     const long long nstep = a.mem[c].step;
-    const int n = (b - C.first_block) * RYR_BLOCK + (int)threadIdx.x;
+    const int n = (b - C.first_block) * RYR_BLOCK + part * WG + (int)threadIdx.x;
     if (n >= C.N) return;
     const int par = (int)(nstep & 1);
     const bool diag = (nstep % a.spo) == 0;
@@ -902,12 +909,13 @@ RYR_DEV void fast_cleft(bool schemeA, float dt, float cif0, float cif1, float cs
 }
 
 /* ------------------------------------------------------------ step kernel */
-template <typename TF, typename TS>
-__global__ __launch_bounds__(RYR_BLOCK) void k_cru_step(RyrKArgs a, double stim_amp)
+template <typename TF, typename TS, int WG>
+__global__ __launch_bounds__(WG) void k_cru_step(RyrKArgs a, double stim_amp)
 {
-    __shared__ RyrDD s_red[3][RYR_BLOCK / 64];
-
-    const int bid = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    constexpr int SUB = RYR_BLOCK / WG;
+    const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int bid = L / SUB;               /* 256-CRU chunk */
+    const int part = L - bid * SUB;
     const int c = a.block_cell[bid];
     const RyrCellInfo C = a.cells[c];
     RyrMem *M = a.mem + c;
@@ -916,7 +924,7 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_cru_step(RyrKArgs a, double stim_
     const int spo = a.spo;
     const bool reset = (nstep == 0) || ((nstep - 1) % spo == 0);
     const bool diag = (nstep % spo) == 0;
-    const int n = (bid - C.first_block) * RYR_BLOCK + (int)threadIdx.x;
+    const int n = (bid - C.first_block) * RYR_BLOCK + part * WG + (int)threadIdx.x;
     const bool active = n < C.N;
     const long long P = a.P;
     const double dt = a.dt;
@@ -1183,21 +1191,19 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_cru_step(RyrKArgs a, double stim_
         r_cs = (double)csn;
     }
 
-    /* whole-cell sums: pairwise double-double tree over this block's 256
-     * CRUs (index order), one partial per block; k_membrane adds them */
+    /* whole-cell sums: pairwise double-double tree over each wave's 64 CRUs
+     * (index order), one partial per wave (4 per chunk, whatever WG is);
+     * k_membrane pairs them into chunk sums and adds those */
     {
-        const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+        const int lane = threadIdx.x & 63;
+        const int w = part * (WG / 64) + ((int)threadIdx.x >> 6);
         RyrDD t0 = {r_icat, 0.0}, t1 = {r_inaca, 0.0}, t2 = {r_cs, 0.0};
         t0 = wave_tree(t0);
         t1 = wave_tree(t1);
         t2 = wave_tree(t2);
-        if (lane == 0) { s_red[0][wv] = t0; s_red[1][wv] = t1; s_red[2][wv] = t2; }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            RyrDD *pp = a.partial + 3LL * bid;
-#pragma unroll
-            for (int q = 0; q < 3; ++q)
-                pp[q] = dd_add(dd_add(s_red[q][0], s_red[q][1]), dd_add(s_red[q][2], s_red[q][3]));
+        if (lane == 0) {
+            RyrDD *pp = a.partial + 12LL * bid + 3 * w;
+            pp[0] = t0; pp[1] = t1; pp[2] = t2;
         }
     }
 }
@@ -1221,14 +1227,17 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_membrane(RyrKArgs a, double stim_
     const double dt = a.dt;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     RyrDD acc[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
-    const RyrDD *pp = a.partial + 3LL * C.first_block;
+    const RyrDD *pp = a.partial + 12LL * C.first_block;
     for (int g0 = 0; g0 < C.nblocks; g0 += RYR_BLOCK) {
         const int bb = g0 + (int)threadIdx.x;
         RyrDD t[3];
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-            if (bb < C.nblocks) t[q] = pp[3LL * bb + q];
-            else { t[q].hi = 0.0; t[q].lo = 0.0; }
+            if (bb < C.nblocks) {
+                /* chunk sum = pairwise tree over its four wave partials */
+                const RyrDD *w = pp + 12LL * bb + q;
+                t[q] = dd_add(dd_add(w[0], w[3]), dd_add(w[6], w[9]));
+            } else { t[q].hi = 0.0; t[q].lo = 0.0; }
             t[q] = wave_tree(t[q]);
         }
         if (lane == 0) { s_red[0][wv] = t[0]; s_red[1][wv] = t[1]; s_red[2][wv] = t[2]; }

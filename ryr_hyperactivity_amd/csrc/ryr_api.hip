@@ -9,6 +9,7 @@
  * is captured once as a HIP graph and replayed.
  */
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -35,6 +36,10 @@ struct ryr_handle {
     hipGraphExec_t graph = nullptr;
     bool graph_failed = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t kev[8] = {};         /* per-kernel start/end stamps: cyto, cru, membrane, diag */
+    int wg_cyto = 0, wg_cru = 0;    /* workgroup sizes of the two CRU-parallel kernels; 0: tune */
+    int tune_i = 0;
+    std::vector<float> tune_ms[2][3];
     bool timing_open = false;
     long long launches = 0;
     std::string err;
@@ -173,6 +178,7 @@ extern "C" int ryr_create(int device, const ryr_params *p, const ryr_cell_desc *
     }
     (void)hipEventCreate(&h->ev0);
     (void)hipEventCreate(&h->ev1);
+    for (auto &e : h->kev) (void)hipEventCreate(&e);
 
     /* layout: each cell padded to whole workgroups */
     std::vector<int> block_cell;
@@ -247,7 +253,7 @@ extern "C" int ryr_create(int device, const ryr_params *p, const ryr_cell_desc *
     ALLOC(d_line, sizeof(double) * (lo > 0 ? lo : 1));
     ALLOC(d_cells, sizeof(RyrCellInfo) * ncells);
     ALLOC(d_bc, sizeof(int) * h->nblocks);
-    ALLOC(d_part, sizeof(RyrDD) * 3 * h->nblocks);
+    ALLOC(d_part, sizeof(RyrDD) * 12 * h->nblocks);
     ALLOC(d_loff, sizeof(long long) * ncells);
     ALLOC(a.beats, sizeof(double) * ncells * RYR_MAXB * RYR_BCOLS);
     ALLOC(a.dpartial, sizeof(RyrDD) * RYR_DIAG_Q * h->nblocks);
@@ -331,15 +337,60 @@ extern "C" int ryr_create(int device, const ryr_params *p, const ryr_cell_desc *
     return RYR_OK;
 }
 
-static hipError_t launch_one(ryr_handle *h, long long s)
+static hipError_t launch_one(ryr_handle *h, long long s, int wg_cyto, int wg_cru, bool timed)
 {
     const bool f32 = h->p.precision == RYR_PREC_F32;
-    hipError_t e = f32 ? ryr_launch_step_f32(h->a, h->p.stim_amp, h->nblocks, h->ncells, h->stream)
-                       : ryr_launch_step_f64(h->a, h->p.stim_amp, h->nblocks, h->ncells, h->stream);
+    hipEvent_t *ev = timed ? h->kev : nullptr;
+    hipError_t e = f32 ? ryr_launch_step_f32(h->a, h->p.stim_amp, h->nblocks, h->ncells, wg_cyto, wg_cru, h->stream, ev)
+                       : ryr_launch_step_f64(h->a, h->p.stim_amp, h->nblocks, h->ncells, wg_cyto, wg_cru, h->stream, ev);
     if (e != hipSuccess) return e;
     if (s % h->p.steps_per_output == 0)
-        e = f32 ? ryr_launch_diag_f32(h->a, h->nblocks, h->stream) : ryr_launch_diag_f64(h->a, h->nblocks, h->stream);
+        e = f32 ? ryr_launch_diag_f32(h->a, h->nblocks, h->stream, timed ? h->kev + 6 : nullptr)
+                : ryr_launch_diag_f64(h->a, h->nblocks, h->stream, timed ? h->kev + 6 : nullptr);
     return e;
+}
+
+static void drop_graph(ryr_handle *h)
+{
+    if (h->graph) hipGraphExecDestroy(h->graph);
+    h->graph = nullptr;
+    h->graph_failed = false;
+}
+
+/* Shape tuning: the first steps of a handle alternate the three workgroup
+ * sizes, each step's kernels stamped by their own events; after
+ * RYR_TUNE_ROUNDS rounds each kernel keeps the size with the smallest median
+ * duration.  The tuning steps are ordinary steps of the simulation (every
+ * shape computes identical values), so tuning changes no result. */
+#define RYR_TUNE_ROUNDS 3
+static int tune_step(ryr_handle *h)
+{
+    const int v = h->tune_i % 3;
+    const int wg = 64 << v;
+    RYR_HIPCHECK(h, launch_one(h, h->step, wg, wg, true));
+    RYR_HIPCHECK(h, hipEventSynchronize(h->kev[5]));
+    for (int k = 0; k < 2; ++k) {
+        float ms = 0.0f;
+        RYR_HIPCHECK(h, hipEventElapsedTime(&ms, h->kev[2 * k], h->kev[2 * k + 1]));
+        h->tune_ms[k][v].push_back(ms);
+    }
+    if (++h->tune_i == 3 * RYR_TUNE_ROUNDS) {
+        int best[2] = {256, 256};
+        for (int k = 0; k < 2; ++k) {
+            float bm = 1e30f;
+            for (int q = 0; q < 3; ++q) {
+                std::vector<float> t = h->tune_ms[k][q];
+                std::sort(t.begin(), t.end());
+                const float med = t[t.size() / 2];
+                if (med < bm) { bm = med; best[k] = 64 << q; }
+                h->tune_ms[k][q].clear();
+            }
+        }
+        h->wg_cyto = best[0];
+        h->wg_cru = best[1];
+        drop_graph(h);
+    }
+    return RYR_OK;
 }
 
 /* one output interval (spo steps starting at an output step) as a graph */
@@ -351,7 +402,7 @@ static bool build_graph(ryr_handle *h)
     const long long spo = h->p.steps_per_output;
     bool ok = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
     if (ok) {
-        for (long long s = 0; s < spo && ok; ++s) ok = launch_one(h, s) == hipSuccess;
+        for (long long s = 0; s < spo && ok; ++s) ok = launch_one(h, s, h->wg_cyto, h->wg_cru, false) == hipSuccess;
         hipError_t e = hipStreamEndCapture(h->stream, &g);
         ok = ok && e == hipSuccess && g;
     }
@@ -376,6 +427,13 @@ extern "C" int ryr_step(ryr_handle *h, int64_t nsteps)
         h->launches = 0;
     }
     const long long spo = h->p.steps_per_output;
+    while (nsteps > 0 && h->wg_cyto == 0) {
+        int rc = tune_step(h);
+        if (rc) return rc;
+        h->step += 1;
+        h->launches += 1;
+        nsteps -= 1;
+    }
     while (nsteps > 0) {
         if (h->step % spo == 0 && nsteps >= spo && spo >= 4 && build_graph(h)) {
             RYR_HIPCHECK(h, hipGraphLaunch(h->graph, h->stream));
@@ -384,7 +442,7 @@ extern "C" int ryr_step(ryr_handle *h, int64_t nsteps)
             nsteps -= spo;
             continue;
         }
-        RYR_HIPCHECK(h, launch_one(h, h->step));
+        RYR_HIPCHECK(h, launch_one(h, h->step, h->wg_cyto, h->wg_cru, false));
         h->step += 1;
         h->launches += 1;
         nsteps -= 1;
@@ -414,6 +472,59 @@ extern "C" int ryr_kernel_time(ryr_handle *h, double *ms, int64_t *launches)
     *ms = (double)f;
     *launches = h->launches;
     h->timing_open = false;
+    return RYR_OK;
+}
+
+extern "C" int ryr_set_launch_shape(ryr_handle *h, int wg_cyto, int wg_cru)
+{
+    if (!h) return RYR_E_ARG;
+    auto ok = [](int w) { return w == 64 || w == 128 || w == 256; };
+    if (wg_cyto == 0 && wg_cru == 0) {
+        h->wg_cyto = h->wg_cru = 0;
+        h->tune_i = 0;
+        for (auto &k : h->tune_ms) for (auto &q : k) q.clear();
+    } else if (ok(wg_cyto) && ok(wg_cru)) {
+        h->wg_cyto = wg_cyto;
+        h->wg_cru = wg_cru;
+    } else {
+        h->err = "workgroup sizes must be 64, 128 or 256 (or both 0 to tune)";
+        return RYR_E_ARG;
+    }
+    drop_graph(h);
+    return RYR_OK;
+}
+
+extern "C" int ryr_get_launch_shape(const ryr_handle *h, int *wg_cyto, int *wg_cru)
+{
+    if (!h || !wg_cyto || !wg_cru) return RYR_E_ARG;
+    *wg_cyto = h->wg_cyto;
+    *wg_cru = h->wg_cru;
+    return RYR_OK;
+}
+
+extern "C" int ryr_profile_kernels(ryr_handle *h, int64_t nsteps, double *avg_us, int64_t *count)
+{
+    if (!h || nsteps < 0 || !avg_us || !count) return RYR_E_ARG;
+    RYR_HIPCHECK(h, hipSetDevice(h->device));
+    for (int k = 0; k < RYR_NPROF; ++k) { avg_us[k] = 0.0; count[k] = 0; }
+    if (h->wg_cyto == 0) {
+        int rc = ryr_step(h, 3 * RYR_TUNE_ROUNDS - h->tune_i);
+        if (rc) return rc;
+    }
+    double tot[RYR_NPROF] = {0, 0, 0, 0};
+    for (int64_t i = 0; i < nsteps; ++i) {
+        const bool d = h->step % h->p.steps_per_output == 0;
+        RYR_HIPCHECK(h, launch_one(h, h->step, h->wg_cyto, h->wg_cru, true));
+        RYR_HIPCHECK(h, hipEventSynchronize(d ? h->kev[7] : h->kev[5]));
+        for (int k = 0; k < (d ? 4 : 3); ++k) {
+            float ms = 0.0f;
+            RYR_HIPCHECK(h, hipEventElapsedTime(&ms, h->kev[2 * k], h->kev[2 * k + 1]));
+            tot[k] += 1e3 * (double)ms;
+            count[k] += 1;
+        }
+        h->step += 1;
+    }
+    for (int k = 0; k < RYR_NPROF; ++k) avg_us[k] = count[k] ? tot[k] / (double)count[k] : 0.0;
     return RYR_OK;
 }
 
@@ -667,6 +778,7 @@ extern "C" void ryr_destroy(ryr_handle *h)
     for (auto &al : h->allocs) hipFree(al.first);
     if (h->stage) hipFree(h->stage);
     for (double *q : h->vclamp) if (q) hipFree(q);
+    for (auto &e : h->kev) if (e) hipEventDestroy(e);
     if (h->ev0) hipEventDestroy(h->ev0);
     if (h->ev1) hipEventDestroy(h->ev1);
     if (h->stream) hipStreamDestroy(h->stream);

@@ -78,6 +78,24 @@ class CellEngine:
         self._check(self.lib.ryr_kernel_time(self.h, ctypes.byref(ms), ctypes.byref(n)), "ryr_kernel_time")
         return ms.value, n.value
 
+    def set_launch_shape(self, wg_cyto: int, wg_cru: int):
+        """Workgroup sizes (64/128/256) of the sub-voxel and CRU kernels; (0, 0) re-tunes."""
+        self._check(self.lib.ryr_set_launch_shape(self.h, int(wg_cyto), int(wg_cru)), "ryr_set_launch_shape")
+
+    def launch_shape(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.ryr_get_launch_shape(self.h, ctypes.byref(a), ctypes.byref(b)), "ryr_get_launch_shape")
+        return a.value, b.value
+
+    def profile_kernels(self, nsteps: int):
+        """Advance nsteps steps with per-kernel event stamps; returns
+        {kernel: (mean_us, launches)} for k_cyto, k_cru_step, k_membrane, k_diag."""
+        us = (ctypes.c_double * 4)()
+        cnt = (ctypes.c_int64 * 4)()
+        self._check(self.lib.ryr_profile_kernels(self.h, int(nsteps), us, cnt), "ryr_profile_kernels")
+        names = ("k_cyto", "k_cru_step", "k_membrane", "k_diag")
+        return {nm: (us[i], cnt[i]) for i, nm in enumerate(names)}
+
     def cru_steps(self) -> int:
         n = ctypes.c_int64()
         self._check(self.lib.ryr_cru_steps(self.h, ctypes.byref(n)), "ryr_cru_steps")

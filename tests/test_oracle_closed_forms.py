@@ -218,3 +218,51 @@ def test_scheme_a_conserves_calcium_exactly_up_to_sarcolemmal_flux():
             checked += 1
         prev = row[44]
     assert checked > 100
+
+
+def test_output_row_columns_recomputed_from_the_state():
+    """The output row (cuda_Cell_Detailed.cu:839-962, :1031-1072) aggregates
+    the state the step leaves behind: every column that is a mean, a count or
+    a histogram of exported fields is recomputed here from those fields with
+    numpy, and the LCC histogram from the gate bits of each channel's state
+    (bit0 c closed, bit1 d closed, bit2 f inactivated, bit3 fCa inactivated;
+    reesica.h:65-113) instead of the reference's 16-way table (:904-923)."""
+    p = Params.default(stim_offset=1.0)
+    d = CellDesc.default(nx=9, ny=6, nz=4, cl=1000.0, seed=3)
+    o = OracleCell(p, d)
+    o.step(301)                                   # the last step (300) is an output step
+    row, st = o.diag()
+    assert st == 300 and row[0] == 300 * p.dt
+    fld = {f: o.field(f) for f in ("CI", "CNSR", "CATF", "CATS", "CACAL", "CASR", "CAMYO", "CS", "CP", "CJSR",
+                                   "CJSRB", "ANCX", "CASAR", "CASARH", "CASARJ", "CASARHJ", "NOU", "NCU", "NOB",
+                                   "NCB", "LCC", "NSPARK")}
+    m8 = {f: fld[f].mean() for f in ("CI", "CNSR", "CATF", "CATS", "CACAL", "CASR", "CAMYO")}
+    m1 = {f: fld[f].mean() for f in ("CS", "CP", "CJSR", "CJSRB", "ANCX", "CASAR", "CASARH", "CASARJ", "CASARHJ")}
+    close = dict(rel=1e-12, abs=0)
+    assert row[1] == pytest.approx(m8["CI"], **close)
+    assert row[5] == pytest.approx(m1["CP"], **close)
+    assert row[6] == pytest.approx(m1["CS"], **close)
+    assert row[7] == pytest.approx(m1["CJSR"], **close)
+    assert row[8] == pytest.approx(m8["CNSR"], **close)
+    assert row[9] == pytest.approx(((fld["NOU"] + fld["NOB"]) / 100.0).mean(), **close)
+    assert row[20] == pytest.approx(fld["NCB"].mean() / 100.0, **close)
+    assert row[21] == pytest.approx(m8["CACAL"], **close)
+    assert row[22] == pytest.approx(m8["CATF"], **close)
+    assert [row[26], row[27], row[28], row[29]] == [fld[f].sum() for f in ("NOU", "NOB", "NCU", "NCB")]
+    lcc = fld["LCC"].astype(np.int64).ravel()
+    lcc = lcc[lcc < 16]                           # 16 marks an absent channel slot
+    b = [(lcc >> q) & 1 for q in range(4)]
+    hist = [b[0].sum(), b[1].sum(), b[2].sum(), b[3].sum(), (b[2] & (1 - b[3])).sum(),
+            (lcc <= 1).sum(), (lcc == 0).sum()]
+    assert list(row[30:37]) == hist
+    assert row[37] == pytest.approx(m1["ANCX"], **close)
+    assert row[40] == fld["NSPARK"].sum()
+    assert row[41] == pytest.approx(m8["CATS"], **close)
+    assert row[42] == pytest.approx(m1["CJSRB"], **close)
+    # whole-cell calcium content: cytosol (8 sub-voxels of Vi each), sub-membrane,
+    # cleft, NSR (8 of Vnsr), JSR (free + CSQN-bound), volumes of :66-69,95
+    Vi, Vs, Vp, Vnsr, Vjsr = 0.5 / 8.0, 0.025, 0.00126, 0.025 / 8.0, 0.02
+    cyt = (m8["CI"] + m8["CATF"] + m8["CATS"] + m8["CASR"] + m8["CAMYO"] + m8["CACAL"]) * Vi * 8
+    tot_ci = cyt + (m1["CS"] + m1["CASAR"] + m1["CASARH"]) * Vs + (m1["CP"] + m1["CASARJ"] + m1["CASARHJ"]) * Vp
+    assert row[43] == pytest.approx(tot_ci, **close)
+    assert row[44] == pytest.approx(tot_ci + m8["CNSR"] * Vnsr * 8 + (m1["CJSR"] + m1["CJSRB"]) * Vjsr, **close)
