@@ -121,7 +121,7 @@ def run_gpu(p, cells, steps, warmup, prof_steps):
         "ncru": ncru,
         "step_bytes": ncru * alg_bytes_per_cru_step(p.precision),
         "step_GBs": ncru * alg_bytes_per_cru_step(p.precision) / step_s / 1e9,
-        "launch_shape": {"k_cyto": shape[0], "k_cru_step": shape[1]},
+        "launch_shape": {"k_cyto": shape[0], "k_cru_step": shape[1], "membrane_fused": shape[2]},
         "kernels": kernels,
         "dominant": dom,
     }

@@ -225,15 +225,18 @@ int ryr_cru_steps(ryr_handle *h, int64_t *out);
  * ran.  Used by the benchmark for the per-launch kernel duration. */
 int ryr_kernel_time(ryr_handle *h, double *ms, int64_t *launches);
 
-/* Workgroup sizes (64, 128 or 256 lanes) of the two CRU-parallel kernels of a
- * step: the sub-voxel (cytosol/NSR) kernel and the CRU kernel.  No reference
+/* Launch shape of a step: the workgroup sizes (64, 128 or 256 lanes) of the
+ * two CRU-parallel kernels (the sub-voxel cytosol/NSR kernel and the CRU
+ * kernel), and `fuse`: whether the membrane step of step n runs inside the
+ * sub-voxel launch of step n+1 (1) or on its own (0).  No reference
  * counterpart (the reference hard-codes a 48x448 launch, :1701).  A new
- * handle tunes them itself: its first 9 steps alternate the sizes with each
- * kernel timed by its own events, then the fastest median is kept.  Every size
- * computes identical values, so tuning never changes a result.  Passing
- * (0, 0) re-arms the tuning; the getter reports 0 while tuning is pending. */
-int ryr_set_launch_shape(ryr_handle *h, int wg_cyto, int wg_cru);
-int ryr_get_launch_shape(const ryr_handle *h, int *wg_cyto, int *wg_cru);
+ * handle tunes the shape itself: its first 9 steps alternate the sizes with
+ * each kernel timed by its own events, the next 12 compare fused and unfused
+ * step pairs, and the fastest medians are kept.  Every shape computes
+ * identical values, so tuning never changes a result.  Passing sizes (0, 0)
+ * re-arms the tuning; the getter reports 0 sizes (fuse -1) while pending. */
+int ryr_set_launch_shape(ryr_handle *h, int wg_cyto, int wg_cru, int fuse);
+int ryr_get_launch_shape(const ryr_handle *h, int *wg_cyto, int *wg_cru, int *fuse);
 
 /* Advances nsteps steps with every kernel launched on its own (no graph) and
  * stamped by HIP events at its start and end; avg_us[k] / count[k] receive the

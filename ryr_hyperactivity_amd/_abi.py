@@ -131,8 +131,8 @@ def load_library(path: str | None = None):
         "ryr_read_counters": (c_int, [vp, P(c_i64)]),
         "ryr_cru_steps": (c_int, [vp, P(c_i64)]),
         "ryr_kernel_time": (c_int, [vp, P(c_double), P(c_i64)]),
-        "ryr_set_launch_shape": (c_int, [vp, c_int, c_int]),
-        "ryr_get_launch_shape": (c_int, [vp, P(c_int), P(c_int)]),
+        "ryr_set_launch_shape": (c_int, [vp, c_int, c_int, c_int]),
+        "ryr_get_launch_shape": (c_int, [vp, P(c_int), P(c_int), P(c_int)]),
         "ryr_profile_kernels": (c_int, [vp, c_i64, P(c_double), P(c_i64)]),
         "ryr_save": (c_int, [vp, ctypes.c_char_p]),
         "ryr_load": (c_int, [vp, ctypes.c_char_p]),

@@ -49,6 +49,7 @@ struct RyrCellInfo {
     int nlcc, line_j, line_k, vclamp_periodic;
     const double *vclamp;   /* V protocol (device array) or null */
     long long vclamp_n;
+    int cell, pad2_;        /* index of the cell (per-chunk copies carry it) */
 };
 
 /* membrane state of a cell plus the V-only factors hoisted for its next step */
@@ -86,6 +87,9 @@ struct RyrKArgs {
     double *d_xire, *d_xinaca;   /* per CRU, written on output steps */
     double *d_up, *d_leak;       /* per sub-voxel (8 planes), written on output steps */
     const RyrCellInfo *cells;
+    const RyrCellInfo *chunks;   /* per 256-CRU chunk: a copy of its cell's entry, so a
+                                    workgroup reaches its cell parameters in one load */
+    long long *gstep;       /* step index a launch's offsets count from (one per handle) */
     RyrMem *mem;
     const int *block_cell;
     RyrDD *partial;         /* whole-cell sums: 3 per wave, 4 waves per chunk */
@@ -243,12 +247,19 @@ RYR_DEV int lcc_step(int i, double cpx, double dt, double ragg, const RyrMem &m,
 RYR_DEV int ryr_transition(ryr_stream *s, int n, double p, int *fails)
 {
     RYR_STRICT_FP
-    /* exp(-0) is exactly 1: skip the call for empty populations or zero rates */
-    double lam = (n == 0 || p == 0.0) ? 1.0 : ryr_exp(-(n) * p);
     int out = -1;
     if (n <= 1 || p < 0.2 || (n <= 5 && p < 0.3)) {
-        int kk = 0;
-        double puu = 1.0;
+        /* First draw of the product loop.  exp(-x) > 1 - x, and for x >= 2^-40
+         * the rounded 1 - 2x lies below the rounded exp(-x) by far more than
+         * the rounding of either, so a draw under 1 - 2x ends the loop with
+         * k = 0 exactly as the comparison with exp(-x) would: the exponential
+         * is only evaluated when the draw is within about 2x of 1. */
+        const double x = (double)(n) * p;
+        double puu = ryr_stream_uniform(s);
+        if (x >= 0x1p-40 && puu < 1.0 - 2.0 * x) return 0;
+        /* exp(-0) is exactly 1: skip the call for empty populations or zero rates */
+        const double lam = (n == 0 || p == 0.0) ? 1.0 : ryr_exp(-(n) * p);
+        int kk = 1;
         while (puu >= lam && kk < 195) {
             kk++;
             double re = ryr_stream_uniform(s);
@@ -677,19 +688,37 @@ RYR_DEV float fbuf(float b, float c, float dt, float sat, float kon, float koff,
  * of the same step (it reads the step-start cp, cs, cjsr).
  * Grid: nchunks * 8 * (256/WG) workgroups, planes fastest, so the eight planes
  * of a chunk (which read each other as stencil partners) are consecutive
- * logical workgroups on one XCD and meet in its L2. */
+ * logical workgroups on one XCD and meet in its L2.
+ *
+ * With FUSE the same launch also carries the membrane step of the previous
+ * time step: the first `ncells` logical workgroups run membrane_block for
+ * step gstep + km.  The two are independent (this kernel reads no membrane
+ * state and the membrane reads only the CRU kernel's sums), so the
+ * latency-bound ODE hides behind the sub-voxel sweep; the price is the
+ * membrane code's registers, hence a separate instance. */
 template <typename TF, typename TS, int WG>
-__global__ __launch_bounds__(WG) void k_cyto(RyrKArgs a)
+RYR_DEV void membrane_block(const RyrKArgs &a, int c, long long nstep, double stim_amp);
+
+template <typename TF, typename TS, int WG, bool FUSE>
+__global__ __launch_bounds__(WG) void k_cyto(RyrKArgs a, int kc, int km, double stim_amp)
 {
     constexpr int SUB = RYR_BLOCK / WG;
-    const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const long long base = *a.gstep;
+    const int nmem = FUSE ? a.ncells : 0;
+    const int Lg = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    if constexpr (FUSE) {
+        if (Lg < nmem) {
+            membrane_block<TF, TS, WG>(a, Lg, base + km, stim_amp);
+            return;
+        }
+    }
+    const int L = Lg - nmem;
     const int b = L / (8 * SUB);
     const int r = L - b * (8 * SUB);
     const int ii = r & 7;
     const int part = r >> 3;
-    const int c = a.block_cell[b];
-    const RyrCellInfo C = a.cells[c];
-    const long long nstep = a.mem[c].step;
+    const RyrCellInfo C = a.chunks[b];
+    const long long nstep = base + kc;
     const int n = (b - C.first_block) * RYR_BLOCK + part * WG + (int)threadIdx.x;
     if (n >= C.N) return;
     const int par = (int)(nstep & 1);
@@ -853,7 +882,8 @@ __global__ __launch_bounds__(WG) void k_cyto(RyrKArgs a)
  * cuda_Cell_Detailed.cu:1396-1464, scheme B :1479-1553), reciprocal
  * constants and float arithmetic.  Under scheme B dotci01 enters with the
  * cytosol kernel's partial rates of sub-voxels 0 and 1. */
-RYR_DEV void fast_cleft(bool schemeA, float dt, float cif0, float cif1, float csf0, float csu, float csd,
+template <bool schemeA>
+RYR_DEV void fast_cleft(float dt, float cif0, float cif1, float csf0, float csu, float csd,
                         float ica, float xna, float xre, float &cp, float &cs, float &sar, float &sarh, float &sarj,
                         float &sarhj, float &dci0, float &dci1)
 {
@@ -864,7 +894,7 @@ RYR_DEV void fast_cleft(bool schemeA, float dt, float cif0, float cif1, float cs
     const float dth = dt * 0.2f;
     const float sath = 0.95f / dth;
     const float cpf0 = cp;
-    if (schemeA) {
+    if constexpr (schemeA) {
         const float csdiff = (csu + csd - 2.0f * csf0) * r_taust;
         const float to_ci = ((csf0 - cif0) + (csf0 - cif1)) * r_tausi2;
         const float cp_ci = ((cif0 - cpf0) + (cif1 - cpf0)) * r_taupi2;
@@ -909,17 +939,17 @@ RYR_DEV void fast_cleft(bool schemeA, float dt, float cif0, float cif1, float cs
 }
 
 /* ------------------------------------------------------------ step kernel */
-template <typename TF, typename TS, int WG>
-__global__ __launch_bounds__(WG) void k_cru_step(RyrKArgs a, double stim_amp)
+template <typename TF, typename TS, int WG, bool SCHB>
+__global__ __launch_bounds__(WG) void k_cru_step(RyrKArgs a, int kc)
 {
     constexpr int SUB = RYR_BLOCK / WG;
     const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     const int bid = L / SUB;               /* 256-CRU chunk */
     const int part = L - bid * SUB;
-    const int c = a.block_cell[bid];
-    const RyrCellInfo C = a.cells[c];
-    RyrMem *M = a.mem + c;
-    const long long nstep = M->step;
+    const RyrCellInfo C = a.chunks[bid];
+    const int c = C.cell;
+    const RyrMem *M = a.mem + c;
+    const long long nstep = *a.gstep + kc;
     const int par = (int)(nstep & 1);
     const int spo = a.spo;
     const bool reset = (nstep == 0) || ((nstep - 1) % spo == 0);
@@ -1071,12 +1101,12 @@ __global__ __launch_bounds__(WG) void k_cru_step(RyrKArgs a, double stim_amp)
         const TF *sbp = (const TF *)a.sb;
         if constexpr (sizeof(TF) != 8) {
             float dci[2] = {0.0f, 0.0f};
-            if (a.scheme != 0) { dci[0] = (float)sbp[g]; dci[1] = (float)sbp[P + g]; }
+            if (SCHB) { dci[0] = (float)sbp[g]; dci[1] = (float)sbp[P + g]; }
             float cpf = (float)cpv, csf = (float)csv;
             float sar = (float)v_casar, sarh = (float)v_casarh, sarj = (float)v_casarj, sarhj = (float)v_casarhj;
-            fast_cleft(a.scheme == 0, (float)dt, (float)cil[0], (float)cil[1], (float)csv, (float)cs_up,
+            fast_cleft<!SCHB>((float)dt, (float)cil[0], (float)cil[1], (float)csv, (float)cs_up,
                        (float)cs_dn, (float)icat1, (float)xinaca, (float)xire, cpf, csf, sar, sarh, sarj, sarhj, dci[0], dci[1]);
-            if (a.scheme != 0) {
+            if (SCHB) {
                 ci1[g] = (TF)fmaxf((float)cil[0] + dci[0] * (float)dt, 0.0f);
                 ci1[P + g] = (TF)fmaxf((float)cil[1] + dci[1] * (float)dt, 0.0f);
             }
@@ -1084,9 +1114,9 @@ __global__ __launch_bounds__(WG) void k_cru_step(RyrKArgs a, double stim_amp)
             v_casar = (TF)sar; v_casarh = (TF)sarh; v_casarj = (TF)sarj; v_casarhj = (TF)sarhj;
         } else {
         TF dotci01[2] = {TF(0), TF(0)};
-        if (a.scheme != 0) { dotci01[0] = sbp[g]; dotci01[1] = sbp[P + g]; }
+        if (SCHB) { dotci01[0] = sbp[g]; dotci01[1] = sbp[P + g]; }
         const TF dth = dtT / TF(5);
-        if (a.scheme == 0) {
+        if (!SCHB) {
             /* scheme A (:1396-1464) */
             const TF cpstart = cpT;
             for (int it = 0; it < 5; ++it) {
@@ -1214,33 +1244,36 @@ __global__ __launch_bounds__(WG) void k_cru_step(RyrKArgs a, double stim_amp)
  * then the membrane ODE on wavefront 0, the clamp protocol, the beat tracker,
  * the membrane columns of an output row and the V-only factors of the next
  * step (cuda_Cell_Detailed.cu:430-739). */
-template <typename TF, typename TS>
-__global__ __launch_bounds__(RYR_BLOCK) void k_membrane(RyrKArgs a, double stim_amp)
+template <typename TF, typename TS, int WG>
+RYR_DEV void membrane_block(const RyrKArgs &a, int c, long long nstep, double stim_amp)
 {
-    __shared__ RyrDD s_red[3][RYR_BLOCK / 64];
-    const int c = blockIdx.x;
+    constexpr int NW = WG / 64;
+    __shared__ RyrDD s_red[3][4];
     const RyrCellInfo C = a.cells[c];
     RyrMem *M = a.mem + c;
-    const long long nstep = M->step;
     const int spo = a.spo;
     const bool diag = (nstep % spo) == 0;
     const double dt = a.dt;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     RyrDD acc[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
     const RyrDD *pp = a.partial + 12LL * C.first_block;
-    for (int g0 = 0; g0 < C.nblocks; g0 += RYR_BLOCK) {
-        const int bb = g0 + (int)threadIdx.x;
-        RyrDD t[3];
+    /* groups of 256 chunk sums, each a pairwise tree of four 64-chunk wave
+     * trees (the same tree whatever the number of waves WG gives) */
+    for (int g0 = 0; g0 < C.nblocks; g0 += 256) {
+        for (int sl = wv; sl < 4; sl += NW) {
+            const int bb = g0 + sl * 64 + lane;
+            RyrDD t[3];
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            if (bb < C.nblocks) {
-                /* chunk sum = pairwise tree over its four wave partials */
-                const RyrDD *w = pp + 12LL * bb + q;
-                t[q] = dd_add(dd_add(w[0], w[3]), dd_add(w[6], w[9]));
-            } else { t[q].hi = 0.0; t[q].lo = 0.0; }
-            t[q] = wave_tree(t[q]);
+            for (int q = 0; q < 3; ++q) {
+                if (bb < C.nblocks) {
+                    /* chunk sum = pairwise tree over its four wave partials */
+                    const RyrDD *w = pp + 12LL * bb + q;
+                    t[q] = dd_add(dd_add(w[0], w[3]), dd_add(w[6], w[9]));
+                } else { t[q].hi = 0.0; t[q].lo = 0.0; }
+                t[q] = wave_tree(t[q]);
+            }
+            if (lane == 0) { s_red[0][sl] = t[0]; s_red[1][sl] = t[1]; s_red[2][sl] = t[2]; }
         }
-        if (lane == 0) { s_red[0][wv] = t[0]; s_red[1][wv] = t[1]; s_red[2][wv] = t[2]; }
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < 3; ++q)
@@ -1286,6 +1319,13 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_membrane(RyrKArgs a, double stim_
     }
     vfactors_wave(M, a.iso2, lane, vnew, xnai_new);
     if (lane == 0) M->step = nstep + 1;
+}
+
+/* the membrane step on its own: one workgroup per cell, step gstep + km */
+template <typename TF, typename TS>
+__global__ __launch_bounds__(RYR_BLOCK) void k_membrane(RyrKArgs a, int km, double stim_amp)
+{
+    membrane_block<TF, TS, RYR_BLOCK>(a, (int)blockIdx.x, *a.gstep + km, stim_amp);
 }
 
 /* ------------------------------------------------------- output-row kernel */
@@ -1539,8 +1579,14 @@ __global__ void k_vfactors(RyrMem *mem, int ncells, int iso2)
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c < ncells) vfactors(mem + c, iso2);
 }
+/* step base of the launches that follow: set (add = 0) or advanced (add = 1) */
+__global__ void k_stepbase(long long *gstep, long long v, int add)
+{
+    if (threadIdx.x == 0) *gstep = add ? *gstep + v : v;
+}
 #else
 __global__ void k_vfactors(RyrMem *mem, int ncells, int iso2);
+__global__ void k_stepbase(long long *gstep, long long v, int add);
 #endif
 
 /* field ids follow enum ryr_field of the public header */

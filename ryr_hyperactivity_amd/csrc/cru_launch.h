@@ -5,10 +5,10 @@
  * The sub-voxel kernel and the CRU kernel come in three workgroup sizes
  * (64, 128, 256 lanes).  Every size computes the same values in the same
  * order (the whole-cell sums are formed per wave), so the shape is a pure
- * speed choice.  With a non-null event array the step kernels are launched
- * with hipExtLaunchKernelGGL, which stamps each kernel's own start and end
- * (ev[0..1] sub-voxel, ev[2..3] CRU, ev[4..5] membrane); such launches are
- * never captured into a graph. */
+ * speed choice, and so is carrying the membrane step inside the sub-voxel
+ * launch.  With a non-null event pair a kernel is launched with
+ * hipExtLaunchKernelGGL, which stamps its own start and end; such launches
+ * are never captured into a graph. */
 #ifndef RYR_CRU_LAUNCH_H
 #define RYR_CRU_LAUNCH_H
 
@@ -16,8 +16,12 @@
 #include "cru_device.h"
 
 #define RYR_DECLARE_LAUNCHERS(sfx)                                                               \
-    hipError_t ryr_launch_step_##sfx(const RyrKArgs &a, double amp, int nchunks, int ncells,     \
-                                     int wg_cyto, int wg_cru, hipStream_t s, hipEvent_t *ev);   \
+    hipError_t ryr_launch_cyto_##sfx(const RyrKArgs &a, int nchunks, int wg, int kc, int km,     \
+                                     double amp, hipStream_t s, hipEvent_t *ev);                \
+    hipError_t ryr_launch_cru_##sfx(const RyrKArgs &a, int nchunks, int wg, int kc,              \
+                                    hipStream_t s, hipEvent_t *ev);                             \
+    hipError_t ryr_launch_mem_##sfx(const RyrKArgs &a, int ncells, int km, double amp,           \
+                                    hipStream_t s, hipEvent_t *ev);                             \
     hipError_t ryr_launch_diag_##sfx(const RyrKArgs &a, int nchunks, hipStream_t s,             \
                                      hipEvent_t *ev);                                           \
     hipError_t ryr_launch_init_##sfx(const RyrKArgs &a, const RyrInit *init, int nblocks,        \
@@ -30,61 +34,79 @@
 RYR_DECLARE_LAUNCHERS(f64)
 RYR_DECLARE_LAUNCHERS(f32)
 
-template <typename TF, typename TS, int WG>
-static hipError_t ryr_go_cyto(const RyrKArgs &a, int nchunks, hipStream_t s, hipEvent_t *ev)
+/* One kernel, optionally stamped by its own start/end events (ev[0], ev[1]). */
+#define RYR_GO(KERN, GRID, BLOCK, S, EV, ...)                                                    \
+    do {                                                                                         \
+        if (EV) hipExtLaunchKernelGGL(KERN, GRID, BLOCK, 0, S, (EV)[0], (EV)[1], 0, __VA_ARGS__); \
+        else hipLaunchKernelGGL((KERN), GRID, BLOCK, 0, S, __VA_ARGS__);                         \
+    } while (0)
+
+/* Sub-voxel kernel of step gstep+kc; with km >= 0 it also runs the membrane
+ * step gstep+km of every cell in its first ncells workgroups. */
+template <typename TF, typename TS>
+static hipError_t ryr_go_cyto(const RyrKArgs &a, int nchunks, int wg, int kc, int km, double amp,
+                              hipStream_t s, hipEvent_t *ev)
 {
-    const dim3 grid((unsigned)nchunks * 8u * (unsigned)(RYR_BLOCK / WG));
-    if (ev) hipExtLaunchKernelGGL(k_cyto<TF, TS, WG>, grid, dim3(WG), 0, s, ev[0], ev[1], 0, a);
-    else hipLaunchKernelGGL((k_cyto<TF, TS, WG>), grid, dim3(WG), 0, s, a);
+    const unsigned nmem = km >= 0 ? (unsigned)a.ncells : 0u;
+    if (km >= 0) {
+        switch (wg) {
+        case 64: RYR_GO((k_cyto<TF, TS, 64, true>), dim3(nmem + (unsigned)nchunks * 32u), dim3(64), s, ev, a, kc, km, amp); break;
+        case 128: RYR_GO((k_cyto<TF, TS, 128, true>), dim3(nmem + (unsigned)nchunks * 16u), dim3(128), s, ev, a, kc, km, amp); break;
+        case 256: RYR_GO((k_cyto<TF, TS, 256, true>), dim3(nmem + (unsigned)nchunks * 8u), dim3(256), s, ev, a, kc, km, amp); break;
+        default: return hipErrorInvalidValue;
+        }
+    } else {
+        switch (wg) {
+        case 64: RYR_GO((k_cyto<TF, TS, 64, false>), dim3((unsigned)nchunks * 32u), dim3(64), s, ev, a, kc, km, amp); break;
+        case 128: RYR_GO((k_cyto<TF, TS, 128, false>), dim3((unsigned)nchunks * 16u), dim3(128), s, ev, a, kc, km, amp); break;
+        case 256: RYR_GO((k_cyto<TF, TS, 256, false>), dim3((unsigned)nchunks * 8u), dim3(256), s, ev, a, kc, km, amp); break;
+        default: return hipErrorInvalidValue;
+        }
+    }
     return hipGetLastError();
 }
 
-template <typename TF, typename TS, int WG>
-static hipError_t ryr_go_cru(const RyrKArgs &a, double amp, int nchunks, hipStream_t s, hipEvent_t *ev)
+template <typename TF, typename TS, bool SCHB>
+static hipError_t ryr_go_cru_s(const RyrKArgs &a, int nchunks, int wg, int kc, hipStream_t s, hipEvent_t *ev)
 {
-    const dim3 grid((unsigned)nchunks * (unsigned)(RYR_BLOCK / WG));
-    if (ev) hipExtLaunchKernelGGL(k_cru_step<TF, TS, WG>, grid, dim3(WG), 0, s, ev[2], ev[3], 0, a, amp);
-    else hipLaunchKernelGGL((k_cru_step<TF, TS, WG>), grid, dim3(WG), 0, s, a, amp);
+    switch (wg) {
+    case 64: RYR_GO((k_cru_step<TF, TS, 64, SCHB>), dim3((unsigned)nchunks * 4u), dim3(64), s, ev, a, kc); break;
+    case 128: RYR_GO((k_cru_step<TF, TS, 128, SCHB>), dim3((unsigned)nchunks * 2u), dim3(128), s, ev, a, kc); break;
+    case 256: RYR_GO((k_cru_step<TF, TS, 256, SCHB>), dim3((unsigned)nchunks), dim3(256), s, ev, a, kc); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
 template <typename TF, typename TS>
-static hipError_t ryr_go_step(const RyrKArgs &a, double amp, int nchunks, int ncells, int wg_cyto,
-                              int wg_cru, hipStream_t s, hipEvent_t *ev)
+static hipError_t ryr_go_cru(const RyrKArgs &a, int nchunks, int wg, int kc, hipStream_t s, hipEvent_t *ev)
 {
-    hipError_t e;
-    switch (wg_cyto) {
-    case 64: e = ryr_go_cyto<TF, TS, 64>(a, nchunks, s, ev); break;
-    case 128: e = ryr_go_cyto<TF, TS, 128>(a, nchunks, s, ev); break;
-    case 256: e = ryr_go_cyto<TF, TS, 256>(a, nchunks, s, ev); break;
-    default: return hipErrorInvalidValue;
-    }
-    if (e != hipSuccess) return e;
-    switch (wg_cru) {
-    case 64: e = ryr_go_cru<TF, TS, 64>(a, amp, nchunks, s, ev); break;
-    case 128: e = ryr_go_cru<TF, TS, 128>(a, amp, nchunks, s, ev); break;
-    case 256: e = ryr_go_cru<TF, TS, 256>(a, amp, nchunks, s, ev); break;
-    default: return hipErrorInvalidValue;
-    }
-    if (e != hipSuccess) return e;
-    if (ev) hipExtLaunchKernelGGL(k_membrane<TF, TS>, dim3(ncells), dim3(RYR_BLOCK), 0, s, ev[4], ev[5], 0, a, amp);
-    else hipLaunchKernelGGL((k_membrane<TF, TS>), dim3(ncells), dim3(RYR_BLOCK), 0, s, a, amp);
-    return hipGetLastError();
+    return a.scheme ? ryr_go_cru_s<TF, TS, true>(a, nchunks, wg, kc, s, ev)
+                    : ryr_go_cru_s<TF, TS, false>(a, nchunks, wg, kc, s, ev);
 }
 
 #define RYR_DEFINE_LAUNCHERS(sfx, TF, TS)                                                        \
-    hipError_t ryr_launch_step_##sfx(const RyrKArgs &a, double amp, int nchunks, int ncells,     \
-                                     int wg_cyto, int wg_cru, hipStream_t s, hipEvent_t *ev)    \
+    hipError_t ryr_launch_cyto_##sfx(const RyrKArgs &a, int nchunks, int wg, int kc, int km,     \
+                                     double amp, hipStream_t s, hipEvent_t *ev)                 \
     {                                                                                            \
-        return ryr_go_step<TF, TS>(a, amp, nchunks, ncells, wg_cyto, wg_cru, s, ev);             \
+        return ryr_go_cyto<TF, TS>(a, nchunks, wg, kc, km, amp, s, ev);                          \
+    }                                                                                            \
+    hipError_t ryr_launch_cru_##sfx(const RyrKArgs &a, int nchunks, int wg, int kc,              \
+                                    hipStream_t s, hipEvent_t *ev)                              \
+    {                                                                                            \
+        return ryr_go_cru<TF, TS>(a, nchunks, wg, kc, s, ev);                                    \
+    }                                                                                            \
+    hipError_t ryr_launch_mem_##sfx(const RyrKArgs &a, int ncells, int km, double amp,           \
+                                    hipStream_t s, hipEvent_t *ev)                              \
+    {                                                                                            \
+        RYR_GO((k_membrane<TF, TS>), dim3(ncells), dim3(RYR_BLOCK), s, ev, a, km, amp);          \
+        return hipGetLastError();                                                                \
     }                                                                                            \
     hipError_t ryr_launch_diag_##sfx(const RyrKArgs &a, int nchunks, hipStream_t s,             \
                                      hipEvent_t *ev)                                            \
     {                                                                                            \
         /* one workgroup per 256-CRU chunk */                                                    \
-        if (ev) hipExtLaunchKernelGGL(k_diag<TF, TS>, dim3(nchunks), dim3(RYR_BLOCK), 0, s,      \
-                                      ev[0], ev[1], 0, a);                                      \
-        else hipLaunchKernelGGL((k_diag<TF, TS>), dim3(nchunks), dim3(RYR_BLOCK), 0, s, a);      \
+        RYR_GO((k_diag<TF, TS>), dim3(nchunks), dim3(RYR_BLOCK), s, ev, a);                      \
         return hipGetLastError();                                                                \
     }                                                                                            \
     hipError_t ryr_launch_init_##sfx(const RyrKArgs &a, const RyrInit *init, int nblocks,        \

@@ -30,7 +30,9 @@ struct ryr_handle {
     long long P = 0;
     long long step = 0;
     RyrKArgs a{};
-    std::vector<std::pair<void *, size_t>> allocs;   /* every device allocation, in save order */
+    std::vector<std::pair<void *, size_t>> allocs;   /* simulation state, in snapshot order */
+    std::vector<void *> tables;     /* layout tables and scratch: rebuilt, never saved */
+    long long gbase = 0;            /* value of *a.gstep as the enqueued work will see it; -1 unknown */
     double *stage = nullptr;
     size_t stage_n = 0;
     hipGraphExec_t graph = nullptr;
@@ -38,6 +40,7 @@ struct ryr_handle {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t kev[8] = {};         /* per-kernel start/end stamps: cyto, cru, membrane, diag */
     int wg_cyto = 0, wg_cru = 0;    /* workgroup sizes of the two CRU-parallel kernels; 0: tune */
+    int fuse = -1;                  /* membrane step carried by the next sub-voxel launch; -1: tune */
     int tune_i = 0;
     std::vector<float> tune_ms[2][3];
     bool timing_open = false;
@@ -120,11 +123,23 @@ static double host_fracbound(double cjsr_b)
     return 1 / (1 + abu / aub);
 }
 
-static int dev_alloc(ryr_handle *h, void **ptr, size_t bytes)
+static int dev_alloc(ryr_handle *h, void **ptr, size_t bytes, bool state)
 {
     RYR_HIPCHECK(h, hipMalloc(ptr, bytes > 0 ? bytes : 16));
     RYR_HIPCHECK(h, hipMemset(*ptr, 0, bytes > 0 ? bytes : 16));
-    h->allocs.push_back({*ptr, bytes});
+    if (state) h->allocs.push_back({*ptr, bytes});
+    else h->tables.push_back(*ptr);
+    return RYR_OK;
+}
+
+/* the per-cell table and its per-chunk copies */
+static int upload_info(ryr_handle *h)
+{
+    std::vector<RyrCellInfo> ch(h->nblocks);
+    for (int c = 0; c < h->ncells; ++c)
+        for (int b = 0; b < h->info[c].nblocks; ++b) ch[h->info[c].first_block + b] = h->info[c];
+    RYR_HIPCHECK(h, hipMemcpy((void *)h->a.cells, h->info.data(), sizeof(RyrCellInfo) * h->ncells, hipMemcpyHostToDevice));
+    RYR_HIPCHECK(h, hipMemcpy((void *)h->a.chunks, ch.data(), sizeof(RyrCellInfo) * h->nblocks, hipMemcpyHostToDevice));
     return RYR_OK;
 }
 
@@ -215,6 +230,7 @@ extern "C" int ryr_create(int device, const ryr_params *p, const ryr_cell_desc *
         int jz = 5 < d.nz ? 5 : d.nz;        /* :972, clamped to the interior */
         I.line_j = jy - 1;
         I.line_k = jz - 1;
+        I.cell = c;
         h->line_off[c] = lo;
         lo += (long long)d.nx * RYR_LINE_COLS;
     }
@@ -231,7 +247,9 @@ extern "C" int ryr_create(int device, const ryr_params *p, const ryr_cell_desc *
     a.iso2 = p->iso2;
     a.spo = p->steps_per_output;
     a.ncells = ncells;
-#define ALLOC(ptr, bytes) do { void *q_ = nullptr; if (dev_alloc(h, &q_, (bytes))) return fail(RYR_E_HIP); (ptr) = decltype(ptr)(q_); } while (0)
+#define ALLOCX(ptr, bytes, st) do { void *q_ = nullptr; if (dev_alloc(h, &q_, (bytes), st)) return fail(RYR_E_HIP); (ptr) = decltype(ptr)(q_); } while (0)
+#define ALLOC(ptr, bytes) ALLOCX(ptr, bytes, true)
+#define ALLOCT(ptr, bytes) ALLOCX(ptr, bytes, false)
     for (int b = 0; b < 2; ++b) {
         ALLOC(a.ci[b], 8 * P * tf);
         ALLOC(a.cnsr[b], 8 * P * ts);
@@ -242,32 +260,36 @@ extern "C" int ryr_create(int device, const ryr_params *p, const ryr_cell_desc *
     ALLOC(a.cp, P * tf); ALLOC(a.cjsr, P * ts); ALLOC(a.cjsrb, P * ts); ALLOC(a.Ancx, P * tf);
     ALLOC(a.casar, P * tf); ALLOC(a.casarh, P * tf); ALLOC(a.casarj, P * tf); ALLOC(a.casarhj, P * tf);
     ALLOC(a.lccw, P * 4); ALLOC(a.ryrw, P * 4); ALLOC(a.aux, P * 4);
-    ALLOC(a.d_xire, P * 8); ALLOC(a.d_xinaca, P * 8); ALLOC(a.d_up, 8 * P * 8); ALLOC(a.d_leak, 8 * P * 8);
-    ALLOC(a.sb, 2 * P * tf);
-    RyrCellInfo *d_cells; RyrMem *d_mem; int *d_bc; RyrDD *d_part;
+    ALLOC(a.d_xire, P * 8); ALLOC(a.d_xinaca, P * 8); ALLOCT(a.d_up, 8 * P * 8); ALLOCT(a.d_leak, 8 * P * 8);
+    ALLOCT(a.sb, 2 * P * tf);
+    RyrCellInfo *d_cells, *d_chunks; RyrMem *d_mem; int *d_bc; RyrDD *d_part;
     unsigned long long *d_cnt; double *d_diag; long long *d_dstep; double *d_line; long long *d_loff;
     ALLOC(d_mem, sizeof(RyrMem) * ncells);
     ALLOC(d_cnt, sizeof(unsigned long long) * ncells * RYR_NCNT);
     ALLOC(d_diag, sizeof(double) * ncells * RYR_DIAG_COLS);
     ALLOC(d_dstep, sizeof(long long) * ncells);
     ALLOC(d_line, sizeof(double) * (lo > 0 ? lo : 1));
-    ALLOC(d_cells, sizeof(RyrCellInfo) * ncells);
-    ALLOC(d_bc, sizeof(int) * h->nblocks);
-    ALLOC(d_part, sizeof(RyrDD) * 12 * h->nblocks);
-    ALLOC(d_loff, sizeof(long long) * ncells);
+    ALLOCT(d_cells, sizeof(RyrCellInfo) * ncells);
+    ALLOCT(d_chunks, sizeof(RyrCellInfo) * h->nblocks);
+    ALLOCT(d_bc, sizeof(int) * h->nblocks);
+    ALLOCT(d_part, sizeof(RyrDD) * 12 * h->nblocks);
+    ALLOCT(d_loff, sizeof(long long) * ncells);
+    ALLOCT(a.gstep, sizeof(long long));
     ALLOC(a.beats, sizeof(double) * ncells * RYR_MAXB * RYR_BCOLS);
-    ALLOC(a.dpartial, sizeof(RyrDD) * RYR_DIAG_Q * h->nblocks);
-    ALLOC(a.dcount, sizeof(unsigned long long) * RYR_DIAG_I * ncells);
-    ALLOC(a.darrive, sizeof(unsigned int) * ncells);
+    ALLOCT(a.dpartial, sizeof(RyrDD) * RYR_DIAG_Q * h->nblocks);
+    ALLOCT(a.dcount, sizeof(unsigned long long) * RYR_DIAG_I * ncells);
+    ALLOCT(a.darrive, sizeof(unsigned int) * ncells);
 #undef ALLOC
+#undef ALLOCT
+#undef ALLOCX
     h->vclamp.assign(ncells, nullptr);
-    a.cells = d_cells; a.mem = d_mem; a.block_cell = d_bc; a.partial = d_part;
+    a.cells = d_cells; a.chunks = d_chunks; a.mem = d_mem; a.block_cell = d_bc; a.partial = d_part;
     a.counters = d_cnt; a.diag = d_diag; a.diag_step = d_dstep; a.line = d_line; a.line_off = d_loff;
 
     auto H2D = [&](void *dst, const void *src, size_t n) {
         return hipMemcpy(dst, src, n, hipMemcpyHostToDevice) == hipSuccess;
     };
-    if (!H2D(d_cells, h->info.data(), sizeof(RyrCellInfo) * ncells) ||
+    if (upload_info(h) != RYR_OK ||
         !H2D(d_bc, block_cell.data(), sizeof(int) * h->nblocks) ||
         !H2D(d_loff, h->line_off.data(), sizeof(long long) * ncells)) {
         h->err = "hipMemcpy of cell tables failed";
@@ -337,16 +359,86 @@ extern "C" int ryr_create(int device, const ryr_params *p, const ryr_cell_desc *
     return RYR_OK;
 }
 
-static hipError_t launch_one(ryr_handle *h, long long s, int wg_cyto, int wg_cru, bool timed)
+/* ---- step sequencing
+ * Kernels take their step as an offset k from a base held on the device
+ * (*a.gstep), so one captured output interval replays with unchanged
+ * arguments.  h->gbase mirrors that base for the work already enqueued. */
+static hipError_t set_base(ryr_handle *h, long long v)
 {
-    const bool f32 = h->p.precision == RYR_PREC_F32;
-    hipEvent_t *ev = timed ? h->kev : nullptr;
-    hipError_t e = f32 ? ryr_launch_step_f32(h->a, h->p.stim_amp, h->nblocks, h->ncells, wg_cyto, wg_cru, h->stream, ev)
-                       : ryr_launch_step_f64(h->a, h->p.stim_amp, h->nblocks, h->ncells, wg_cyto, wg_cru, h->stream, ev);
-    if (e != hipSuccess) return e;
-    if (s % h->p.steps_per_output == 0)
-        e = f32 ? ryr_launch_diag_f32(h->a, h->nblocks, h->stream, timed ? h->kev + 6 : nullptr)
-                : ryr_launch_diag_f64(h->a, h->nblocks, h->stream, timed ? h->kev + 6 : nullptr);
+    hipLaunchKernelGGL(k_stepbase, dim3(1), dim3(64), 0, h->stream, h->a.gstep, v, 0);
+    h->gbase = v;
+    return hipGetLastError();
+}
+
+/* offsets must stay small and non-negative */
+static hipError_t ensure_base(ryr_handle *h, long long nahead)
+{
+    if (h->gbase < 0 || h->step < h->gbase || h->step + nahead - h->gbase > (1LL << 30))
+        return set_base(h, h->step);
+    return hipSuccess;
+}
+
+#define RYR_F32(h) ((h)->p.precision == RYR_PREC_F32)
+static hipError_t go_cyto(ryr_handle *h, int wg, int kc, int km, hipEvent_t *ev)
+{
+    return RYR_F32(h) ? ryr_launch_cyto_f32(h->a, h->nblocks, wg, kc, km, h->p.stim_amp, h->stream, ev)
+                      : ryr_launch_cyto_f64(h->a, h->nblocks, wg, kc, km, h->p.stim_amp, h->stream, ev);
+}
+static hipError_t go_cru(ryr_handle *h, int wg, int kc, hipEvent_t *ev)
+{
+    return RYR_F32(h) ? ryr_launch_cru_f32(h->a, h->nblocks, wg, kc, h->stream, ev)
+                      : ryr_launch_cru_f64(h->a, h->nblocks, wg, kc, h->stream, ev);
+}
+static hipError_t go_mem(ryr_handle *h, int km, hipEvent_t *ev)
+{
+    return RYR_F32(h) ? ryr_launch_mem_f32(h->a, h->ncells, km, h->p.stim_amp, h->stream, ev)
+                      : ryr_launch_mem_f64(h->a, h->ncells, km, h->p.stim_amp, h->stream, ev);
+}
+static hipError_t go_diag(ryr_handle *h, hipEvent_t *ev)
+{
+    return RYR_F32(h) ? ryr_launch_diag_f32(h->a, h->nblocks, h->stream, ev)
+                      : ryr_launch_diag_f64(h->a, h->nblocks, h->stream, ev);
+}
+
+/* Steps h->step .. h->step+n-1 (h->step itself is not advanced here).
+ * Step m: the sub-voxel kernel, carrying the membrane step of m-1 when that
+ * one is still pending; the CRU kernel; and the membrane step of m on its own
+ * when m is an output step (then the output-row kernel follows), the last
+ * step of the run (so every run ends with the whole state advanced) or when
+ * fusion is off. */
+static hipError_t enqueue_run(ryr_handle *h, long long n, bool fuse)
+{
+    const long long spo = h->p.steps_per_output;
+    bool pending = false;
+    hipError_t e = hipSuccess;
+    for (long long i = 0; i < n && e == hipSuccess; ++i) {
+        const long long m = h->step + i;
+        const int k = (int)(m - h->gbase);
+        e = go_cyto(h, h->wg_cyto, k, pending ? k - 1 : -1, nullptr);
+        if (e == hipSuccess) e = go_cru(h, h->wg_cru, k, nullptr);
+        const bool out = m % spo == 0;
+        if (e == hipSuccess && (out || i == n - 1 || !fuse)) {
+            e = go_mem(h, k, nullptr);
+            if (e == hipSuccess && out) e = go_diag(h, nullptr);
+            pending = false;
+        } else {
+            pending = true;
+        }
+    }
+    return e;
+}
+
+/* One step with every kernel on its own and stamped by its own events
+ * (kev[0..1] sub-voxel, [2..3] CRU, [4..5] membrane, [6..7] output row);
+ * returns whether it was an output step. */
+static hipError_t enqueue_timed_step(ryr_handle *h, int wg_cyto, int wg_cru, bool *out)
+{
+    const int k = (int)(h->step - h->gbase);
+    *out = h->step % h->p.steps_per_output == 0;
+    hipError_t e = go_cyto(h, wg_cyto, k, -1, h->kev);
+    if (e == hipSuccess) e = go_cru(h, wg_cru, k, h->kev + 2);
+    if (e == hipSuccess) e = go_mem(h, k, h->kev + 4);
+    if (e == hipSuccess && *out) e = go_diag(h, h->kev + 6);
     return e;
 }
 
@@ -360,15 +452,18 @@ static void drop_graph(ryr_handle *h)
 /* Shape tuning: the first steps of a handle alternate the three workgroup
  * sizes, each step's kernels stamped by their own events; after
  * RYR_TUNE_ROUNDS rounds each kernel keeps the size with the smallest median
- * duration.  The tuning steps are ordinary steps of the simulation (every
- * shape computes identical values), so tuning changes no result. */
+ * duration, 256 unless another is faster by more than 3%.  The tuning steps
+ * are ordinary steps of the simulation (every shape computes identical
+ * values), so tuning changes no result. */
 #define RYR_TUNE_ROUNDS 3
 static int tune_step(ryr_handle *h)
 {
     const int v = h->tune_i % 3;
     const int wg = 64 << v;
-    RYR_HIPCHECK(h, launch_one(h, h->step, wg, wg, true));
-    RYR_HIPCHECK(h, hipEventSynchronize(h->kev[5]));
+    bool out = false;
+    RYR_HIPCHECK(h, ensure_base(h, 1));
+    RYR_HIPCHECK(h, enqueue_timed_step(h, wg, wg, &out));
+    RYR_HIPCHECK(h, hipEventSynchronize(out ? h->kev[7] : h->kev[5]));
     for (int k = 0; k < 2; ++k) {
         float ms = 0.0f;
         RYR_HIPCHECK(h, hipEventElapsedTime(&ms, h->kev[2 * k], h->kev[2 * k + 1]));
@@ -377,14 +472,16 @@ static int tune_step(ryr_handle *h)
     if (++h->tune_i == 3 * RYR_TUNE_ROUNDS) {
         int best[2] = {256, 256};
         for (int k = 0; k < 2; ++k) {
-            float bm = 1e30f;
+            float med[3];
             for (int q = 0; q < 3; ++q) {
                 std::vector<float> t = h->tune_ms[k][q];
                 std::sort(t.begin(), t.end());
-                const float med = t[t.size() / 2];
-                if (med < bm) { bm = med; best[k] = 64 << q; }
+                med[q] = t[t.size() / 2];
                 h->tune_ms[k][q].clear();
             }
+            float bm = med[2];
+            for (int q = 1; q >= 0; --q)
+                if (med[q] < 0.97f * bm) { bm = med[q]; best[k] = 64 << q; }
         }
         h->wg_cyto = best[0];
         h->wg_cru = best[1];
@@ -393,7 +490,34 @@ static int tune_step(ryr_handle *h)
     return RYR_OK;
 }
 
-/* one output interval (spo steps starting at an output step) as a graph */
+/* Fusion tuning, after the shapes: pairs of steps with and without the
+ * membrane step carried by the sub-voxel launch, alternating, timed as a
+ * whole by events around them; fusion is kept if its median is lower by more
+ * than 3%.  Advances 4 * RYR_TUNE_ROUNDS steps. */
+static int tune_fusion(ryr_handle *h)
+{
+    std::vector<float> t[2];
+    for (int r = 0; r < RYR_TUNE_ROUNDS; ++r)
+        for (int f = 0; f < 2; ++f) {
+            RYR_HIPCHECK(h, ensure_base(h, 2));
+            RYR_HIPCHECK(h, hipEventRecord(h->kev[0], h->stream));
+            RYR_HIPCHECK(h, enqueue_run(h, 2, f == 1));
+            RYR_HIPCHECK(h, hipEventRecord(h->kev[1], h->stream));
+            RYR_HIPCHECK(h, hipEventSynchronize(h->kev[1]));
+            float ms = 0.0f;
+            RYR_HIPCHECK(h, hipEventElapsedTime(&ms, h->kev[0], h->kev[1]));
+            t[f].push_back(ms);
+            h->step += 2;
+            h->launches += 2;
+        }
+    for (auto &v : t) std::sort(v.begin(), v.end());
+    h->fuse = t[1][t[1].size() / 2] < 0.97f * t[0][t[0].size() / 2] ? 1 : 0;
+    drop_graph(h);
+    return RYR_OK;
+}
+
+/* one output interval (spo steps starting at an output step, offsets 0..spo-1
+ * from the base) and the advance of the base, as a graph */
 static bool build_graph(ryr_handle *h)
 {
     if (h->graph) return true;
@@ -402,7 +526,11 @@ static bool build_graph(ryr_handle *h)
     const long long spo = h->p.steps_per_output;
     bool ok = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
     if (ok) {
-        for (long long s = 0; s < spo && ok; ++s) ok = launch_one(h, s, h->wg_cyto, h->wg_cru, false) == hipSuccess;
+        ok = enqueue_run(h, spo, h->fuse == 1) == hipSuccess;
+        if (ok) {
+            hipLaunchKernelGGL(k_stepbase, dim3(1), dim3(64), 0, h->stream, h->a.gstep, spo, 1);
+            ok = hipGetLastError() == hipSuccess;
+        }
         hipError_t e = hipStreamEndCapture(h->stream, &g);
         ok = ok && e == hipSuccess && g;
     }
@@ -434,18 +562,31 @@ extern "C" int ryr_step(ryr_handle *h, int64_t nsteps)
         h->launches += 1;
         nsteps -= 1;
     }
+    if (h->fuse < 0 && nsteps >= 4 * RYR_TUNE_ROUNDS) {
+        int rc = tune_fusion(h);
+        if (rc) return rc;
+        nsteps -= 4 * RYR_TUNE_ROUNDS;
+    }
     while (nsteps > 0) {
-        if (h->step % spo == 0 && nsteps >= spo && spo >= 4 && build_graph(h)) {
-            RYR_HIPCHECK(h, hipGraphLaunch(h->graph, h->stream));
-            h->step += spo;
-            h->launches += spo;
-            nsteps -= spo;
-            continue;
+        if (h->step % spo == 0 && nsteps >= spo && spo >= 4) {
+            if (h->gbase != h->step) RYR_HIPCHECK(h, set_base(h, h->step));
+            if (build_graph(h)) {
+                RYR_HIPCHECK(h, hipGraphLaunch(h->graph, h->stream));
+                h->step += spo;
+                h->gbase += spo;
+                h->launches += spo;
+                nsteps -= spo;
+                continue;
+            }
         }
-        RYR_HIPCHECK(h, launch_one(h, h->step, h->wg_cyto, h->wg_cru, false));
-        h->step += 1;
-        h->launches += 1;
-        nsteps -= 1;
+        /* eager run up to the next output step (or the end) */
+        long long n = spo - h->step % spo;
+        if (n > nsteps) n = nsteps;
+        RYR_HIPCHECK(h, ensure_base(h, n));
+        RYR_HIPCHECK(h, enqueue_run(h, n, h->fuse == 1));
+        h->step += n;
+        h->launches += n;
+        nsteps -= n;
     }
     RYR_HIPCHECK(h, hipEventRecord(h->ev1, h->stream));
     return RYR_OK;
@@ -475,30 +616,33 @@ extern "C" int ryr_kernel_time(ryr_handle *h, double *ms, int64_t *launches)
     return RYR_OK;
 }
 
-extern "C" int ryr_set_launch_shape(ryr_handle *h, int wg_cyto, int wg_cru)
+extern "C" int ryr_set_launch_shape(ryr_handle *h, int wg_cyto, int wg_cru, int fuse)
 {
     if (!h) return RYR_E_ARG;
     auto ok = [](int w) { return w == 64 || w == 128 || w == 256; };
     if (wg_cyto == 0 && wg_cru == 0) {
         h->wg_cyto = h->wg_cru = 0;
+        h->fuse = -1;
         h->tune_i = 0;
         for (auto &k : h->tune_ms) for (auto &q : k) q.clear();
-    } else if (ok(wg_cyto) && ok(wg_cru)) {
+    } else if (ok(wg_cyto) && ok(wg_cru) && (fuse == 0 || fuse == 1)) {
         h->wg_cyto = wg_cyto;
         h->wg_cru = wg_cru;
+        h->fuse = fuse;
     } else {
-        h->err = "workgroup sizes must be 64, 128 or 256 (or both 0 to tune)";
+        h->err = "workgroup sizes must be 64, 128 or 256 and fuse 0 or 1 (or sizes 0, 0 to tune)";
         return RYR_E_ARG;
     }
     drop_graph(h);
     return RYR_OK;
 }
 
-extern "C" int ryr_get_launch_shape(const ryr_handle *h, int *wg_cyto, int *wg_cru)
+extern "C" int ryr_get_launch_shape(const ryr_handle *h, int *wg_cyto, int *wg_cru, int *fuse)
 {
-    if (!h || !wg_cyto || !wg_cru) return RYR_E_ARG;
+    if (!h || !wg_cyto || !wg_cru || !fuse) return RYR_E_ARG;
     *wg_cyto = h->wg_cyto;
     *wg_cru = h->wg_cru;
+    *fuse = h->fuse;
     return RYR_OK;
 }
 
@@ -513,8 +657,9 @@ extern "C" int ryr_profile_kernels(ryr_handle *h, int64_t nsteps, double *avg_us
     }
     double tot[RYR_NPROF] = {0, 0, 0, 0};
     for (int64_t i = 0; i < nsteps; ++i) {
-        const bool d = h->step % h->p.steps_per_output == 0;
-        RYR_HIPCHECK(h, launch_one(h, h->step, h->wg_cyto, h->wg_cru, true));
+        bool d = false;
+        RYR_HIPCHECK(h, ensure_base(h, 1));
+        RYR_HIPCHECK(h, enqueue_timed_step(h, h->wg_cyto, h->wg_cru, &d));
         RYR_HIPCHECK(h, hipEventSynchronize(d ? h->kev[7] : h->kev[5]));
         for (int k = 0; k < (d ? 4 : 3); ++k) {
             float ms = 0.0f;
@@ -696,8 +841,7 @@ extern "C" int ryr_set_vclamp(ryr_handle *h, int cell, const double *v, int64_t 
         I.vclamp = h->vclamp[cell];
         I.vclamp_n = n;
     }
-    RYR_HIPCHECK(h, hipMemcpy((RyrCellInfo *)h->a.cells + cell, &I, sizeof I, hipMemcpyHostToDevice));
-    return RYR_OK;
+    return upload_info(h);
 }
 
 extern "C" int ryr_cru_steps(ryr_handle *h, int64_t *out)
@@ -766,6 +910,7 @@ extern "C" int ryr_load(ryr_handle *h, const char *path)
     std::fclose(f);
     if (!ok) { h->err = "snapshot does not match this handle or is corrupt"; return RYR_E_STATE; }
     h->step = hd.step;
+    h->gbase = -1;
     return RYR_OK;
 }
 
@@ -776,6 +921,7 @@ extern "C" void ryr_destroy(ryr_handle *h)
     if (h->stream) hipStreamSynchronize(h->stream);
     if (h->graph) hipGraphExecDestroy(h->graph);
     for (auto &al : h->allocs) hipFree(al.first);
+    for (void *t : h->tables) hipFree(t);
     if (h->stage) hipFree(h->stage);
     for (double *q : h->vclamp) if (q) hipFree(q);
     for (auto &e : h->kev) if (e) hipEventDestroy(e);

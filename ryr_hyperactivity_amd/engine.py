@@ -78,14 +78,18 @@ class CellEngine:
         self._check(self.lib.ryr_kernel_time(self.h, ctypes.byref(ms), ctypes.byref(n)), "ryr_kernel_time")
         return ms.value, n.value
 
-    def set_launch_shape(self, wg_cyto: int, wg_cru: int):
-        """Workgroup sizes (64/128/256) of the sub-voxel and CRU kernels; (0, 0) re-tunes."""
-        self._check(self.lib.ryr_set_launch_shape(self.h, int(wg_cyto), int(wg_cru)), "ryr_set_launch_shape")
+    def set_launch_shape(self, wg_cyto: int, wg_cru: int, fuse: int = 1):
+        """Workgroup sizes (64/128/256) of the sub-voxel and CRU kernels and whether
+        the membrane step rides in the next sub-voxel launch; sizes (0, 0) re-tune."""
+        self._check(self.lib.ryr_set_launch_shape(self.h, int(wg_cyto), int(wg_cru), int(fuse)),
+                    "ryr_set_launch_shape")
 
     def launch_shape(self):
-        a, b = ctypes.c_int(), ctypes.c_int()
-        self._check(self.lib.ryr_get_launch_shape(self.h, ctypes.byref(a), ctypes.byref(b)), "ryr_get_launch_shape")
-        return a.value, b.value
+        """(wg_cyto, wg_cru, fuse); (0, 0, -1) while tuning is pending."""
+        a, b, f = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.ryr_get_launch_shape(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(f)),
+                    "ryr_get_launch_shape")
+        return a.value, b.value, f.value
 
     def profile_kernels(self, nsteps: int):
         """Advance nsteps steps with per-kernel event stamps; returns

@@ -194,11 +194,12 @@ def test_beats_and_clamp_match_the_oracle():
         assert gb.shape[0] == 3 and list(gb[:, 9]) == [1.0, 1.0, 0.0]
 
 
-@pytest.mark.parametrize("shape", [(64, 64), (128, 256), (256, 128)])
+@pytest.mark.parametrize("shape", [(64, 64, 1), (128, 256, 0), (256, 128, 1), (256, 256, 0)])
 def test_every_launch_shape_is_bitexact(shape):
-    """Workgroup sizes change no value: FP64 against the oracle, and FP32
-    against the FP32 run at the default shape (whole-cell sums are formed per
-    wave, so the reduction order is the same for every size)."""
+    """Workgroup sizes and membrane fusion change no value: FP64 against the
+    oracle, and FP32 against the FP32 run at another shape (whole-cell sums
+    are formed per wave, so the reduction order is the same for every size;
+    the fused membrane step reads only the CRU kernel's sums)."""
     print(f"case shape={shape}", flush=True)
     p = Params.default(stim_offset=2.0)
     d = CellDesc.default(nx=20, ny=13, nz=5, cl=1000.0, seed=5)
@@ -211,7 +212,7 @@ def test_every_launch_shape_is_bitexact(shape):
         _assert_same_state(eng, 0, o, f"shape {shape}")
     p32 = Params.default(stim_offset=2.0, precision=PREC_F32)
     with CellEngine(p32, d) as a, CellEngine(p32, d) as b:
-        a.set_launch_shape(256, 256)
+        a.set_launch_shape(256, 256, 1 - shape[2])
         b.set_launch_shape(*shape)
         a.step(250)
         b.step(250)
@@ -224,17 +225,17 @@ def test_tuning_and_kernel_profile():
     p = Params.default(precision=PREC_F32)
     d = CellDesc.default(nx=30, ny=10, nz=6, seed=2)
     with CellEngine(p, d) as eng:
-        assert eng.launch_shape() == (0, 0)        # tuning pending
+        assert eng.launch_shape() == (0, 0, -1)    # tuning pending
         eng.step(5)
-        assert eng.launch_shape() == (0, 0)
-        eng.step(20)                               # tuning ends after 9 steps
-        wc, wr = eng.launch_shape()
-        assert wc in (64, 128, 256) and wr in (64, 128, 256)
+        assert eng.launch_shape() == (0, 0, -1)
+        eng.step(30)                               # sizes after 9 steps, fusion after 12 more
+        wc, wr, fu = eng.launch_shape()
+        assert wc in (64, 128, 256) and wr in (64, 128, 256) and fu in (0, 1)
         prof = eng.profile_kernels(120)
-        assert eng.cru_steps() == 145 * d.ncru
+        assert eng.cru_steps() == 155 * d.ncru
         for k in ("k_cyto", "k_cru_step", "k_membrane"):
             us, n = prof[k]
             assert n == 120 and 0.5 < us < 5e4, (k, us, n)
-        assert prof["k_diag"][1] == 1              # steps 25..144 hold one output step (100)
-        eng.set_launch_shape(0, 0)
-        assert eng.launch_shape() == (0, 0)
+        assert prof["k_diag"][1] == 1              # steps 35..154 hold one output step (100)
+        eng.set_launch_shape(0, 0, 0)
+        assert eng.launch_shape() == (0, 0, -1)

@@ -266,3 +266,35 @@ def test_output_row_columns_recomputed_from_the_state():
     tot_ci = cyt + (m1["CS"] + m1["CASAR"] + m1["CASARH"]) * Vs + (m1["CP"] + m1["CASARJ"] + m1["CASARHJ"]) * Vp
     assert row[43] == pytest.approx(tot_ci, **close)
     assert row[44] == pytest.approx(tot_ci + m8["CNSR"] * Vnsr * 8 + (m1["CJSR"] + m1["CJSRB"]) * Vjsr, **close)
+
+
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_uniform_equilibrium_is_a_fixed_point_of_both_schemes(scheme):
+    """With every compartment at one concentration c, every buffer at its own
+    steady state for c (myosin with Mg 500 uM competing), and the membrane
+    sources off (no LCC, no NCX, no SERCA, no stimulus), no calcium flux is
+    left in either cleft scheme (cuda_Cell_Detailed.cu:1356-1473 and
+    :1475-1597): the calcium state must stay where it is.  A wrong buffer
+    constant, a mis-scaled volume ratio on a non-difference term or a flux
+    applied twice moves it."""
+    c = 0.3
+    p = Params.default(scheme=scheme, tapsi=1, lqt=0, nlcc=0, stim_amp=0.0)   # tapsi+lqt would keep SERCA at 0.05
+    d = CellDesc.default(nx=4, ny=3, nz=3, ncx_scale=0.0, seed=9)
+    o = OracleCell(p, d)
+    n = d.ncru
+    ss = lambda kon, koff, B: kon * c * B / (kon * c + koff)      # simple buffer: kon c (B-b) = koff b
+    free = 140.0 / (1 + 13.8 * c / 0.46 + 0.0157 * 500.0 / 0.057)  # myosin: Ca and Mg sites share B = 140
+    values8 = {"CI": c, "CNSR": c, "CATF": ss(0.0327, 0.0196, 70.0), "CATS": ss(0.00254, 0.000033, 134.0),
+               "CACAL": ss(0.0543, 0.238, 24.0), "CASR": ss(0.1, 0.06, 19.0),
+               "CAMYO": 13.8 * c * free / 0.46, "MGMYO": 0.0157 * 500.0 * free / 0.057}
+    vr = (0.5 / 8.0) * 8.0 / 0.025          # sarcolemmal sites scale with Vcyt/Vsub (buffers.c:118-155)
+    values1 = {"CS": c, "CP": c, "CJSR": c, "CASAR": ss(0.1, 1.3, 42.0 * vr), "CASARH": ss(0.1, 0.03, 15.0 * vr),
+               "CASARJ": ss(0.1, 1.3, 42.0 * vr), "CASARHJ": ss(0.1, 0.03, 15.0 * vr)}
+    for f, v in values8.items():
+        o.set_field(f, np.full(8 * n, v))
+    for f, v in values1.items():
+        o.set_field(f, np.full(n, v))
+    before = {f: o.field(f).copy() for f in list(values8) + list(values1) + ["CJSRB"]}
+    o.step(200)
+    for f, v in before.items():
+        assert np.allclose(o.field(f), v, rtol=1e-11, atol=0), f
