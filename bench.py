@@ -79,7 +79,7 @@ def make_batch(prec, ncells=256, seed=2024):
     return Params.default(precision=prec), cells
 
 
-def run_gpu(p, cells, steps, warmup, prof_steps):
+def run_gpu(p, cells, steps, warmup, prof_steps, shape=None):
     """Timed region: `steps` graph-replayed steps, bracketed by a device
     synchronisation on both sides (wall clock) and by HIP events on the
     engine's stream.  Then `prof_steps` more steps with every kernel stamped
@@ -87,7 +87,11 @@ def run_gpu(p, cells, steps, warmup, prof_steps):
     from ryr_hyperactivity_amd import CellEngine
     ncru = sum(c.ncru for c in cells)
     with CellEngine(p, cells, device=0) as eng:
+        if shape:
+            eng.set_launch_shape(*shape)
         eng.step(warmup)
+        while eng.launch_shape()[2] < 0:       # the shape tuner runs inside the warm-up, never in the timed region
+            eng.step(50)
         eng.sync()
         eng.kernel_time()                      # reset the event window
         t0 = time.perf_counter()
@@ -121,7 +125,7 @@ def run_gpu(p, cells, steps, warmup, prof_steps):
         "ncru": ncru,
         "step_bytes": ncru * alg_bytes_per_cru_step(p.precision),
         "step_GBs": ncru * alg_bytes_per_cru_step(p.precision) / step_s / 1e9,
-        "launch_shape": {"k_cyto": shape[0], "k_cru_step": shape[1], "membrane_fused": shape[2]},
+        "launch_shape": {"k_cyto": shape[0], "k_cru_step": shape[1], "schedule": ["sequential", "membrane-fused", "concurrent"][shape[2]]},
         "kernels": kernels,
         "dominant": dom,
     }
@@ -178,6 +182,8 @@ def main():
                     help="steps with per-kernel event stamps after the timed region (default 400 cell / 100 batch)")
     ap.add_argument("--cpu-steps", type=int, default=4000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--shape", default=None,
+                    help="fix the launch shape WG_CYTO,WG_CRU,SCHED instead of tuning it (e.g. 256,64,2)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (independent cells shard trivially across GPUs; "
@@ -187,16 +193,17 @@ def main():
     steps = args.steps if args.steps is not None else 8000
     warmup = args.warmup if args.warmup is not None else 400
 
+    shape = tuple(int(x) for x in args.shape.split(",")) if args.shape else None
     p, cells = make_cell(prec)
     r = None
     if args.workload in ("cell", "both"):
-        r = run_gpu(p, cells, steps, warmup, args.prof_steps or 400)
+        r = run_gpu(p, cells, steps, warmup, args.prof_steps or 400, shape)
         print(f"# cell: {json.dumps(r)}", file=sys.stderr, flush=True)
     batch = None
     if args.workload in ("batch", "both"):
         pb, bcells = make_batch(prec)
         bw = args.warmup if args.warmup is not None else 100
-        rb = run_gpu(pb, bcells, args.batch_steps, bw, args.prof_steps or 100)
+        rb = run_gpu(pb, bcells, args.batch_steps, bw, args.prof_steps or 100, shape)
         print(f"# batch: {json.dumps(rb)}", file=sys.stderr, flush=True)
         batch = {"value": rb["cell_ms_per_s"], "unit": "simulated cell-ms/wall-s", "ncells": len(bcells),
                  "ncru": rb["ncru"], "steps": args.batch_steps, "warmup": bw, "ms_per_step": rb["ms_per_step"],

@@ -227,16 +227,27 @@ int ryr_kernel_time(ryr_handle *h, double *ms, int64_t *launches);
 
 /* Launch shape of a step: the workgroup sizes (64, 128 or 256 lanes) of the
  * two CRU-parallel kernels (the sub-voxel cytosol/NSR kernel and the CRU
- * kernel), and `fuse`: whether the membrane step of step n runs inside the
- * sub-voxel launch of step n+1 (1) or on its own (0).  No reference
- * counterpart (the reference hard-codes a 48x448 launch, :1701).  A new
- * handle tunes the shape itself: its first 9 steps alternate the sizes with
- * each kernel timed by its own events, the next 12 compare fused and unfused
- * step pairs, and the fastest medians are kept.  Every shape computes
- * identical values, so tuning never changes a result.  Passing sizes (0, 0)
- * re-arms the tuning; the getter reports 0 sizes (fuse -1) while pending. */
-int ryr_set_launch_shape(ryr_handle *h, int wg_cyto, int wg_cru, int fuse);
-int ryr_get_launch_shape(const ryr_handle *h, int *wg_cyto, int *wg_cru, int *fuse);
+ * kernel), and the schedule of a step's kernels:
+ *   RYR_SCHED_SEQ         sub-voxel kernel, CRU kernel, membrane step
+ *   RYR_SCHED_FUSED       the membrane step of step n rides in the sub-voxel
+ *                         launch of step n+1
+ *   RYR_SCHED_CONCURRENT  the sub-voxel and CRU kernels of a step run side by
+ *                         side on two streams (cleft scheme A only)
+ * No reference counterpart (the reference hard-codes one 48x448 launch,
+ * :1701).  A new handle tunes the shape itself: its first 9 steps alternate
+ * the sizes with each kernel timed by its own events (fastest median kept);
+ * then each schedule candidate (the tuned sizes with every schedule, and the
+ * concurrent schedule with smaller sizes) runs 12 steps as three replays of a
+ * captured 4-step graph, timed by events, and the fastest is kept.  Output
+ * steps are stepped over eagerly, so tuning takes about 100 steps.  Every
+ * shape computes identical values, so tuning never changes a result.  Passing
+ * sizes (0, 0) re-arms the tuning; the getter reports sizes 0 and schedule -1
+ * while it is pending. */
+#define RYR_SCHED_SEQ        0
+#define RYR_SCHED_FUSED      1
+#define RYR_SCHED_CONCURRENT 2
+int ryr_set_launch_shape(ryr_handle *h, int wg_cyto, int wg_cru, int sched);
+int ryr_get_launch_shape(const ryr_handle *h, int *wg_cyto, int *wg_cru, int *sched);
 
 /* Advances nsteps steps with every kernel launched on its own (no graph) and
  * stamped by HIP events at its start and end; avg_us[k] / count[k] receive the

@@ -50,7 +50,20 @@ struct RyrCellInfo {
     const double *vclamp;   /* V protocol (device array) or null */
     long long vclamp_n;
     int cell, pad2_;        /* index of the cell (per-chunk copies carry it) */
+    /* n / X and n / (X*Y) by multiply-high (ryr_fastdiv): multiplier, shifts */
+    uint32_t dX_m, dX_s1, dX_s2, dXY_m, dXY_s1, dXY_s2;
 };
+
+/* Division of a 32-bit unsigned n by an invariant d >= 1 (Granlund and
+ * Montgomery, PLDI 1994, Fig. 4.1): with l = ceil(log2 d),
+ * m = floor(2^32 (2^l - d) / d) + 1 and t = mulhi(m, n),
+ * n / d = (t + ((n - t) >> 1)) >> (l - 1); d = 1 is m = 0 with both shifts 0.
+ * The constants are computed on the host (ryr_fastdiv_consts). */
+RYR_DEV uint32_t ryr_fastdiv(uint32_t n, uint32_t m, uint32_t s1, uint32_t s2)
+{
+    const uint32_t t = __umulhi(n, m);
+    return (t + ((n - t) >> s1)) >> s2;
+}
 
 /* membrane state of a cell plus the V-only factors hoisted for its next step */
 struct RyrMem {
@@ -79,9 +92,9 @@ struct RyrKArgs {
     long long P;            /* padded slots */
     double dt;
     int scheme, iso2, spo, ncells;
-    void *ci[2], *cnsr[2], *cs[2];
+    void *ci[2], *cnsr[2], *cs[2], *cp[2], *cjsr[2];  /* double-buffered: read buffer = step parity */
     void *catf, *cats, *cacal, *casr, *camyo, *mgmyo;
-    void *cp, *cjsr, *cjsrb, *Ancx, *casar, *casarh, *casarj, *casarhj;
+    void *cjsrb, *Ancx, *casar, *casarh, *casarj, *casarhj;
     void *sb;               /* scheme B: partial cytosol rates of sub-voxels 0,1 (2 planes) */
     uint32_t *lccw, *ryrw, *aux;          /* aux: nl (bits 0-15) | nspark (bit 16) */
     double *d_xire, *d_xinaca;   /* per CRU, written on output steps */
@@ -180,7 +193,7 @@ RYR_DEV float powf_fast(float x, float y)
 RYR_DEV float uptake_f(float ci, float cnsr)
 {
     float a = powf_fast(ci * (1.0f / 0.123f), 1.787f), b = powf_fast(cnsr * (1.0f / 1700.0f), 1.787f);
-    return 0.3f * (a - b) / (1.0f + a + b);
+    return 0.3f * (a - b) * __builtin_amdgcn_rcpf(1.0f + a + b);
 }
 template <typename T> RYR_DEV T leak_t(T cnsr, T ci)
 {
@@ -725,8 +738,10 @@ __global__ __launch_bounds__(WG) void k_cyto(RyrKArgs a, int kc, int km, double 
     const bool diag = (nstep % a.spo) == 0;
     const long long P = a.P, g = C.off + n, e = ii * P + g;
     const int X = C.X, Y = C.Y, Z = C.Z;
-    const int i = n % X, j = (n / X) % Y, k = n / (X * Y);
-    const long long sy = X, sz = (long long)X * Y;
+    /* position in the lattice: i = n mod X, rxy = n mod XY (j = rxy / X) */
+    const int sy = X, sz = X * Y;
+    const int i = n - X * (int)ryr_fastdiv((uint32_t)n, C.dX_m, C.dX_s1, C.dX_s2);
+    const int rxy = n - sz * (int)ryr_fastdiv((uint32_t)n, C.dXY_m, C.dXY_s1, C.dXY_s2);
     const TF *__restrict__ ci0 = (const TF *)a.ci[par];
     TF *__restrict__ ci1 = (TF *)a.ci[par ^ 1];
     const TS *__restrict__ cn0 = (const TS *)a.cnsr[par];
@@ -743,20 +758,20 @@ __global__ __launch_bounds__(WG) void k_cyto(RyrKArgs a, int kc, int km, double 
     TF inN, inS, inE, inW, inT, inB;
     TS nnN, nnS, nnE, nnW, nnT, nnB;
     if (ii & 1) {
-        const bool in = k + 1 < Z;
+        const bool in = n + sz < C.N;      /* k + 1 < Z */
         inN = in ? ci0[(ii - 1) * P + g + sz] : cc; nnN = in ? cn0[(ii - 1) * P + g + sz] : cn;
         inS = ci0[(ii - 1) * P + g]; nnS = cn0[(ii - 1) * P + g];
     } else {
-        const bool in = k > 0;
+        const bool in = n >= sz;           /* k > 0 */
         inN = ci0[(ii + 1) * P + g]; nnN = cn0[(ii + 1) * P + g];
         inS = in ? ci0[(ii + 1) * P + g - sz] : cc; nnS = in ? cn0[(ii + 1) * P + g - sz] : cn;
     }
     if (ii & 2) {
-        const bool in = j + 1 < Y;
+        const bool in = rxy + sy < sz;     /* j + 1 < Y */
         inE = in ? ci0[(ii - 2) * P + g + sy] : cc; nnE = in ? cn0[(ii - 2) * P + g + sy] : cn;
         inW = ci0[(ii - 2) * P + g]; nnW = cn0[(ii - 2) * P + g];
     } else {
-        const bool in = j > 0;
+        const bool in = rxy >= sy;         /* j > 0 */
         inE = ci0[(ii + 2) * P + g]; nnE = cn0[(ii + 2) * P + g];
         inW = in ? ci0[(ii + 2) * P + g - sy] : cc; nnW = in ? cn0[(ii + 2) * P + g - sy] : cn;
     }
@@ -792,11 +807,11 @@ __global__ __launch_bounds__(WG) void k_cyto(RyrKArgs a, int kc, int km, double 
         const double buffers = (f_tf + f_ts + f_cal + f_sr + f_myo);
         double dotci = (-xiup + xileak) - buffers + xicoupi;
         if (ii < 2) {
-            const double cj = ((const TS *)a.cjsr)[g];
+            const double cj = ((const TS *)a.cjsr[par])[g];
             const double diffjn = (cj - cn) / (25.0 * 1.0 / 4.) / 2.;
             dotcnsr += diffjn * (0.02 * 1) / (0.025 / 8.);
             if (schemeA) {
-                const double csv = ((const TF *)a.cs[par])[g], cpv = ((const TF *)a.cp)[g];
+                const double csv = ((const TF *)a.cs[par])[g], cpv = ((const TF *)a.cp[par])[g];
                 dotci += (csv - cc) / (0.04) / 2. * ((0.025) / (0.5 / 8.));
                 dotci += (cpv - cc) / (0.1) / 2. * ((0.00126) / (0.5 / 8.));
             } else {
@@ -829,7 +844,7 @@ __global__ __launch_bounds__(WG) void k_cyto(RyrKArgs a, int kc, int km, double 
          * constants replaced by reciprocal products; cnsr and slow troponin
          * keep double storage */
         const float dt = (float)a.dt;
-        const float sat = 0.95f / dt;
+        const float sat = 0.95f * __builtin_amdgcn_rcpf(dt);
         const float rIT = (float)(1.0 / ((0.33 * 2.) * 0.7)), rIL = (float)(1.0 / ((0.7 * 2.) * 0.7));
         const float rNT = (float)(1.0 / (1.8 * 0.7)), rNL = (float)(1.0 / (6. * 0.7));
         const float cf = (float)cc, cnf = (float)cn;
@@ -852,10 +867,10 @@ __global__ __launch_bounds__(WG) void k_cyto(RyrKArgs a, int kc, int km, double 
         const float f_myo = (on_my * dt > 1.0f ? sat : on_my) * (140.0f - v_camyo - v_mgmyo) - 0.46f * v_camyo;
         float dotci = (xileak - xiup) - (f_tf + f_ts + f_cal + f_sr + f_myo) + xicoupi;
         if (ii < 2) {
-            const TS cj = ((const TS *)a.cjsr)[g];
+            const TS cj = ((const TS *)a.cjsr[par])[g];
             dotcnsr += (float)((cj - cn) * TS(1.0 / (25.0 * 1.0 / 4.) / 2.)) * (float)((0.02 * 1) / (0.025 / 8.));
             if (schemeA) {
-                const float csv = (float)((const TF *)a.cs[par])[g], cpv = (float)((const TF *)a.cp)[g];
+                const float csv = (float)((const TF *)a.cs[par])[g], cpv = (float)((const TF *)a.cp[par])[g];
                 dotci += (csv - cf) * (float)(1.0 / 0.04 / 2. * ((0.025) / (0.5 / 8.))) +
                          (cpv - cf) * (float)(1.0 / 0.1 / 2. * ((0.00126) / (0.5 / 8.)));
             } else {
@@ -964,8 +979,7 @@ __global__ __launch_bounds__(WG) void k_cru_step(RyrKArgs a, int kc)
     if (active) {
         const long long g = C.off + n;
         const int X = C.X, Y = C.Y, Z = C.Z;
-        const int i = n % X, j = (n / X) % Y, k = n / (X * Y);
-        const long long sy = X, sz = (long long)X * Y;
+        const int sz = X * Y;
         const TF *__restrict__ ci0 = (const TF *)a.ci[par];
         TF *__restrict__ ci1 = (TF *)a.ci[par ^ 1];
         const TS *__restrict__ cn0 = (const TS *)a.cnsr[par];
@@ -975,9 +989,11 @@ __global__ __launch_bounds__(WG) void k_cru_step(RyrKArgs a, int kc)
         TF *catf = (TF *)a.catf, *cacal = (TF *)a.cacal, *casr = (TF *)a.casr;
         TF *camyo = (TF *)a.camyo, *mgmyo = (TF *)a.mgmyo;
         TS *cats = (TS *)a.cats;
-        TF *cpp = (TF *)a.cp, *ancx = (TF *)a.Ancx;
+        const TF *cp0 = (const TF *)a.cp[par];
+        TF *cp1 = (TF *)a.cp[par ^ 1], *ancx = (TF *)a.Ancx;
         TF *casar = (TF *)a.casar, *casarh = (TF *)a.casarh, *casarj = (TF *)a.casarj, *casarhj = (TF *)a.casarhj;
-        TS *cjsrp = (TS *)a.cjsr, *cjsrbp = (TS *)a.cjsrb;
+        const TS *cjsr0 = (const TS *)a.cjsr[par];
+        TS *cjsr1 = (TS *)a.cjsr[par ^ 1], *cjsrbp = (TS *)a.cjsrb;
 
         const double v = M->m[0];
         TF cil[2];
@@ -985,8 +1001,8 @@ __global__ __launch_bounds__(WG) void k_cru_step(RyrKArgs a, int kc)
 #pragma unroll
         for (int ii = 0; ii < 2; ++ii) { cil[ii] = ci0[ii * P + g]; cnl[ii] = cn0[ii * P + g]; }
         const TF csv = cs0[g];
-        TF cpv = cpp[g];
-        const TS cjsrv = cjsrp[g];
+        TF cpv = cp0[g];
+        const TS cjsrv = cjsr0[g];
         uint32_t lw = a.lccw[g], rw = a.ryrw[g], ax = a.aux[g];
         if (reset) ax = 0;
         int nl = (int)(ax & 0xffffu), nspark = (int)((ax >> 16) & 1u);
@@ -1094,8 +1110,8 @@ __global__ __launch_bounds__(WG) void k_cru_step(RyrKArgs a, int kc)
         /* cleft/sub-membrane sub-steps (:1356-1473 / :1475-1597); the
          * cytosol/NSR sub-voxels were advanced by k_cyto */
         TF v_casar = casar[g], v_casarh = casarh[g], v_casarj = casarj[g], v_casarhj = casarhj[g];
-        const TF cs_up = (k + 1 < Z) ? cs0[g + sz] : csv;
-        const TF cs_dn = (k > 0) ? cs0[g - sz] : csv;
+        const TF cs_up = (n + sz < C.N) ? cs0[g + sz] : csv;   /* k + 1 < Z */
+        const TF cs_dn = (n >= sz) ? cs0[g - sz] : csv;        /* k > 0 */
         const TF icatT = (TF)icat1, xinacaT = (TF)xinaca, xireT = (TF)xire;
         TF csn = csv;
         const TF *sbp = (const TF *)a.sb;
@@ -1199,9 +1215,9 @@ __global__ __launch_bounds__(WG) void k_cru_step(RyrKArgs a, int kc)
         if (cpv < TF(0)) { cpv = TF(0); ++nclampcp; }
         if (csn < TF(0)) { csn = TF(1e-6); ++nclampcs; }
         casar[g] = v_casar; casarh[g] = v_casarh; casarj[g] = v_casarj; casarhj[g] = v_casarhj;
-        cpp[g] = cpv;
+        cp1[g] = cpv;
         cs1[g] = csn;
-        cjsrp[g] = cjsrv + dotcjsr * (TS)dtT;
+        cjsr1[g] = cjsrv + dotcjsr * (TS)dtT;
         cjsrbp[g] = cjsrb_new;
         a.lccw[g] = lw;
         a.ryrw[g] = (uint32_t)nou | ((uint32_t)ncu << 8) | ((uint32_t)nob << 16) | ((uint32_t)ncb << 24);
@@ -1355,10 +1371,10 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_diag(RyrKArgs a)
     const TF *catf = (const TF *)a.catf, *cacal = (const TF *)a.cacal, *casr = (const TF *)a.casr;
     const TF *camyo = (const TF *)a.camyo;
     const TS *cats = (const TS *)a.cats;
-    const TF *cp = (const TF *)a.cp, *ancx = (const TF *)a.Ancx;
+    const TF *cp = (const TF *)a.cp[par], *ancx = (const TF *)a.Ancx;
     const TF *casar = (const TF *)a.casar, *casarh = (const TF *)a.casarh;
     const TF *casarj = (const TF *)a.casarj, *casarhj = (const TF *)a.casarhj;
-    const TS *cjsr = (const TS *)a.cjsr, *cjsrb = (const TS *)a.cjsrb;
+    const TS *cjsr = (const TS *)a.cjsr[par], *cjsrb = (const TS *)a.cjsrb;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (threadIdx.x < RYR_DIAG_I) s_cnt[threadIdx.x] = 0;
     __syncthreads();
@@ -1558,8 +1574,10 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_init(RyrKArgs a, const RyrInit *_
         ((TF *)a.camyo)[e] = (TF)I.camyo0;
         ((TF *)a.mgmyo)[e] = (TF)I.mgmyo0;
     }
-    ((TF *)a.cp)[g] = (TF)I.ci0;
-    ((TS *)a.cjsr)[g] = (TS)I.cjsr0;
+    for (int buf = 0; buf < 2; ++buf) {
+        ((TF *)a.cp[buf])[g] = (TF)I.ci0;
+        ((TS *)a.cjsr[buf])[g] = (TS)I.cjsr0;
+    }
     ((TS *)a.cjsrb)[g] = (TS)I.cjsrb0;
     ((TF *)a.Ancx)[g] = (TF)I.ancx0;
     ((TF *)a.casar)[g] = (TF)I.casar0;
@@ -1603,8 +1621,8 @@ RYR_DEV double field_get(const RyrKArgs &a, int f, int cur, long long e, long lo
     case 6: return (double)((const TF *)a.camyo)[e];
     case 7: return (double)((const TF *)a.mgmyo)[e];
     case 8: return (double)((const TF *)a.cs[cur])[g];
-    case 9: return (double)((const TF *)a.cp)[g];
-    case 10: return (double)((const TS *)a.cjsr)[g];
+    case 9: return (double)((const TF *)a.cp[cur])[g];
+    case 10: return (double)((const TS *)a.cjsr[cur])[g];
     case 11: return (double)((const TS *)a.cjsrb)[g];
     case 12: return (double)((const TF *)a.Ancx)[g];
     case 13: return (double)((const TF *)a.casar)[g];
@@ -1635,8 +1653,8 @@ RYR_DEV void field_set(const RyrKArgs &a, int f, int cur, long long e, long long
     case 6: ((TF *)a.camyo)[e] = (TF)v; break;
     case 7: ((TF *)a.mgmyo)[e] = (TF)v; break;
     case 8: ((TF *)a.cs[cur])[g] = (TF)v; break;
-    case 9: ((TF *)a.cp)[g] = (TF)v; break;
-    case 10: ((TS *)a.cjsr)[g] = (TS)v; break;
+    case 9: ((TF *)a.cp[cur])[g] = (TF)v; break;
+    case 10: ((TS *)a.cjsr[cur])[g] = (TS)v; break;
     case 11: ((TS *)a.cjsrb)[g] = (TS)v; break;
     case 12: ((TF *)a.Ancx)[g] = (TF)v; break;
     case 13: ((TF *)a.casar)[g] = (TF)v; break;

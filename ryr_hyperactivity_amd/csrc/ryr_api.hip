@@ -10,6 +10,7 @@
  */
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -40,9 +41,14 @@ struct ryr_handle {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t kev[8] = {};         /* per-kernel start/end stamps: cyto, cru, membrane, diag */
     int wg_cyto = 0, wg_cru = 0;    /* workgroup sizes of the two CRU-parallel kernels; 0: tune */
-    int fuse = -1;                  /* membrane step carried by the next sub-voxel launch; -1: tune */
+    int sched = -1;                 /* step schedule RYR_SCHED_*; -1: tune */
+    hipStream_t stream2 = nullptr;  /* second stream of the concurrent schedule */
+    hipEvent_t fork = nullptr, join = nullptr;
     int tune_i = 0;
     std::vector<float> tune_ms[2][3];
+    std::vector<std::array<int, 3>> cand;   /* schedule tuning: candidates and their times */
+    std::vector<float> cand_ms;
+    size_t cand_i = 0;
     bool timing_open = false;
     long long launches = 0;
     std::string err;
@@ -132,6 +138,17 @@ static int dev_alloc(ryr_handle *h, void **ptr, size_t bytes, bool state)
     return RYR_OK;
 }
 
+/* constants of ryr_fastdiv for the divisor d >= 1 */
+static void ryr_fastdiv_consts(uint32_t d, uint32_t *m, uint32_t *s1, uint32_t *s2)
+{
+    if (d <= 1) { *m = 0; *s1 = 0; *s2 = 0; return; }
+    uint32_t l = 0;
+    while ((1ull << l) < d) ++l;
+    *m = (uint32_t)(((1ull << 32) * ((1ull << l) - d)) / d + 1);
+    *s1 = 1;
+    *s2 = l - 1;
+}
+
 /* the per-cell table and its per-chunk copies */
 static int upload_info(ryr_handle *h)
 {
@@ -194,6 +211,12 @@ extern "C" int ryr_create(int device, const ryr_params *p, const ryr_cell_desc *
     (void)hipEventCreate(&h->ev0);
     (void)hipEventCreate(&h->ev1);
     for (auto &e : h->kev) (void)hipEventCreate(&e);
+    if (hipStreamCreateWithFlags(&h->stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&h->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&h->join, hipEventDisableTiming) != hipSuccess) {
+        h->err = "hipStreamCreate/hipEventCreate failed";
+        return fail(RYR_E_HIP);
+    }
 
     /* layout: each cell padded to whole workgroups */
     std::vector<int> block_cell;
@@ -231,6 +254,8 @@ extern "C" int ryr_create(int device, const ryr_params *p, const ryr_cell_desc *
         I.line_j = jy - 1;
         I.line_k = jz - 1;
         I.cell = c;
+        ryr_fastdiv_consts((uint32_t)d.nx, &I.dX_m, &I.dX_s1, &I.dX_s2);
+        ryr_fastdiv_consts((uint32_t)(d.nx * d.ny), &I.dXY_m, &I.dXY_s1, &I.dXY_s2);
         h->line_off[c] = lo;
         lo += (long long)d.nx * RYR_LINE_COLS;
     }
@@ -254,10 +279,12 @@ extern "C" int ryr_create(int device, const ryr_params *p, const ryr_cell_desc *
         ALLOC(a.ci[b], 8 * P * tf);
         ALLOC(a.cnsr[b], 8 * P * ts);
         ALLOC(a.cs[b], P * tf);
+        ALLOC(a.cp[b], P * tf);
+        ALLOC(a.cjsr[b], P * ts);
     }
     ALLOC(a.catf, 8 * P * tf); ALLOC(a.cats, 8 * P * ts); ALLOC(a.cacal, 8 * P * tf);
     ALLOC(a.casr, 8 * P * tf); ALLOC(a.camyo, 8 * P * tf); ALLOC(a.mgmyo, 8 * P * tf);
-    ALLOC(a.cp, P * tf); ALLOC(a.cjsr, P * ts); ALLOC(a.cjsrb, P * ts); ALLOC(a.Ancx, P * tf);
+    ALLOC(a.cjsrb, P * ts); ALLOC(a.Ancx, P * tf);
     ALLOC(a.casar, P * tf); ALLOC(a.casarh, P * tf); ALLOC(a.casarj, P * tf); ALLOC(a.casarhj, P * tf);
     ALLOC(a.lccw, P * 4); ALLOC(a.ryrw, P * 4); ALLOC(a.aux, P * 4);
     ALLOC(a.d_xire, P * 8); ALLOC(a.d_xinaca, P * 8); ALLOCT(a.d_up, 8 * P * 8); ALLOCT(a.d_leak, 8 * P * 8);
@@ -384,10 +411,11 @@ static hipError_t go_cyto(ryr_handle *h, int wg, int kc, int km, hipEvent_t *ev)
     return RYR_F32(h) ? ryr_launch_cyto_f32(h->a, h->nblocks, wg, kc, km, h->p.stim_amp, h->stream, ev)
                       : ryr_launch_cyto_f64(h->a, h->nblocks, wg, kc, km, h->p.stim_amp, h->stream, ev);
 }
-static hipError_t go_cru(ryr_handle *h, int wg, int kc, hipEvent_t *ev)
+static hipError_t go_cru(ryr_handle *h, int wg, int kc, hipEvent_t *ev, hipStream_t st = nullptr)
 {
-    return RYR_F32(h) ? ryr_launch_cru_f32(h->a, h->nblocks, wg, kc, h->stream, ev)
-                      : ryr_launch_cru_f64(h->a, h->nblocks, wg, kc, h->stream, ev);
+    if (!st) st = h->stream;
+    return RYR_F32(h) ? ryr_launch_cru_f32(h->a, h->nblocks, wg, kc, st, ev)
+                      : ryr_launch_cru_f64(h->a, h->nblocks, wg, kc, st, ev);
 }
 static hipError_t go_mem(ryr_handle *h, int km, hipEvent_t *ev)
 {
@@ -400,13 +428,19 @@ static hipError_t go_diag(ryr_handle *h, hipEvent_t *ev)
                       : ryr_launch_diag_f64(h->a, h->nblocks, h->stream, ev);
 }
 
-/* Steps h->step .. h->step+n-1 (h->step itself is not advanced here).
- * Step m: the sub-voxel kernel, carrying the membrane step of m-1 when that
- * one is still pending; the CRU kernel; and the membrane step of m on its own
- * when m is an output step (then the output-row kernel follows), the last
- * step of the run (so every run ends with the whole state advanced) or when
- * fusion is off. */
-static hipError_t enqueue_run(ryr_handle *h, long long n, bool fuse)
+/* Steps h->step .. h->step+n-1 (h->step itself is not advanced here), in
+ * one of three schedules (every schedule computes identical values):
+ *  RYR_SCHED_SEQ: sub-voxel kernel, CRU kernel, membrane step;
+ *  RYR_SCHED_FUSED: the sub-voxel kernel of step m carries the membrane step
+ *    of m-1; the membrane step runs on its own only at an output step (the
+ *    output-row kernel follows it) and at the last step of the run, so every
+ *    run ends with the whole state advanced;
+ *  RYR_SCHED_CONCURRENT (cleft scheme A): the sub-voxel and CRU kernels of a
+ *    step read only step-start values of each other's fields (cp, cs, cjsr
+ *    and ci/cnsr are double-buffered) and write disjoint fields, so the CRU
+ *    kernel runs on the second stream beside the memory-bound sub-voxel
+ *    sweep; the membrane step waits for both. */
+static hipError_t enqueue_run(ryr_handle *h, long long n, int sched)
 {
     const long long spo = h->p.steps_per_output;
     bool pending = false;
@@ -414,9 +448,21 @@ static hipError_t enqueue_run(ryr_handle *h, long long n, bool fuse)
     for (long long i = 0; i < n && e == hipSuccess; ++i) {
         const long long m = h->step + i;
         const int k = (int)(m - h->gbase);
+        const bool out = m % spo == 0;
+        if (sched == RYR_SCHED_CONCURRENT) {
+            e = hipEventRecord(h->fork, h->stream);
+            if (e == hipSuccess) e = hipStreamWaitEvent(h->stream2, h->fork, 0);
+            if (e == hipSuccess) e = go_cru(h, h->wg_cru, k, nullptr, h->stream2);
+            if (e == hipSuccess) e = hipEventRecord(h->join, h->stream2);
+            if (e == hipSuccess) e = go_cyto(h, h->wg_cyto, k, -1, nullptr);
+            if (e == hipSuccess) e = hipStreamWaitEvent(h->stream, h->join, 0);
+            if (e == hipSuccess) e = go_mem(h, k, nullptr);
+            if (e == hipSuccess && out) e = go_diag(h, nullptr);
+            continue;
+        }
+        const bool fuse = sched == RYR_SCHED_FUSED;
         e = go_cyto(h, h->wg_cyto, k, pending ? k - 1 : -1, nullptr);
         if (e == hipSuccess) e = go_cru(h, h->wg_cru, k, nullptr);
-        const bool out = m % spo == 0;
         if (e == hipSuccess && (out || i == n - 1 || !fuse)) {
             e = go_mem(h, k, nullptr);
             if (e == hipSuccess && out) e = go_diag(h, nullptr);
@@ -490,29 +536,76 @@ static int tune_step(ryr_handle *h)
     return RYR_OK;
 }
 
-/* Fusion tuning, after the shapes: pairs of steps with and without the
- * membrane step carried by the sub-voxel launch, alternating, timed as a
- * whole by events around them; fusion is kept if its median is lower by more
- * than 3%.  Advances 4 * RYR_TUNE_ROUNDS steps. */
-static int tune_fusion(ryr_handle *h)
+/* Schedule tuning, after the shapes.  Candidates: the tuned sizes with each
+ * schedule, plus (scheme A) the concurrent schedule with smaller sizes (two
+ * kernels sharing the chip favour smaller workgroups).  Each candidate is
+ * timed the way production runs, as a captured graph: RYR_TUNE_GSTEPS steps
+ * that hold no output step, replayed three times (the first untimed), events
+ * around each replay.  The sequential candidate is kept unless another is
+ * faster by more than 3%.  Each candidate advances 3 * RYR_TUNE_GSTEPS
+ * steps; candidates run while a call has that many steps left. */
+#define RYR_TUNE_GSTEPS 4
+static int nsched(const ryr_handle *h) { return h->p.scheme == RYR_SCHEME_A ? 3 : 2; }
+
+static void sched_candidates(const ryr_handle *h, std::vector<std::array<int, 3>> &c)
 {
-    std::vector<float> t[2];
-    for (int r = 0; r < RYR_TUNE_ROUNDS; ++r)
-        for (int f = 0; f < 2; ++f) {
-            RYR_HIPCHECK(h, ensure_base(h, 2));
-            RYR_HIPCHECK(h, hipEventRecord(h->kev[0], h->stream));
-            RYR_HIPCHECK(h, enqueue_run(h, 2, f == 1));
-            RYR_HIPCHECK(h, hipEventRecord(h->kev[1], h->stream));
-            RYR_HIPCHECK(h, hipEventSynchronize(h->kev[1]));
-            float ms = 0.0f;
-            RYR_HIPCHECK(h, hipEventElapsedTime(&ms, h->kev[0], h->kev[1]));
-            t[f].push_back(ms);
-            h->step += 2;
-            h->launches += 2;
-        }
-    for (auto &v : t) std::sort(v.begin(), v.end());
-    h->fuse = t[1][t[1].size() / 2] < 0.97f * t[0][t[0].size() / 2] ? 1 : 0;
-    drop_graph(h);
+    c.clear();
+    auto add = [&](int a, int b, int s) {
+        for (auto &x : c) if (x[0] == a && x[1] == b && x[2] == s) return;
+        c.push_back({a, b, s});
+    };
+    add(h->wg_cyto, h->wg_cru, RYR_SCHED_SEQ);
+    add(h->wg_cyto, h->wg_cru, RYR_SCHED_FUSED);
+    if (nsched(h) > 2) {
+        add(h->wg_cyto, h->wg_cru, RYR_SCHED_CONCURRENT);
+        add(128, 64, RYR_SCHED_CONCURRENT);
+        add(256, 64, RYR_SCHED_CONCURRENT);
+        add(128, 128, RYR_SCHED_CONCURRENT);
+        add(64, 64, RYR_SCHED_CONCURRENT);
+    }
+}
+
+/* one candidate: advances 3 * RYR_TUNE_GSTEPS steps (none of them an output
+ * step) and records the best timed replay in ms; false if it cannot be
+ * timed as a graph (then it is skipped) */
+static int tune_candidate(ryr_handle *h, const std::array<int, 3> &c, float *best_ms, bool *ok)
+{
+    const int sw_c = h->wg_cyto, sw_r = h->wg_cru;
+    h->wg_cyto = c[0];
+    h->wg_cru = c[1];
+    *ok = false;
+    *best_ms = 1e30f;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    RYR_HIPCHECK(h, set_base(h, h->step));
+    bool cap = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (cap) {
+        bool run = enqueue_run(h, RYR_TUNE_GSTEPS, c[2]) == hipSuccess;
+        cap = hipStreamEndCapture(h->stream, &g) == hipSuccess && g && run;
+    }
+    if (cap) cap = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) == hipSuccess;
+    if (g) hipGraphDestroy(g);
+    h->wg_cyto = sw_c;
+    h->wg_cru = sw_r;
+    if (!cap) {
+        (void)hipGetLastError();
+        if (ge) hipGraphExecDestroy(ge);
+        return RYR_OK;
+    }
+    for (int r = 0; r < 3; ++r) {
+        RYR_HIPCHECK(h, set_base(h, h->step));
+        RYR_HIPCHECK(h, hipEventRecord(h->kev[0], h->stream));
+        RYR_HIPCHECK(h, hipGraphLaunch(ge, h->stream));
+        RYR_HIPCHECK(h, hipEventRecord(h->kev[1], h->stream));
+        RYR_HIPCHECK(h, hipEventSynchronize(h->kev[1]));
+        float ms = 0.0f;
+        RYR_HIPCHECK(h, hipEventElapsedTime(&ms, h->kev[0], h->kev[1]));
+        if (r > 0 && ms < *best_ms) *best_ms = ms;
+        h->step += RYR_TUNE_GSTEPS;
+        h->launches += RYR_TUNE_GSTEPS;
+    }
+    hipGraphExecDestroy(ge);
+    *ok = true;
     return RYR_OK;
 }
 
@@ -526,7 +619,7 @@ static bool build_graph(ryr_handle *h)
     const long long spo = h->p.steps_per_output;
     bool ok = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
     if (ok) {
-        ok = enqueue_run(h, spo, h->fuse == 1) == hipSuccess;
+        ok = enqueue_run(h, spo, h->sched < 0 ? RYR_SCHED_SEQ : h->sched) == hipSuccess;
         if (ok) {
             hipLaunchKernelGGL(k_stepbase, dim3(1), dim3(64), 0, h->stream, h->a.gstep, spo, 1);
             ok = hipGetLastError() == hipSuccess;
@@ -562,10 +655,42 @@ extern "C" int ryr_step(ryr_handle *h, int64_t nsteps)
         h->launches += 1;
         nsteps -= 1;
     }
-    if (h->fuse < 0 && nsteps >= 4 * RYR_TUNE_ROUNDS) {
-        int rc = tune_fusion(h);
+    while (h->sched < 0 && nsteps > 0) {
+        if (h->cand.empty()) {
+            sched_candidates(h, h->cand);
+            h->cand_ms.assign(h->cand.size(), 1e30f);
+            h->cand_i = 0;
+        }
+        const long long span = 3 * RYR_TUNE_GSTEPS;
+        const long long spo_left = spo - h->step % spo;     /* steps before the next output step */
+        if (h->step % spo == 0 || spo_left <= span) {
+            /* an output step is at hand: step past it eagerly */
+            long long n = h->step % spo == 0 ? 1 : spo_left + 1;
+            if (n > nsteps) n = nsteps;
+            RYR_HIPCHECK(h, ensure_base(h, n));
+            RYR_HIPCHECK(h, enqueue_run(h, n, RYR_SCHED_SEQ));
+            h->step += n;
+            h->launches += n;
+            nsteps -= n;
+            continue;
+        }
+        if (nsteps < span) break;
+        float ms = 0.0f;
+        bool ok = false;
+        int rc = tune_candidate(h, h->cand[h->cand_i], &ms, &ok);
         if (rc) return rc;
-        nsteps -= 4 * RYR_TUNE_ROUNDS;
+        if (ok) h->cand_ms[h->cand_i] = ms;
+        nsteps -= span;
+        if (++h->cand_i == h->cand.size()) {
+            size_t best = 0;
+            for (size_t q = 1; q < h->cand.size(); ++q)
+                if (h->cand_ms[q] < 0.97f * h->cand_ms[best]) best = q;
+            h->wg_cyto = h->cand[best][0];
+            h->wg_cru = h->cand[best][1];
+            h->sched = h->cand[best][2];
+            h->cand.clear();
+            drop_graph(h);
+        }
     }
     while (nsteps > 0) {
         if (h->step % spo == 0 && nsteps >= spo && spo >= 4) {
@@ -583,7 +708,7 @@ extern "C" int ryr_step(ryr_handle *h, int64_t nsteps)
         long long n = spo - h->step % spo;
         if (n > nsteps) n = nsteps;
         RYR_HIPCHECK(h, ensure_base(h, n));
-        RYR_HIPCHECK(h, enqueue_run(h, n, h->fuse == 1));
+        RYR_HIPCHECK(h, enqueue_run(h, n, h->sched < 0 ? RYR_SCHED_SEQ : h->sched));
         h->step += n;
         h->launches += n;
         nsteps -= n;
@@ -616,33 +741,35 @@ extern "C" int ryr_kernel_time(ryr_handle *h, double *ms, int64_t *launches)
     return RYR_OK;
 }
 
-extern "C" int ryr_set_launch_shape(ryr_handle *h, int wg_cyto, int wg_cru, int fuse)
+extern "C" int ryr_set_launch_shape(ryr_handle *h, int wg_cyto, int wg_cru, int sched)
 {
     if (!h) return RYR_E_ARG;
     auto ok = [](int w) { return w == 64 || w == 128 || w == 256; };
     if (wg_cyto == 0 && wg_cru == 0) {
         h->wg_cyto = h->wg_cru = 0;
-        h->fuse = -1;
+        h->sched = -1;
         h->tune_i = 0;
+        h->cand.clear();
         for (auto &k : h->tune_ms) for (auto &q : k) q.clear();
-    } else if (ok(wg_cyto) && ok(wg_cru) && (fuse == 0 || fuse == 1)) {
+    } else if (ok(wg_cyto) && ok(wg_cru) && sched >= 0 && sched < nsched(h)) {
         h->wg_cyto = wg_cyto;
         h->wg_cru = wg_cru;
-        h->fuse = fuse;
+        h->sched = sched;
     } else {
-        h->err = "workgroup sizes must be 64, 128 or 256 and fuse 0 or 1 (or sizes 0, 0 to tune)";
+        h->err = "workgroup sizes must be 64, 128 or 256 and the schedule one of RYR_SCHED_* "
+                 "(concurrent needs cleft scheme A), or sizes 0, 0 to tune";
         return RYR_E_ARG;
     }
     drop_graph(h);
     return RYR_OK;
 }
 
-extern "C" int ryr_get_launch_shape(const ryr_handle *h, int *wg_cyto, int *wg_cru, int *fuse)
+extern "C" int ryr_get_launch_shape(const ryr_handle *h, int *wg_cyto, int *wg_cru, int *sched)
 {
-    if (!h || !wg_cyto || !wg_cru || !fuse) return RYR_E_ARG;
+    if (!h || !wg_cyto || !wg_cru || !sched) return RYR_E_ARG;
     *wg_cyto = h->wg_cyto;
     *wg_cru = h->wg_cru;
-    *fuse = h->fuse;
+    *sched = h->sched;
     return RYR_OK;
 }
 
@@ -925,6 +1052,9 @@ extern "C" void ryr_destroy(ryr_handle *h)
     if (h->stage) hipFree(h->stage);
     for (double *q : h->vclamp) if (q) hipFree(q);
     for (auto &e : h->kev) if (e) hipEventDestroy(e);
+    if (h->fork) hipEventDestroy(h->fork);
+    if (h->join) hipEventDestroy(h->join);
+    if (h->stream2) { hipStreamSynchronize(h->stream2); hipStreamDestroy(h->stream2); }
     if (h->ev0) hipEventDestroy(h->ev0);
     if (h->ev1) hipEventDestroy(h->ev1);
     if (h->stream) hipStreamDestroy(h->stream);

@@ -78,14 +78,15 @@ class CellEngine:
         self._check(self.lib.ryr_kernel_time(self.h, ctypes.byref(ms), ctypes.byref(n)), "ryr_kernel_time")
         return ms.value, n.value
 
-    def set_launch_shape(self, wg_cyto: int, wg_cru: int, fuse: int = 1):
-        """Workgroup sizes (64/128/256) of the sub-voxel and CRU kernels and whether
-        the membrane step rides in the next sub-voxel launch; sizes (0, 0) re-tune."""
-        self._check(self.lib.ryr_set_launch_shape(self.h, int(wg_cyto), int(wg_cru), int(fuse)),
+    def set_launch_shape(self, wg_cyto: int, wg_cru: int, sched: int = 0):
+        """Workgroup sizes (64/128/256) of the sub-voxel and CRU kernels and the step
+        schedule (0 sequential, 1 membrane fused into the next sub-voxel launch,
+        2 sub-voxel and CRU kernels concurrent); sizes (0, 0) re-tune."""
+        self._check(self.lib.ryr_set_launch_shape(self.h, int(wg_cyto), int(wg_cru), int(sched)),
                     "ryr_set_launch_shape")
 
     def launch_shape(self):
-        """(wg_cyto, wg_cru, fuse); (0, 0, -1) while tuning is pending."""
+        """(wg_cyto, wg_cru, sched); (0, 0, -1) while tuning is pending."""
         a, b, f = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         self._check(self.lib.ryr_get_launch_shape(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(f)),
                     "ryr_get_launch_shape")

@@ -37,7 +37,7 @@ def _assert_same_state(eng, cell, o, label):
 
 
 @pytest.mark.parametrize("scheme", [SCHEME_A, SCHEME_B])
-@pytest.mark.parametrize("dims", [(7, 5, 3), (20, 13, 5)])
+@pytest.mark.parametrize("dims", [(7, 5, 3), (20, 13, 5), (1, 3, 4)])
 def test_fp64_bitexact_through_an_action_potential(scheme, dims):
     p = Params.default(scheme=scheme, stim_offset=2.0)
     d = CellDesc.default(nx=dims[0], ny=dims[1], nz=dims[2], cl=1000.0, seed=11)
@@ -194,12 +194,14 @@ def test_beats_and_clamp_match_the_oracle():
         assert gb.shape[0] == 3 and list(gb[:, 9]) == [1.0, 1.0, 0.0]
 
 
-@pytest.mark.parametrize("shape", [(64, 64, 1), (128, 256, 0), (256, 128, 1), (256, 256, 0)])
+@pytest.mark.parametrize("shape", [(64, 64, 1), (128, 256, 0), (256, 128, 1), (256, 256, 0), (64, 128, 2),
+                                   (256, 64, 2)])
 def test_every_launch_shape_is_bitexact(shape):
-    """Workgroup sizes and membrane fusion change no value: FP64 against the
+    """Workgroup sizes and step schedules change no value: FP64 against the
     oracle, and FP32 against the FP32 run at another shape (whole-cell sums
     are formed per wave, so the reduction order is the same for every size;
-    the fused membrane step reads only the CRU kernel's sums)."""
+    the fused membrane step reads only the CRU kernel's sums; the concurrent
+    kernels read only step-start values of each other's fields)."""
     print(f"case shape={shape}", flush=True)
     p = Params.default(stim_offset=2.0)
     d = CellDesc.default(nx=20, ny=13, nz=5, cl=1000.0, seed=5)
@@ -212,7 +214,7 @@ def test_every_launch_shape_is_bitexact(shape):
         _assert_same_state(eng, 0, o, f"shape {shape}")
     p32 = Params.default(stim_offset=2.0, precision=PREC_F32)
     with CellEngine(p32, d) as a, CellEngine(p32, d) as b:
-        a.set_launch_shape(256, 256, 1 - shape[2])
+        a.set_launch_shape(256, 256, (shape[2] + 1) % 3)
         b.set_launch_shape(*shape)
         a.step(250)
         b.step(250)
@@ -228,14 +230,14 @@ def test_tuning_and_kernel_profile():
         assert eng.launch_shape() == (0, 0, -1)    # tuning pending
         eng.step(5)
         assert eng.launch_shape() == (0, 0, -1)
-        eng.step(30)                               # sizes after 9 steps, fusion after 12 more
-        wc, wr, fu = eng.launch_shape()
-        assert wc in (64, 128, 256) and wr in (64, 128, 256) and fu in (0, 1)
+        eng.step(200)                              # sizes after 9 steps, then the schedule candidates
+        wc, wr, sc = eng.launch_shape()
+        assert wc in (64, 128, 256) and wr in (64, 128, 256) and sc in (0, 1, 2)
         prof = eng.profile_kernels(120)
-        assert eng.cru_steps() == 155 * d.ncru
+        assert eng.cru_steps() == 325 * d.ncru
         for k in ("k_cyto", "k_cru_step", "k_membrane"):
             us, n = prof[k]
             assert n == 120 and 0.5 < us < 5e4, (k, us, n)
-        assert prof["k_diag"][1] == 1              # steps 35..154 hold one output step (100)
+        assert prof["k_diag"][1] == 1              # steps 205..324 hold one output step (300)
         eng.set_launch_shape(0, 0, 0)
         assert eng.launch_shape() == (0, 0, -1)
