@@ -10,8 +10,8 @@ Workloads (synthetic, model ICs; no datasets involved):
   cell  : one 100x20x10-CRU cell (20,000 CRUs), RyR multiplier Ku2 = 2.5,
           I_Kr scaled to 0, CL 2000 ms, FP32 production precision, seed 1
   batch : 256 independent cells, lattices drawn from {50x10x10, 75x15x10,
-          100x20x10}, Ku2 ~ U[1,3], I_Kr scale ~ U[0,1], CL in {1000,2000,3000},
-          seed = cell index, same precision
+          100x20x10}, Ku2 ~ U[1,3], I_Kr scale ~ U[0,1] (lqt off, so the scale
+          applies), CL in {1000,2000,3000}, seed = cell index, same precision
 The JSON line reports the `cell` workload as `value` and the batch beside it.
 """
 import argparse
@@ -76,7 +76,7 @@ def make_batch(prec, ncells=256, seed=2024):
         cells.append(CellDesc.default(nx=nx, ny=ny, nz=nz, Ku2=float(rng.uniform(1, 3)),
                                       ikr_scale=float(rng.uniform(0, 1)),
                                       cl=float([1000.0, 2000.0, 3000.0][rng.integers(0, 3)]), seed=c))
-    return Params.default(precision=prec), cells
+    return Params.default(precision=prec, lqt=0), cells    # lqt off: the I_Kr scale applies
 
 
 def run_gpu(p, cells, steps, warmup, prof_steps, shape=None):

@@ -140,6 +140,18 @@ def load_library(path: str | None = None):
         "ryr_last_error": (ctypes.c_char_p, [vp]),
         "ryr_read_beats": (c_int, [vp, c_int, P(c_double), c_int, P(c_int)]),
         "ryr_set_vclamp": (c_int, [vp, c_int, P(c_double), c_i64, c_int]),
+        "ryr_mlp_create": (c_int, [c_int, P(vp)]),
+        "ryr_mlp_destroy": (None, [vp]),
+        "ryr_mlp_set_params": (c_int, [vp, P(ctypes.c_float), c_i64]),
+        "ryr_mlp_get_params": (c_int, [vp, P(ctypes.c_float), c_i64]),
+        "ryr_mlp_forward": (c_int, [vp, P(ctypes.c_float), c_i64, P(ctypes.c_float)]),
+        "ryr_mlp_grad": (c_int, [vp, P(ctypes.c_float), P(ctypes.c_float), P(ctypes.c_float),
+                                 P(ctypes.c_float), P(ctypes.c_float), c_i64, ctypes.c_float,
+                                 P(ctypes.c_float), P(ctypes.c_float)]),
+        "ryr_mlp_train": (c_int, [vp, P(ctypes.c_float), P(ctypes.c_float), P(ctypes.c_float),
+                                  P(ctypes.c_float), c_i64, ctypes.c_float, c_i64, c_i64, ctypes.c_float,
+                                  ctypes.c_float, ctypes.c_float, ctypes.c_float, c_u64, P(ctypes.c_float)]),
+        "ryr_mlp_last_error": (ctypes.c_char_p, [vp]),
         "ryr_gating_probe": (c_int, [c_int, c_i64, c_u64, P(ctypes.c_uint32), c_u64, c_double, c_double,
                                      c_double, c_double, c_int, P(ctypes.c_int32), P(ctypes.c_int32),
                                      P(c_double), P(ctypes.c_int32), P(ctypes.c_int32),
@@ -163,3 +175,6 @@ EXPORTED = ["ryr_abi_version", "ryr_params_default", "ryr_cell_desc_default", "r
             "ryr_kernel_time", "ryr_save", "ryr_load", "ryr_destroy", "ryr_last_error",
             "ryr_read_beats", "ryr_set_vclamp", "ryr_gating_probe", "ryr_set_launch_shape",
             "ryr_get_launch_shape", "ryr_profile_kernels"]
+# include/ryr_mlp.h
+EXPORTED_MLP = ["ryr_mlp_create", "ryr_mlp_destroy", "ryr_mlp_set_params", "ryr_mlp_get_params",
+                "ryr_mlp_forward", "ryr_mlp_grad", "ryr_mlp_train", "ryr_mlp_last_error"]

@@ -11,22 +11,24 @@ from ryr_hyperactivity_amd import _abi
 from ryr_hyperactivity_amd import CellDesc, Params
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "ryr_cru.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("ryr_cru.h", "ryr_mlp.h")]
 
 
 def _declared():
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(ryr_[a-z0-9_]+)\s*\(", text)))
+    names = set()
+    for hdr in HEADERS:
+        text = re.sub(r"/\*.*?\*/", "", open(hdr).read(), flags=re.S)
+        names |= set(re.findall(r"\b(ryr_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
 
 
 def test_library_exports_every_declared_symbol():
     lib = _abi.load_library()
     names = _declared()
-    assert "ryr_create" in names and "ryr_gating_probe" in names
+    assert "ryr_create" in names and "ryr_gating_probe" in names and "ryr_mlp_train" in names
     for n in names:
         assert hasattr(lib, n), n
-    assert set(_abi.EXPORTED) <= set(names)
+    assert set(_abi.EXPORTED) | set(_abi.EXPORTED_MLP) <= set(names)
     out = subprocess.run(["nm", "-D", "--defined-only", _abi.LIB_PATH], capture_output=True, text=True).stdout
     for n in names:
         assert re.search(rf"\bT {n}\b", out), n
