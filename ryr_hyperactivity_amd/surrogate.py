@@ -214,18 +214,11 @@ class BeatSurrogate:
         ps, aps = np.array(ps), np.array(aps)
         return ps.mean(0), aps.mean(0), ps.std(0), aps.std(0)
 
-    def rollout(self, ku2, ikr, cl_ms, first_beat, ncru: int, nbeats: int):
-        """Per-beat EAD probability for beats 2..nbeats of a point, feeding each
-        predicted beat (APD90, EAD probability) back as the previous beat; the
-        other previous-beat features stay those of `first_beat`."""
-        prev = np.asarray(first_beat, dtype=np.float64).copy()
-        p_beats, spread = [], []
-        for _ in range(1, nbeats):
-            x = beat_features(ku2, ikr, cl_ms, prev, ncru)[None, :]
-            p, apd, sp, _ = self.predict(x)
-            p_beats.append(p[0])
-            spread.append(sp[0])
-            prev[1] = apd[0]
-            prev[4] = p[0]
-            prev[5] = 0.0
-        return np.array(p_beats), np.array(spread)
+    def predict_point(self, ku2, ikr, cl_ms, prev_beats, ncru: int):
+        """EAD incidence of a point and the networks' spread: the per-beat EAD
+        probability averaged over a set of previous-beat records (rows of
+        ryr_read_beats), typically the beats of the nearest simulated points,
+        with this point's parameters."""
+        x = np.array([beat_features(ku2, ikr, cl_ms, b, ncru) for b in prev_beats])
+        p, _, sp, _ = self.predict(x)
+        return float(p.mean()), float(sp.mean())

@@ -1,0 +1,91 @@
+"""Sweep driver: interval arithmetic and per-point statistics (CPU), and the
+surrogate-steered map against the plain method on a small grid (GPU)."""
+import math
+
+import numpy as np
+import pytest
+
+from ryr_hyperactivity_amd.sweep import (Point, PointResult, grid_points, plan_batches, point_cost, point_seed,
+                                         summarize, wilson)
+
+
+def test_wilson_interval():
+    lo, hi = wilson(5, 10)
+    assert lo == pytest.approx(0.2366, abs=1e-4) and hi == pytest.approx(0.7634, abs=1e-4)
+    lo, hi = wilson(0, 20)
+    assert lo == 0.0 and hi == pytest.approx(0.1611, abs=1e-4)
+    # coverage: the interval holds p in about 95% of binomial draws
+    rng = np.random.default_rng(0)
+    hits = 0
+    for _ in range(4000):
+        k = rng.binomial(40, 0.3)
+        lo, hi = wilson(int(k), 40)
+        hits += lo <= 0.3 <= hi
+    assert 0.93 < hits / 4000 < 0.97
+
+
+def test_summarize_counts_beats_after_the_first():
+    def beat(apd, ead, peak=1.0, jsr=700.0, sparks=10.0):
+        return [0, apd, 30, -80, ead, 60.0 if ead else -1, peak, jsr, sparks, 1]
+    reps = [np.array([beat(300, 1), beat(310, 1), beat(320, 0)]),        # first beat ignored
+            np.array([beat(250, 0), beat(260, 0), beat(-1, 1)])]         # a beat that never repolarised
+    r = summarize(PointResult(Point(2.0, 0.0, 1000.0), reps), ncru=1000)
+    assert r.ead == pytest.approx(0.5 * (0.5 + 0.5))
+    assert (r.ead_lo, r.ead_hi) == pytest.approx(wilson(2, 4))
+    assert r.apd90 == pytest.approx(0.5 * (315.0 + 260.0))
+    assert r.sparks == pytest.approx(10.0)
+
+
+def test_seeds_and_costs_are_functions_of_the_point():
+    a, b = Point(2.5, 0.25, 2000.0), Point(2.5, 0.25, 3000.0)
+    assert point_seed(a, 3) == point_seed(Point(2.5, 0.25, 2000.0), 3)
+    assert len({point_seed(a, r) for r in range(64)} | {point_seed(b, r) for r in range(64)}) == 128
+    assert point_cost(a, 4, 5) == 4 * (round((100 + 5 * 2000) / 0.025) + 2)
+    assert len(grid_points([1, 2], [0, 1], [1000, 2000, 3000])) == 12
+
+
+def test_batch_plan_merges_cycle_lengths_only_when_latency_bound():
+    pts = grid_points([1, 2], [0, 1], [1000.0, 2000.0, 4000.0])        # 4 points per cycle length
+    small = plan_batches(pts, replicas=4, nbeats=5, ncru=640)          # 16 cells of 640 CRUs per length
+    assert len(small) == 1 and len(small[0][0]) == 12                  # one batch at the longest length
+    assert small[0][1] == round((100 + 5 * 4000) / 0.025) + 2
+    big = plan_batches(pts, replicas=512, nbeats=5, ncru=20000)        # each length fills the chip
+    assert [len(g) for g, _ in big] == [4, 4, 4] and [s for _, s in big] == sorted([s for _, s in big], reverse=True)
+    assert [p for g, _ in big for p in g] and {p for g, _ in big for p in g} == set(pts)
+
+
+@pytest.mark.gpu
+def test_steered_map_spends_less_than_the_plain_map():
+    """The learned part against the method without it, on a small grid: the
+    plain method simulates all 18 points; the steered map simulates the 10
+    training points and the points the surrogate is unsure of, and takes the
+    prediction elsewhere.  Both use the same seeds, so a simulated point has
+    the same value either way and the plain results serve as the cache.
+    16 replicas x 4 scored beats put the plain estimate's own noise near 0.06."""
+    from ryr_hyperactivity_amd.sweep import Cost, simulate_points, steered_map
+    print("case steered map 16x6x4", flush=True)
+    lat, R, NB = (16, 6, 4), 16, 5
+    ikr = [0, 0.125, 0.25, 0.375, 0.5, 0.625, 0.75, 0.875, 1.0]
+    grid = grid_points([1.0, 4.0], ikr, [1000.0])
+    plain, pcost = simulate_points(grid, R, NB, lat, progress=lambda s: None)
+    train = grid_points([1.0, 4.0], ikr[::2], [1000.0])
+    calls = []
+
+    def cached(pts):
+        calls.append(len(pts))
+        return {p: plain[p] for p in pts}, Cost(cell_steps=sum(point_cost(p, R, NB) for p in pts))
+
+    sm = steered_map(grid, train, R, NB, lat, simulate=cached, train_steps=2000)
+    srcs = [sm.source[p] for p in grid]
+    n_sur = srcs.count("surrogate")
+    ratio = sm.cost.cell_steps / pcost.cell_steps
+    err = [abs(sm.values[p] - plain[p].ead) for p in grid if sm.source[p] == "surrogate"]
+    print(f"surrogate points {n_sur}/{len(grid)}, cell-step ratio {ratio:.2f}, "
+          f"surrogate-point error mean {np.mean(err) if err else 0:.3f} max {max(err) if err else 0:.3f}", flush=True)
+    assert calls[0] == len(train) and srcs.count("train") == len(train)
+    assert n_sur >= 3 and ratio < 0.85
+    assert np.mean(err) < 0.1 and max(err) < 0.25
+    # every uncertain prediction went to the simulator
+    for p, (inc, spr) in sm.predicted.items():
+        if 0.1 <= inc <= 0.9 or spr > 0.1:
+            assert sm.source[p] == "plain"
