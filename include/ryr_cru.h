@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define RYR_ABI_VERSION 2
+#define RYR_ABI_VERSION 3
 
 #define RYR_OK            0
 #define RYR_E_ARG        -1   /* invalid argument */
@@ -60,7 +60,33 @@ typedef struct ryr_params {
     double v0;              /* initial V (:354) = -80 mV */
     int    cm_mode;         /* 0: Cm = cm * N/16120 (current density preserved), 1: Cm = cm */
     double cm;              /* Cm (:100) = 45 */
+    int    variants;        /* OR of RYR_VAR_* (the reference's remaining -D builds); 0 = none */
+    int    reserved_;
 } ryr_params;
+
+/* Variants of ryr_params.variants (cuda_Cell_Detailed.cu, compile-time there):
+ *  PERM       permeabilised cell (:87-93,697-699,1249-1263,1294-1304,1409-1411,
+ *             1499-1501,1662-1689): no LCC gating or I_CaL, no NCX, no
+ *             stimulus; cs relaxes to ci0 with tau 5 ms; the ci and cs
+ *             neighbours outside the lattice are a bath at ci0 (their ghosts
+ *             are never refreshed).  The reference's perm build also fixes
+ *             ci_basal 0.3 and cjsr_basal 900: set ci0 / cjsr0 for that.
+ *  ISO        isoproterenol protocol (:683-693,1213-1229): stimulus at phase
+ *             0 of the cycle and only while t < 11.9 CL; from t >= 5.5 CL
+ *             SERCA doubles (unless iso2 or tapsi override it, as in the
+ *             reference); at t = 5.5 CL (mod 15.5 CL) absent LCC slots 4..7
+ *             become channels in state 3.
+ *  SLOWNCX    slowncxchange (:540-543,1267-1272,1420,1509): I_Ks qks gate
+ *             qks_inf = 0.2(1 + 1.6/(1 + (0.28/csubt)^3)), tau 2000 ms, and a
+ *             background Ca current 0.003016/32 (V - E_Ca(cs)) per CRU in the
+ *             sub-membrane balance and in the whole-cell I_CaL sum.
+ *  RANDOMLCC  LCCs per CRU drawn uniformly from 0..8 at set-up (:1114-1117,
+ *             1150-1155) instead of nlcc, from the Philox stream of the CRU
+ *             at counter step 2^64-1 (draw % 9). */
+#define RYR_VAR_PERM      1
+#define RYR_VAR_ISO       2
+#define RYR_VAR_SLOWNCX   4
+#define RYR_VAR_RANDOMLCC 8
 
 /* One independent cell.  Lattice dims are interior CRUs (the reference's
  * nx,ny,nz = 64,28,12 of :80-82 include a ghost shell: interior 62x26x10). */

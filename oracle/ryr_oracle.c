@@ -572,6 +572,30 @@ struct orc_cell {
 
 static void *orc_alloc(int64_t n, size_t sz) { return calloc((size_t)n, sz); }
 
+/* variant timing in whole steps (t = n dt; the reference tests t with fmod) */
+static int64_t orc_ncl(const orc_cell *c) { return (int64_t)llround(c->d.cl / c->p.dt); }
+/* iso: t >= 5.5 CL (:1220) */
+static int64_t orc_iso_n(const orc_cell *c) { return (11 * orc_ncl(c) + 1) / 2; }
+/* iso: fmod(t + 10 CL, 15.5 CL) < dt (:1214) */
+static int orc_iso_active(const orc_cell *c, int64_t n)
+{
+    const int64_t ncl = orc_ncl(c);
+    return ncl > 0 && (2 * (n + 10 * ncl)) % (31 * ncl) < 2;
+}
+/* stimulus phase: iso paces at phase 0 (:684), the default 100 ms in (:689) */
+static int64_t orc_stim_offset(const orc_cell *c)
+{
+    return (c->p.variants & RYR_VAR_ISO) ? 0 : (int64_t)llround(c->p.stim_offset / c->p.dt);
+}
+/* whether step n may be stimulated: never under perm (:697-699), while
+ * t < 11.9 CL under iso (:684) */
+static int orc_paced(const orc_cell *c, int64_t n)
+{
+    if (c->p.variants & RYR_VAR_PERM) return 0;
+    if (c->p.variants & RYR_VAR_ISO) return n < (119 * orc_ncl(c) + 9) / 10;
+    return 1;
+}
+
 orc_cell *orc_create(const ryr_params *p, const ryr_cell_desc *d)
 {
     if (d->nx < 1 || d->ny < 1 || d->nz < 1 || p->dt <= 0) return NULL;
@@ -638,7 +662,18 @@ orc_cell *orc_create(const ryr_params *p, const ryr_cell_desc *d)
         c->casarh[ac] = O_KSARHON * ci_basal * O_BSARH / (O_KSARHON * ci_basal + O_KSARHOFF);
         c->casarj[ac] = O_KSARON * ci_basal * O_BSAR / (O_KSARON * ci_basal + O_KSAROFF);
         c->casarhj[ac] = O_KSARHON * ci_basal * O_BSARH / (O_KSARHON * ci_basal + O_KSARHOFF);
-        for (int ll = 0; ll < 8; ll++) c->lcc[ac * 8 + ll] = (ll < p->nlcc) ? 3 : 16;
+        int nlc = p->nlcc;
+        if (p->variants & RYR_VAR_RANDOMLCC) {
+            /* randpoint = curand % 9 (:1150-1152): one 32-bit draw of the
+             * CRU's stream at counter step 2^64-1 */
+            const int64_t gi = ac % c->nx, gj = (ac / c->nx) % c->ny, gk = ac / ((int64_t)c->nx * c->ny);
+            if (gi >= 1 && gi <= c->X && gj >= 1 && gj <= c->Y && gk >= 1 && gk <= c->Z) {
+                ryr_stream s0;
+                ryr_stream_init(&s0, d->seed, (uint32_t)INTERIOR(c, gi, gj, gk), ~(uint64_t)0);
+                nlc = (int)(ryr_stream_u32(&s0) % 9u);
+            }
+        }
+        for (int ll = 0; ll < 8; ll++) c->lcc[ac * 8 + ll] = (ll < nlc) ? 3 : 16;
         c->ncb[ac] = (int)(fb * O_NRYR);
         c->ncu[ac] = O_NRYR - (int)(fb * O_NRYR);
         c->nob[ac] = 0;
@@ -700,19 +735,33 @@ static void orc_compute_cru(orc_cell *c, int i, int j, int k, int reset, int64_t
     memset(&g, 0, sizeof g);
     ryr_stream_init(&g.ph, c->d.seed, (uint32_t)INTERIOR(c, i, j, k), (uint64_t)c->step);
 
+    const int var = c->p.variants;
+    const int perm = (var & RYR_VAR_PERM) != 0;
+    double sviup = c->sviup;                                        /* :1205-1238 */
+    if (var & RYR_VAR_ISO) {
+        if (orc_iso_active(c, c->step)) {                           /* :1214-1219 */
+            for (int ll = 4; ll < 8; ++ll)
+                if (c->lcc[ac * 8 + ll] == 16) c->lcc[ac * 8 + ll] = 3;
+        }
+        if (!c->p.iso2 && !c->p.tapsi && c->step >= orc_iso_n(c)) sviup = O_SV_IUP * 2.;
+    }
     if (reset) { c->nspark[ac] = 0; c->nl[ac] = 0; }                /* :1244-1247 */
 
     /* LCC gating and I_CaL (:1248-1263) */
     double icat1 = 0;
-    int nlcp = 0;
-    for (int ll = 0; ll < 8 && c->lcc[ac * 8 + ll] < 16; ll++) {
-        double cpx = (c->cp[ac] + cs[ac] * O_ICAGAMMA) / (1 + O_ICAGAMMA);
-        int jj = orc_lcc_markov(c->lcc[ac * 8 + ll], cpx, v, dt, orc_u(&g), c->p.iso2);
-        c->lcc[ac * 8 + ll] = jj;
-        if (jj == 0) { ++c->nl[ac]; ++nlcp; }
+    if (!perm) {
+        int nlcp = 0;
+        for (int ll = 0; ll < 8 && c->lcc[ac * 8 + ll] < 16; ll++) {
+            double cpx = (c->cp[ac] + cs[ac] * O_ICAGAMMA) / (1 + O_ICAGAMMA);
+            int jj = orc_lcc_markov(c->lcc[ac * 8 + ll], cpx, v, dt, orc_u(&g), c->p.iso2);
+            c->lcc[ac * 8 + ll] = jj;
+            if (jj == 0) { ++c->nl[ac]; ++nlcp; }
+        }
+        icat1 = c->svica * (double)(nlcp) * orc_lcccurrent(v, (c->cp[ac] + cs[ac] * O_ICAGAMMA) / (1 + O_ICAGAMMA));
     }
-    icat1 = c->svica * (double)(nlcp) * orc_lcccurrent(v, (c->cp[ac] + cs[ac] * O_ICAGAMMA) / (1 + O_ICAGAMMA));
-    c->xicat[ac] = icat1 + 0 + 0 * (O_VS / O_VP);                   /* :1265-1272 */
+    double bcgcur = 0;                                              /* :1265-1272 */
+    if (var & RYR_VAR_SLOWNCX) bcgcur = 1.0 * 0.003016 / 32. * (v - ryr_log(180000 / cs[ac]) / 2. / O_FRT);
+    c->xicat[ac] = icat1 + 0 + bcgcur * (O_VS / O_VP);
 
     /* RyR gating, spark onset, release (:1274-1293) */
     int prevsp = c->nou[ac] + c->nob[ac];
@@ -726,11 +775,11 @@ static void orc_compute_cru(orc_cell *c, int i, int j, int k, int reset, int64_t
     if (c->xire[ac] < 0) c->xire[ac] = 0;
 
     /* NCX (:1294-1304) */
-    c->xinaca[ac] = c->svncx * 1.0 * orc_ncx(cs[ac] / 1000.0, v, xnai, &c->Ancx[ac], dt);
+    c->xinaca[ac] = perm ? 0 : c->svncx * 1.0 * orc_ncx(cs[ac] / 1000.0, v, xnai, &c->Ancx[ac], dt);
 
     /* SERCA and leak (:1305-1308) */
     for (int ii = 0; ii < 8; ++ii) {
-        c->xiup[ac * 8 + ii] = (c->sviup + c->sviup_lqt) * orc_uptake(ci[ac * 8 + ii], cnsr[ac * 8 + ii]);
+        c->xiup[ac * 8 + ii] = (sviup + c->sviup_lqt) * orc_uptake(ci[ac * 8 + ii], cnsr[ac * 8 + ii]);
         c->xileak[ac * 8 + ii] = O_SVILEAK * orc_leak(cnsr[ac * 8 + ii], ci[ac * 8 + ii]);
     }
     /* exchange fluxes and JSR (:1309-1318) */
@@ -791,6 +840,7 @@ static void orc_compute_cru(orc_cell *c, int i, int j, int k, int reset, int64_t
             double csbuffers = (b_busar(c->casar[ac], csnext[ac], dth) +
                                 b_busarh(c->casarh[ac], csnext[ac], dth));
             double csdiff = (cs[POS(c, i, j, k + 1)] + cs[POS(c, i, j, k - 1)] - 2 * cs[ac]) / (O_TAUST);
+            if (perm) csdiff += (c->p.ci0 - csnext[ac]) / 5;          /* :1409-1411 */
             double dotcs = O_VP / O_VS * (c->cp[ac] - csnext[ac]) / (O_TAUPS)
                            - O_ICAGAMMA * icat1 * O_VP / O_VS
                            + c->xinaca[ac]
@@ -798,7 +848,7 @@ static void orc_compute_cru(orc_cell *c, int i, int j, int k, int reset, int64_t
                            - (cs[ac] - ci[ac * 8 + 1]) / O_TAUSI / 2.
                            + csdiff
                            - csbuffers
-                           - 0;
+                           - bcgcur;
             double cpbuff = b_busarj(c->casarj[ac], c->cp[ac], dth);
             double cpbuffh = b_busarhj(c->casarhj[ac], c->cp[ac], dth);
             double cpdot = ((csnext[ac] - c->cp[ac]) / O_TAUPS +
@@ -833,13 +883,14 @@ static void orc_compute_cru(orc_cell *c, int i, int j, int k, int reset, int64_t
             double csbuffers = (b_busar(c->casar[ac], csnext[ac], dth) +
                                 b_busarh(c->casarh[ac], csnext[ac], dth));
             double csdiff = (cs[POS(c, i, j, k + 1)] + cs[POS(c, i, j, k - 1)] - 2 * csnext[ac]) / (O_TAUST);
+            if (perm) csdiff += (c->p.ci0 - csnext[ac]) / 5;          /* :1499-1501 */
             double dotcs = O_VP / O_VS * (c->cp[ac] - O_ICAGAMMA * O_TAUPS * icat1 - csnext[ac]) / (O_TAUPS)
                            + c->xinaca[ac]
                            - (csnext[ac] - ci[ac * 8]) / O_TAUSI / 2.
                            - (csnext[ac] - ci[ac * 8 + 1]) / O_TAUSI / 2.
                            + csdiff
                            - csbuffers
-                           - 0;
+                           - bcgcur;
             double cpbuff = b_busarj(c->casarj[ac], c->cp[ac], dth);
             double cpbuffh = b_busarhj(c->casarhj[ac], c->cp[ac], dth);
             double cpdot = ((cs[ac] - c->cp[ac]) / O_TAUPS +
@@ -882,8 +933,10 @@ static void orc_compute_cru(orc_cell *c, int i, int j, int k, int reset, int64_t
 }
 
 /* Ghost copies for one interior CRU (cuda_Cell_Detailed.cu:1601-1652). */
+static void orc_boundary_perm(orc_cell *c, int i, int j, int k);
 static void orc_boundary(orc_cell *c, int i, int j, int k)
 {
+    if (c->p.variants & RYR_VAR_PERM) { orc_boundary_perm(c, i, j, k); return; }
     double *cinext = c->cinext, *cnsrnext = c->cnsrnext, *csnext = c->csnext;
     if (k == 1) {
         for (int ii = 1; ii < 8; ii += 2) {
@@ -929,12 +982,39 @@ static void orc_boundary(orc_cell *c, int i, int j, int k)
     }
 }
 
+/* perm build (:1662-1689): only the NSR ghosts are refreshed; the ci and cs
+ * ghosts keep their set-up value ci0 for the whole run (a bath) */
+static void orc_boundary_perm(orc_cell *c, int i, int j, int k)
+{
+    double *cnsrnext = c->cnsrnext;
+    if (k == 1)
+        for (int ii = 1; ii < 8; ii += 2) cnsrnext[POS(c, i, j, 0) * 8 + ii] = cnsrnext[POS(c, i, j, 1) * 8 + ii - 1];
+    if (k == c->nz - 2)
+        for (int ii = 0; ii < 8; ii += 2)
+            cnsrnext[POS(c, i, j, c->nz - 1) * 8 + ii] = cnsrnext[POS(c, i, j, c->nz - 2) * 8 + ii + 1];
+    if (j == 1)
+        for (int ii = 2; ii < 8; ii += 4) {
+            cnsrnext[POS(c, i, 0, k) * 8 + ii] = cnsrnext[POS(c, i, 1, k) * 8 + ii - 2];
+            cnsrnext[POS(c, i, 0, k) * 8 + ii + 1] = cnsrnext[POS(c, i, 1, k) * 8 + ii - 1];
+        }
+    if (j == c->ny - 2)
+        for (int ii = 0; ii < 8; ii += 4) {
+            cnsrnext[POS(c, i, c->ny - 1, k) * 8 + ii] = cnsrnext[POS(c, i, c->ny - 2, k) * 8 + ii + 2];
+            cnsrnext[POS(c, i, c->ny - 1, k) * 8 + ii + 1] = cnsrnext[POS(c, i, c->ny - 2, k) * 8 + ii + 3];
+        }
+    if (i == 1)
+        for (int ii = 4; ii < 8; ++ii) cnsrnext[POS(c, 0, j, k) * 8 + ii] = cnsrnext[POS(c, 1, j, k) * 8 + ii - 4];
+    if (i == c->nx - 2)
+        for (int ii = 0; ii < 4; ++ii)
+            cnsrnext[POS(c, c->nx - 1, j, k) * 8 + ii] = cnsrnext[POS(c, c->nx - 2, j, k) * 8 + ii + 4];
+}
+
 /* Whole-cell reduction + membrane ODE (cuda_Cell_Detailed.cu:447-739) on a
  * 13-value state [v xm xh xj xr xs1 xs2 qks xtos ytos xtof ytof xnai].
  * currents out: [csubt xicatto xinacato xina xiks xikr xik1 xinak xitos xitof] */
 static void orc_membrane(double *s, double sum_icat, double sum_inaca, double sum_cs,
                          double ncru, double Cm, double dt, double svikr, double stim,
-                         double *cur)
+                         double *cur, int slowncx)
 {
     double v = s[0], xm = s[1], xh = s[2], xj = s[3], xr = s[4], xs1 = s[5], xs2 = s[6];
     double qks = s[7], xtos = s[8], ytos = s[9], xtof = s[10], ytof = s[11], xnai = s[12];
@@ -987,6 +1067,10 @@ static void orc_membrane(double *s, double sum_icat, double sum_inaca, double su
     double prnak = 0.0183300;
     double qks_inf = 0.2 * (1 + 0.8 / (1 + ryr_pow((0.28 / csubt), 3)));
     double tauqks = 1000.00;
+    if (slowncx) {                                                  /* :540-543 */
+        qks_inf = 0.2 * (1 + 1.6 / (1 + ryr_pow((0.28 / csubt), 3)));
+        tauqks = 2000;
+    }
     double eks = (1.00 / O_FRT) * ryr_log((xko + prnak * O_XNAO) / (xki + prnak * xnai));
     double xs1ss = 1.0 / (1.0 + ryr_exp(-(v - 1.500) / 16.700));
     double tauxs = 1.00 / (0.0000719 * (v + 30.00) / (1.00 - ryr_exp(-0.1480 * (v + 30.0)))
@@ -1061,7 +1145,7 @@ int orc_membrane_probe(double *state13, double sum_icat, double sum_inaca, doubl
                        double ncru, double cm, double dt, double ikr_scale, double stim,
                        double *currents)
 {
-    orc_membrane(state13, sum_icat, sum_inaca, sum_cs, ncru, cm, dt, 3.0 * ikr_scale, stim, currents);
+    orc_membrane(state13, sum_icat, sum_inaca, sum_cs, ncru, cm, dt, 3.0 * ikr_scale, stim, currents, 0);
     return 0;
 }
 
@@ -1080,8 +1164,8 @@ static void orc_beat_close(orc_cell *c, double complete)
 static void orc_beat_track(orc_cell *c, int64_t n, double vo, double vn)
 {
     const double dt = c->p.dt;
-    int64_t ncl = (int64_t)llround(c->d.cl / dt), noff = (int64_t)llround(c->p.stim_offset / dt);
-    if (ncl > 0 && n >= noff && (n - noff) % ncl == 0) {
+    int64_t ncl = (int64_t)llround(c->d.cl / dt), noff = orc_stim_offset(c);
+    if (ncl > 0 && n >= noff && (n - noff) % ncl == 0 && orc_paced(c, n)) {
         if (c->beat_on >= 0) orc_beat_close(c, 1.0);
         c->beat_on = n;
         c->b_vrest = vo; c->b_vmax = vo; c->b_vmin = vo;
@@ -1145,9 +1229,9 @@ int orc_read_beats(const orc_cell *c, double *out, int maxbeats, int *nbeats)
 static double orc_stim(const orc_cell *c, int64_t n)
 {
     int64_t ncl = (int64_t)llround(c->d.cl / c->p.dt);
-    int64_t noff = (int64_t)llround(c->p.stim_offset / c->p.dt);
+    int64_t noff = orc_stim_offset(c);
     int64_t ndur = (int64_t)llround(c->p.stim_dur / c->p.dt);
-    if (ncl <= 0) return 0.0;
+    if (ncl <= 0 || !orc_paced(c, n)) return 0.0;
     int64_t ph = (n - noff) % ncl;
     if (ph < 0) ph += ncl;
     return ph < ndur ? c->p.stim_amp : 0.0;
@@ -1322,7 +1406,8 @@ static void orc_one_step(orc_cell *c)
     double cur[10];
     c->stim = orc_stim(c, n);
     const double vold = c->v;
-    orc_membrane(st, sum_icat, sum_inaca, sum_cs, (double)c->N, c->Cm, c->p.dt, c->svikr, c->stim, cur);
+    orc_membrane(st, sum_icat, sum_inaca, sum_cs, (double)c->N, c->Cm, c->p.dt, c->svikr, c->stim, cur,
+                 (c->p.variants & RYR_VAR_SLOWNCX) != 0);
     if (c->vclamp_n > 0)   /* clamp protocols (cuda_Cell_Detailed.cu:733-773) */
         st[0] = c->vclamp[c->vclamp_periodic ? n % c->vclamp_n : (n < c->vclamp_n ? n : c->vclamp_n - 1)];
     c->v = st[0]; c->xm = st[1]; c->xh = st[2]; c->xj = st[3]; c->xr = st[4]; c->xs1 = st[5];

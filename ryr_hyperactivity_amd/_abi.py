@@ -21,6 +21,7 @@ RYR_E_NODEV = -3
 RYR_E_STATE = -4
 
 SCHEME_A, SCHEME_B = 0, 1
+VAR_PERM, VAR_ISO, VAR_SLOWNCX, VAR_RANDOMLCC = 1, 2, 4, 8
 PREC_F64, PREC_F32 = 0, 1
 
 DIAG_COLS = 45
@@ -61,14 +62,15 @@ class Params(ctypes.Structure):
                 ("nlcc", ctypes.c_int), ("steps_per_output", ctypes.c_int),
                 ("stim_amp", ctypes.c_double), ("stim_dur", ctypes.c_double),
                 ("stim_offset", ctypes.c_double), ("ci0", ctypes.c_double), ("v0", ctypes.c_double),
-                ("cm_mode", ctypes.c_int), ("cm", ctypes.c_double)]
+                ("cm_mode", ctypes.c_int), ("cm", ctypes.c_double), ("variants", ctypes.c_int),
+                ("reserved_", ctypes.c_int)]
 
     @classmethod
     def default(cls, **kw):
         """The reference's values (run.sh `make il`: iso2 + lqt)."""
         p = cls(dt=0.025, scheme=SCHEME_A, precision=PREC_F64, iso2=1, tapsi=0, lqt=1, nlcc=4,
                 steps_per_output=100, stim_amp=80.0, stim_dur=1.0, stim_offset=100.0, ci0=0.0944,
-                v0=-80.0, cm_mode=0, cm=45.0)
+                v0=-80.0, cm_mode=0, cm=45.0, variants=0, reserved_=0)
         for k, v in kw.items():
             setattr(p, k, v)
         return p
@@ -161,7 +163,7 @@ def load_library(path: str | None = None):
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    if lib.ryr_abi_version() != 2:
+    if lib.ryr_abi_version() != 3:
         raise LibraryMissing(f"{p}: ABI version mismatch")
     if path is None:
         _lib = lib

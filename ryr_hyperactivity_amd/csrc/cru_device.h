@@ -52,7 +52,26 @@ struct RyrCellInfo {
     int cell, pad2_;        /* index of the cell (per-chunk copies carry it) */
     /* n / X and n / (X*Y) by multiply-high (ryr_fastdiv): multiplier, shifts */
     uint32_t dX_m, dX_s1, dX_s2, dXY_m, dXY_s1, dXY_s2;
+    /* variants (RYR_VAR_* of ryr_params) */
+    int variants, pad3_;
+    long long iso_n;        /* iso: first step with t >= 5.5 CL */
+    long long nstim_end;    /* no stimulus (and no beat onset) from this step on; -1: none */
+    double sviup_iso;       /* iso: SERCA scale from iso_n on */
+    double ci_bath;         /* perm: ci/cs outside the lattice */
 };
+
+#define RYR_VAR_PERM      1
+#define RYR_VAR_ISO       2
+#define RYR_VAR_SLOWNCX   4
+#define RYR_VAR_RANDOMLCC 8
+/* aux word: nl (bits 0-15), spark flag (16), LCC slots present (17-24) */
+#define RYR_AUX_MASK_SHIFT 17
+
+/* iso: fmod(t + 10 CL, 15.5 CL) < dt (cuda_Cell_Detailed.cu:1214) in steps */
+RYR_DEV bool iso_active(long long n, long long ncl)
+{
+    return ncl > 0 && (2 * (n + 10 * ncl)) % (31 * ncl) < 2;
+}
 
 /* Division of a 32-bit unsigned n by an invariant d >= 1 (Granlund and
  * Montgomery, PLDI 1994, Fig. 4.1): with l = ceil(log2 d),
@@ -383,7 +402,7 @@ RYR_DEV double lanev(double x, int src) { return __shfl(x, src, 64); }
 
 RYR_DEV void membrane_wave(const double *s, double svikr, double Cm, double dN, double sum_icat,
                            double sum_inaca, double sum_cs, double dt, double stim, int lane, double *out,
-                           double *cur)
+                           double *cur, bool slowncx)
 {
     RYR_STRICT_FP
     double v = s[0], xm = s[1], xh = s[2], xj = s[3], xr = s[4], xs1 = s[5], xs2 = s[6];
@@ -481,8 +500,10 @@ RYR_DEV void membrane_wave(const double *s, double svikr, double Cm, double dN, 
     /* I_Ks (:534-559) */
     const double prnak = 0.0183300;
     (void)prnak;
-    const double qks_inf = 0.2 * (1 + 0.8 / (1 + ryr_pow((0.28 / csubt), 3)));
-    const double tauqks = 1000.00;
+    /* slowncxchange (:540-543) doubles the Ca dependence and the time constant */
+    const double qks_inf = slowncx ? 0.2 * (1 + 1.6 / (1 + ryr_pow((0.28 / csubt), 3)))
+                                   : 0.2 * (1 + 0.8 / (1 + ryr_pow((0.28 / csubt), 3)));
+    const double tauqks = slowncx ? 2000 : 1000.00;
     const double eks = (1.00 / frt) * lanev(lg, 1);
     const double xs1ss = 1.0 / (1.0 + lanev(ex, 17));
     double tauxs = 1.00 / (0.0000719 * (v + 30.00) / (1.00 - lanev(ex, 18))
@@ -613,7 +634,7 @@ RYR_DEV void vfactors_wave(RyrMem *M, int iso2, int lane, const double v, const 
 
 RYR_DEV double stim_of(const RyrCellInfo &C, long long n, double amp)
 {
-    if (C.ncl <= 0) return 0.0;
+    if (C.ncl <= 0 || (C.nstim_end >= 0 && n >= C.nstim_end)) return 0.0;
     long long ph = (n - C.noff) % C.ncl;
     if (ph < 0) ph += C.ncl;
     return ph < C.ndur ? amp : 0.0;
@@ -654,7 +675,8 @@ RYR_DEV void beat_track(RyrMem *M, const RyrCellInfo &C, long long n, double dt,
                         double *beats)
 {
     RYR_STRICT_FP
-    const bool onset = C.ncl > 0 && n >= C.noff && ((n - C.noff) % C.ncl) == 0;
+    const bool onset = C.ncl > 0 && n >= C.noff && ((n - C.noff) % C.ncl) == 0 &&
+                       (C.nstim_end < 0 || n < C.nstim_end);
     if (onset) {
         if (M->beat_on >= 0) beat_close(M, beats, 1.0);
         M->beat_on = n;
@@ -755,40 +777,44 @@ __global__ __launch_bounds__(WG) void k_cyto(RyrKArgs a, int kc, int km, double 
      * interior the mirrored ghost equals the centre value */
     const TF cc = ci0[e];
     const TS cn = cn0[e];
+    /* outside the lattice: the centre value, or under perm the bath ci0
+     * (the reference never refreshes those ghosts, :1662-1689) */
+    const TF cb = (C.variants & RYR_VAR_PERM) ? (TF)C.ci_bath : cc;
     TF inN, inS, inE, inW, inT, inB;
     TS nnN, nnS, nnE, nnW, nnT, nnB;
     if (ii & 1) {
         const bool in = n + sz < C.N;      /* k + 1 < Z */
-        inN = in ? ci0[(ii - 1) * P + g + sz] : cc; nnN = in ? cn0[(ii - 1) * P + g + sz] : cn;
+        inN = in ? ci0[(ii - 1) * P + g + sz] : cb; nnN = in ? cn0[(ii - 1) * P + g + sz] : cn;
         inS = ci0[(ii - 1) * P + g]; nnS = cn0[(ii - 1) * P + g];
     } else {
         const bool in = n >= sz;           /* k > 0 */
         inN = ci0[(ii + 1) * P + g]; nnN = cn0[(ii + 1) * P + g];
-        inS = in ? ci0[(ii + 1) * P + g - sz] : cc; nnS = in ? cn0[(ii + 1) * P + g - sz] : cn;
+        inS = in ? ci0[(ii + 1) * P + g - sz] : cb; nnS = in ? cn0[(ii + 1) * P + g - sz] : cn;
     }
     if (ii & 2) {
         const bool in = rxy + sy < sz;     /* j + 1 < Y */
-        inE = in ? ci0[(ii - 2) * P + g + sy] : cc; nnE = in ? cn0[(ii - 2) * P + g + sy] : cn;
+        inE = in ? ci0[(ii - 2) * P + g + sy] : cb; nnE = in ? cn0[(ii - 2) * P + g + sy] : cn;
         inW = ci0[(ii - 2) * P + g]; nnW = cn0[(ii - 2) * P + g];
     } else {
         const bool in = rxy >= sy;         /* j > 0 */
         inE = ci0[(ii + 2) * P + g]; nnE = cn0[(ii + 2) * P + g];
-        inW = in ? ci0[(ii + 2) * P + g - sy] : cc; nnW = in ? cn0[(ii + 2) * P + g - sy] : cn;
+        inW = in ? ci0[(ii + 2) * P + g - sy] : cb; nnW = in ? cn0[(ii + 2) * P + g - sy] : cn;
     }
     if (ii & 4) {
         const bool in = i + 1 < X;
-        inT = in ? ci0[(ii - 4) * P + g + 1] : cc; nnT = in ? cn0[(ii - 4) * P + g + 1] : cn;
+        inT = in ? ci0[(ii - 4) * P + g + 1] : cb; nnT = in ? cn0[(ii - 4) * P + g + 1] : cn;
         inB = ci0[(ii - 4) * P + g]; nnB = cn0[(ii - 4) * P + g];
     } else {
         const bool in = i > 0;
         inT = ci0[(ii + 4) * P + g]; nnT = cn0[(ii + 4) * P + g];
-        inB = in ? ci0[(ii + 4) * P + g - 1] : cc; nnB = in ? cn0[(ii + 4) * P + g - 1] : cn;
+        inB = in ? ci0[(ii + 4) * P + g - 1] : cb; nnB = in ? cn0[(ii + 4) * P + g - 1] : cn;
     }
 
     if constexpr (sizeof(TF) == 8) {
         /* exact path: the reference's operand order throughout */
         const double dt = a.dt;
-        const double xiup = C.sviup_tot * uptake_d(cc, cn);
+        const double sviup = ((C.variants & RYR_VAR_ISO) && nstep >= C.iso_n) ? C.sviup_iso : C.sviup_tot;
+        const double xiup = sviup * uptake_d(cc, cn);
         const double xileak = 1.0 * leak_t<double>(cn, cc);
         const double xicoupn = ((nnN - cn) / (1.8 * 0.7) + (nnS - cn) / (1.8 * 0.7) +
                                 (nnE - cn) / (1.8 * 0.7) + (nnW - cn) / (1.8 * 0.7) +
@@ -848,7 +874,8 @@ __global__ __launch_bounds__(WG) void k_cyto(RyrKArgs a, int kc, int km, double 
         const float rIT = (float)(1.0 / ((0.33 * 2.) * 0.7)), rIL = (float)(1.0 / ((0.7 * 2.) * 0.7));
         const float rNT = (float)(1.0 / (1.8 * 0.7)), rNL = (float)(1.0 / (6. * 0.7));
         const float cf = (float)cc, cnf = (float)cn;
-        const float xiup = (float)C.sviup_tot * uptake_f(cf, cnf);
+        const double sviup = ((C.variants & RYR_VAR_ISO) && nstep >= C.iso_n) ? C.sviup_iso : C.sviup_tot;
+        const float xiup = (float)sviup * uptake_f(cf, cnf);
         const float cn2 = cnf * cnf;
         const float xileak = 1.035e-5f * (cnf - cf) * cn2 * __builtin_amdgcn_rcpf(cn2 + 250000.0f);
         const float xicoupi = (((float)inN - cf) + ((float)inS - cf) + ((float)inE - cf) + ((float)inW - cf)) * rIT +
@@ -899,8 +926,8 @@ __global__ __launch_bounds__(WG) void k_cyto(RyrKArgs a, int kc, int km, double 
  * cytosol kernel's partial rates of sub-voxels 0 and 1. */
 template <bool schemeA>
 RYR_DEV void fast_cleft(float dt, float cif0, float cif1, float csf0, float csu, float csd,
-                        float ica, float xna, float xre, float &cp, float &cs, float &sar, float &sarh, float &sarj,
-                        float &sarhj, float &dci0, float &dci1)
+                        float ica, float xna, float xre, float bcg, bool perm, float bath, float &cp, float &cs,
+                        float &sar, float &sarh, float &sarj, float &sarhj, float &dci0, float &dci1)
 {
     const float Vs_i = (float)((0.025) / (0.5 / 8.)), Vp_i = (float)((0.00126) / (0.5 / 8.));
     const float Vp_s = (float)((0.00126) / (0.025));
@@ -918,7 +945,8 @@ RYR_DEV void fast_cleft(float dt, float cif0, float cif1, float csf0, float csu,
         for (int it = 0; it < 5; ++it) {
             const float fsar = fbuf(sar, cs, dth, sath, 0.1f, 1.3f, (float)RYR_BSAR);
             const float fsarh = fbuf(sarh, cs, dth, sath, 0.1f, 0.03f, (float)RYR_BSARH);
-            const float dotcs = Vp_s * (cp - cs) * r_taups - ica_s + xna - to_ci + csdiff - (fsar + fsarh);
+            const float dotcs = Vp_s * (cp - cs) * r_taups - ica_s + xna - to_ci + csdiff - (fsar + fsarh) - bcg +
+                                (perm ? (bath - cs) * 0.2f : 0.0f);
             const float cpbuff = fbuf(sarj, cp, dth, sath, 0.1f, 1.3f, (float)RYR_BSAR);
             const float cpbuffh = fbuf(sarhj, cp, dth, sath, 0.1f, 0.03f, (float)RYR_BSARH);
             const float cpdot = (cs - cp) * r_taups + cp_ci + xre - ica - cpbuff - cpbuffh;
@@ -938,7 +966,8 @@ RYR_DEV void fast_cleft(float dt, float cif0, float cif1, float csf0, float csu,
             const float fsar = fbuf(sar, cs, dth, sath, 0.1f, 1.3f, (float)RYR_BSAR);
             const float fsarh = fbuf(sarh, cs, dth, sath, 0.1f, 0.03f, (float)RYR_BSARH);
             const float dotcs = Vp_s * (cp - 0.5f * 0.0283f * ica - cs) * r_taups + xna -
-                                ((cs - cif0) + (cs - cif1)) * r_tausi2 + (csum - 2.0f * cs) * r_taust - (fsar + fsarh);
+                                ((cs - cif0) + (cs - cif1)) * r_tausi2 + (csum - 2.0f * cs) * r_taust - (fsar + fsarh) -
+                                bcg + (perm ? (bath - cs) * 0.2f : 0.0f);
             const float cpbuff = fbuf(sarj, cp, dth, sath, 0.1f, 1.3f, (float)RYR_BSAR);
             const float cpbuffh = fbuf(sarhj, cp, dth, sath, 0.1f, 0.03f, (float)RYR_BSARH);
             const float cpdot = (csf0 - cp) * r_taups + ((cif0 - cp) + (cif1 - cp)) * r_taupi2 + xre - ica -
@@ -1004,34 +1033,50 @@ __global__ __launch_bounds__(WG) void k_cru_step(RyrKArgs a, int kc)
         TF cpv = cp0[g];
         const TS cjsrv = cjsr0[g];
         uint32_t lw = a.lccw[g], rw = a.ryrw[g], ax = a.aux[g];
-        if (reset) ax = 0;
+        if (reset) ax &= ~0x1ffffu;                 /* nl and spark flag; the LCC slots stay */
         int nl = (int)(ax & 0xffffu), nspark = (int)((ax >> 16) & 1u);
+        uint32_t lmask = (ax >> RYR_AUX_MASK_SHIFT) & 0xffu;
         int nfail = 0, nclampcp = 0, nclampcs = 0;
+        const bool perm = (C.variants & RYR_VAR_PERM) != 0;
+        if ((C.variants & RYR_VAR_ISO) && iso_active(nstep, C.ncl)) {
+            /* iso (:1214-1219): absent slots 4..7 become channels in state 3 */
+            for (int ll = 4; ll < 8; ++ll)
+                if (!((lmask >> ll) & 1u)) {
+                    lmask |= 1u << ll;
+                    lw = (lw & ~(15u << (4 * ll))) | (3u << (4 * ll));
+                }
+        }
 
         ryr_stream rs;
         ryr_stream_init(&rs, C.seed, (uint32_t)n, (uint64_t)nstep);
 
-        /* LCC gating and I_CaL (cuda_Cell_Detailed.cu:1248-1272) */
+        /* LCC gating and I_CaL (cuda_Cell_Detailed.cu:1248-1272): the slots
+         * up to the first absent one, as the reference's loop */
         const double cp_d = (double)cpv, cs_d = (double)csv;
-        double icat1, xicat;
+        double icat1 = 0.0, xicat, bcgcur = 0.0;
         {
             RYR_STRICT_FP
-            const double cpx = (cp_d + cs_d * (0.5)) / (1 + (0.5));
-            const double betac = a.iso2 ? 2.0 : 4.0;
-            int nlcp = 0;
-            uint32_t nw = lw;
-            for (int ll = 0; ll < C.nlcc; ++ll) {
-                int st = (int)((lw >> (4 * ll)) & 15u);
-                int jj = lcc_step(st, cpx, dt, ryr_stream_uniform(&rs), *M, betac);
-                nw = (nw & ~(15u << (4 * ll))) | ((uint32_t)jj << (4 * ll));
-                if (jj == 0) { ++nl; ++nlcp; }
+            if (!perm) {
+                const double cpx = (cp_d + cs_d * (0.5)) / (1 + (0.5));
+                const double betac = a.iso2 ? 2.0 : 4.0;
+                int nlcp = 0;
+                uint32_t nw = lw;
+                for (int ll = 0; ll < 8 && ((lmask >> ll) & 1u); ++ll) {
+                    int st = (int)((lw >> (4 * ll)) & 15u);
+                    int jj = lcc_step(st, cpx, dt, ryr_stream_uniform(&rs), *M, betac);
+                    nw = (nw & ~(15u << (4 * ll))) | ((uint32_t)jj << (4 * ll));
+                    if (jj == 0) { ++nl; ++nlcp; }
+                }
+                lw = nw;
+                double X_ = cpx / 1000.0 * M->ghk_e2 - (1 * 1.8);
+                double ica = M->ghk_small ? 2.0 * 11.9 * 96.485 * 0.341 * X_ : M->ghk_A * X_ / M->ghk_D;
+                if (ica > 0.0) ica = 0.0;
+                icat1 = C.svica * (double)(nlcp) * ica;
             }
-            lw = nw;
-            double X_ = cpx / 1000.0 * M->ghk_e2 - (1 * 1.8);
-            double ica = M->ghk_small ? 2.0 * 11.9 * 96.485 * 0.341 * X_ : M->ghk_A * X_ / M->ghk_D;
-            if (ica > 0.0) ica = 0.0;
-            icat1 = C.svica * (double)(nlcp) * ica;
-            xicat = icat1 + 0.0 + 0.0 * ((0.025) / (0.00126));
+            /* slowncxchange background Ca current (:1266-1268) */
+            if (C.variants & RYR_VAR_SLOWNCX)
+                bcgcur = 1.0 * 0.003016 / 32. * (v - ryr_log(180000 / cs_d) / 2. / (96.485 / 8.314 / 308.0));
+            xicat = icat1 + 0.0 + bcgcur * ((0.025) / (0.00126));
         }
 
         /* RyR gating, spark, release (:1274-1293) */
@@ -1056,7 +1101,10 @@ __global__ __launch_bounds__(WG) void k_cru_step(RyrKArgs a, int kc)
          * still enters an FP64 compensated sum) */
         double xinaca;
         TF ancx_new;
-        if constexpr (sizeof(TF) != 8) {
+        if (perm) {                                 /* no NCX (:1302) */
+            xinaca = 0.0;
+            ancx_new = ancx[g];
+        } else if constexpr (sizeof(TF) != 8) {
             const float csm = (float)cs_d * 1e-3f;
             const float Ka = (float)ancx[g];
             const float t2 = 669921.875f * csm * (1.0f + csm * (float)(1.0 / 0.00359));
@@ -1110,8 +1158,10 @@ __global__ __launch_bounds__(WG) void k_cru_step(RyrKArgs a, int kc)
         /* cleft/sub-membrane sub-steps (:1356-1473 / :1475-1597); the
          * cytosol/NSR sub-voxels were advanced by k_cyto */
         TF v_casar = casar[g], v_casarh = casarh[g], v_casarj = casarj[g], v_casarhj = casarhj[g];
-        const TF cs_up = (n + sz < C.N) ? cs0[g + sz] : csv;   /* k + 1 < Z */
-        const TF cs_dn = (n >= sz) ? cs0[g - sz] : csv;        /* k > 0 */
+        const TF csb = perm ? (TF)C.ci_bath : csv;             /* perm: bath outside the lattice */
+        const TF cs_up = (n + sz < C.N) ? cs0[g + sz] : csb;   /* k + 1 < Z */
+        const TF cs_dn = (n >= sz) ? cs0[g - sz] : csb;        /* k > 0 */
+        const TF bcgT = (TF)bcgcur;
         const TF icatT = (TF)icat1, xinacaT = (TF)xinaca, xireT = (TF)xire;
         TF csn = csv;
         const TF *sbp = (const TF *)a.sb;
@@ -1121,7 +1171,8 @@ __global__ __launch_bounds__(WG) void k_cru_step(RyrKArgs a, int kc)
             float cpf = (float)cpv, csf = (float)csv;
             float sar = (float)v_casar, sarh = (float)v_casarh, sarj = (float)v_casarj, sarhj = (float)v_casarhj;
             fast_cleft<!SCHB>((float)dt, (float)cil[0], (float)cil[1], (float)csv, (float)cs_up,
-                       (float)cs_dn, (float)icat1, (float)xinaca, (float)xire, cpf, csf, sar, sarh, sarj, sarhj, dci[0], dci[1]);
+                       (float)cs_dn, (float)icat1, (float)xinaca, (float)xire, (float)bcgcur, perm, (float)C.ci_bath,
+                       cpf, csf, sar, sarh, sarj, sarhj, dci[0], dci[1]);
             if (SCHB) {
                 ci1[g] = (TF)fmaxf((float)cil[0] + dci[0] * (float)dt, 0.0f);
                 ci1[P + g] = (TF)fmaxf((float)cil[1] + dci[1] * (float)dt, 0.0f);
@@ -1139,7 +1190,8 @@ __global__ __launch_bounds__(WG) void k_cru_step(RyrKArgs a, int kc)
                 const TF fsar = b_busar<TF>(v_casar, csn, dth);
                 const TF fsarh = b_busarh<TF>(v_casarh, csn, dth);
                 const TF csbuffers = (fsar + fsarh);
-                const TF csdiff = (cs_up + cs_dn - TF(2) * csv) / (TF(1.42));
+                TF csdiff = (cs_up + cs_dn - TF(2) * csv) / (TF(1.42));
+                if (perm) csdiff += (TF(C.ci_bath) - csn) / TF(5);      /* :1409-1411 */
                 const TF dotcs = TF((0.00126) / (0.025)) * (cpv - csn) / (TF(0.0283))
                                  - TF(0.5) * icatT * TF(0.00126) / TF(0.025)
                                  + xinacaT
@@ -1147,7 +1199,7 @@ __global__ __launch_bounds__(WG) void k_cru_step(RyrKArgs a, int kc)
                                  - (csv - cil[1]) / TF(0.04) / TF(2.)
                                  + csdiff
                                  - csbuffers
-                                 - TF(0);
+                                 - bcgT;
                 const TF cpbuff = b_busar<TF>(v_casarj, cpv, dth);
                 const TF cpbuffh = b_busarh<TF>(v_casarhj, cpv, dth);
                 const TF cpdot = ((csn - cpv) / TF(0.0283) +
@@ -1179,14 +1231,15 @@ __global__ __launch_bounds__(WG) void k_cru_step(RyrKArgs a, int kc)
                 const TF fsar = b_busar<TF>(v_casar, csn, dth);
                 const TF fsarh = b_busarh<TF>(v_casarh, csn, dth);
                 const TF csbuffers = (fsar + fsarh);
-                const TF csdiff = (cs_up + cs_dn - TF(2) * csn) / (TF(1.42));
+                TF csdiff = (cs_up + cs_dn - TF(2) * csn) / (TF(1.42));
+                if (perm) csdiff += (TF(C.ci_bath) - csn) / TF(5);      /* :1499-1501 */
                 const TF dotcs = TF((0.00126) / (0.025)) * (cpv - TF(0.5 * 0.0283) * icatT - csn) / (TF(0.0283))
                                  + xinacaT
                                  - (csn - cil[0]) / TF(0.04) / TF(2.)
                                  - (csn - cil[1]) / TF(0.04) / TF(2.)
                                  + csdiff
                                  - csbuffers
-                                 - TF(0);
+                                 - bcgT;
                 const TF cpbuff = b_busar<TF>(v_casarj, cpv, dth);
                 const TF cpbuffh = b_busarh<TF>(v_casarhj, cpv, dth);
                 const TF cpdot = ((csT - cpv) / TF(0.0283) +
@@ -1221,7 +1274,7 @@ __global__ __launch_bounds__(WG) void k_cru_step(RyrKArgs a, int kc)
         cjsrbp[g] = cjsrb_new;
         a.lccw[g] = lw;
         a.ryrw[g] = (uint32_t)nou | ((uint32_t)ncu << 8) | ((uint32_t)nob << 16) | ((uint32_t)ncb << 24);
-        a.aux[g] = (uint32_t)(nl & 0xffff) | ((uint32_t)nspark << 16);
+        a.aux[g] = (uint32_t)(nl & 0xffff) | ((uint32_t)nspark << 16) | (lmask << RYR_AUX_MASK_SHIFT);
         if (diag) {
             a.d_xire[g] = xire;
             a.d_xinaca[g] = xinaca;
@@ -1300,7 +1353,8 @@ RYR_DEV void membrane_block(const RyrKArgs &a, int c, long long nstep, double st
 
     const double stim = stim_of(C, nstep, stim_amp);
     double mo[13], cu[10];
-    membrane_wave(M->m, C.svikr, C.Cm, C.dN, acc[0].hi, acc[1].hi, acc[2].hi, dt, stim, lane, mo, cu);
+    membrane_wave(M->m, C.svikr, C.Cm, C.dN, acc[0].hi, acc[1].hi, acc[2].hi, dt, stim, lane, mo, cu,
+                  (C.variants & RYR_VAR_SLOWNCX) != 0);
     double vnew = mo[0];
     if (C.vclamp_n > 0) {
         /* clamp protocols (cuda_Cell_Detailed.cu:733-773) */
@@ -1422,7 +1476,8 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_diag(RyrKArgs a)
         val[20] = a.d_xire[g];
         ic[0] += nou; ic[1] += nob; ic[2] += ncu; ic[3] += ncb;
         ic[11] += (ax >> 16) & 1u;
-        for (int jj = 0; jj < C.nlcc; ++jj) {
+        for (int jj = 0; jj < 8; ++jj) {
+            if (!((ax >> (RYR_AUX_MASK_SHIFT + jj)) & 1u)) continue;   /* absent slot */
             const int st = (lw >> (4 * jj)) & 15;
             /* histogram of cuda_Cell_Detailed.cu:904-923 */
             if (st == 0) { ic[10]++; ic[9]++; continue; }
@@ -1584,9 +1639,19 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_init(RyrKArgs a, const RyrInit *_
     ((TF *)a.casarh)[g] = (TF)I.casarh0;
     ((TF *)a.casarj)[g] = (TF)I.casar0;
     ((TF *)a.casarhj)[g] = (TF)I.casarh0;
-    a.lccw[g] = I.lcc0;
+    int nlc = C.nlcc;
+    if (C.variants & RYR_VAR_RANDOMLCC) {
+        /* randpoint = curand % 9 (:1150-1152): one 32-bit draw of the CRU's
+         * stream at counter step 2^64-1 */
+        ryr_stream s0;
+        ryr_stream_init(&s0, C.seed, (uint32_t)n, ~(uint64_t)0);
+        nlc = (int)(ryr_stream_u32(&s0) % 9u);
+    }
+    uint32_t lw = 0;
+    for (int ll = 0; ll < nlc; ++ll) lw |= 3u << (4 * ll);
+    a.lccw[g] = lw;
     a.ryrw[g] = (uint32_t)0 | ((uint32_t)I.ncu0 << 8) | ((uint32_t)0 << 16) | ((uint32_t)I.ncb0 << 24);
-    a.aux[g] = 0u;
+    a.aux[g] = ((1u << nlc) - 1u) << RYR_AUX_MASK_SHIFT;
     a.d_xire[g] = 0.0; a.d_xinaca[g] = 0.0;
 }
 
@@ -1665,9 +1730,15 @@ RYR_DEV void field_set(const RyrKArgs &a, int f, int cur, long long e, long long
     case 18: a.ryrw[g] = (a.ryrw[g] & ~(255u << 8)) | (((uint32_t)v & 255u) << 8); break;
     case 19: a.ryrw[g] = (a.ryrw[g] & ~(255u << 16)) | (((uint32_t)v & 255u) << 16); break;
     case 20: a.ryrw[g] = (a.ryrw[g] & ~(255u << 24)) | (((uint32_t)v & 255u) << 24); break;
-    case 21: a.lccw[g] = (a.lccw[g] & ~(15u << (4 * p))) | (((uint32_t)v & 15u) << (4 * p)); break;
+    case 21:   /* 16 = absent slot */
+        if (v >= 16.0) a.aux[g] &= ~(1u << (RYR_AUX_MASK_SHIFT + p));
+        else {
+            a.aux[g] |= 1u << (RYR_AUX_MASK_SHIFT + p);
+            a.lccw[g] = (a.lccw[g] & ~(15u << (4 * p))) | (((uint32_t)v & 15u) << (4 * p));
+        }
+        break;
     case 22: a.aux[g] = (a.aux[g] & ~0xffffu) | ((uint32_t)v & 0xffffu); break;
-    case 23: a.aux[g] = (a.aux[g] & 0xffffu) | (((uint32_t)v & 1u) << 16); break;
+    case 23: a.aux[g] = (a.aux[g] & ~(1u << 16)) | (((uint32_t)v & 1u) << 16); break;
     }
 }
 
@@ -1683,7 +1754,8 @@ __global__ void k_export(RyrKArgs a, int cell, int field, int planes, double *ou
     const long long g = C.off + n;
     const int cur = (int)(a.mem[cell].step & 1);
     /* slots past the cell's channel count are absent: state 16 (:1157-1162) */
-    out[e] = (field == 21 && p >= C.nlcc) ? 16.0 : field_get<TF, TS>(a, field, cur, (long long)p * a.P + g, g, p);
+    const bool absent = field == 21 && !((a.aux[g] >> (RYR_AUX_MASK_SHIFT + p)) & 1u);
+    out[e] = absent ? 16.0 : field_get<TF, TS>(a, field, cur, (long long)p * a.P + g, g, p);
 }
 
 template <typename TF, typename TS>

@@ -88,6 +88,7 @@ void ryr_params_default(ryr_params *p)
     p->v0 = -80.0;
     p->cm_mode = 0;
     p->cm = 45.0;
+    p->variants = 0;
 }
 
 void ryr_cell_desc_default(ryr_cell_desc *d)
@@ -169,6 +170,10 @@ static int validate(const ryr_params *p, const ryr_cell_desc *cells, int ncells,
     if (p->nlcc < 0 || p->nlcc > 8) { msg = "nlcc must be in [0, 8]"; return RYR_E_ARG; }
     if (p->steps_per_output < 1) { msg = "steps_per_output must be >= 1"; return RYR_E_ARG; }
     if (!(p->cm > 0.0)) { msg = "cm must be > 0"; return RYR_E_ARG; }
+    if (p->variants & ~(RYR_VAR_PERM | RYR_VAR_ISO | RYR_VAR_SLOWNCX | RYR_VAR_RANDOMLCC)) {
+        msg = "unknown bits in variants";
+        return RYR_E_ARG;
+    }
     for (int c = 0; c < ncells; ++c) {
         const ryr_cell_desc &d = cells[c];
         if (d.nx < 1 || d.ny < 1 || d.nz < 1) { msg = "cell " + std::to_string(c) + ": lattice dims must be >= 1"; return RYR_E_ARG; }
@@ -249,6 +254,18 @@ extern "C" int ryr_create(int device, const ryr_params *p, const ryr_cell_desc *
         I.noff = (long long)std::llround(p->stim_offset / p->dt);
         I.ndur = (long long)std::llround(p->stim_dur / p->dt);
         I.nlcc = p->nlcc;
+        /* variants (the reference's -D builds) */
+        I.variants = p->variants;
+        I.nstim_end = -1;
+        I.iso_n = (11 * I.ncl + 1) / 2;                 /* t >= 5.5 CL (:1220) */
+        I.sviup_iso = I.sviup_tot;
+        if (p->variants & RYR_VAR_ISO) {
+            I.noff = 0;                                 /* fmod(t, CL) < 1 (:684) */
+            I.nstim_end = (119 * I.ncl + 9) / 10;       /* t < 11.9 CL */
+            if (!p->iso2 && !p->tapsi) I.sviup_iso = (1.0 + 0.75) * 2. + sviup_lqt;
+        }
+        if (p->variants & RYR_VAR_PERM) I.nstim_end = 0;    /* stim = 0 (:697-699) */
+        I.ci_bath = p->ci0;
         int jy = (d.ny + 2) / 2;            /* ny/2 with the ghost shell (:973) */
         int jz = 5 < d.nz ? 5 : d.nz;        /* :972, clamped to the interior */
         I.line_j = jy - 1;

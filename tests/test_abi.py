@@ -46,8 +46,8 @@ def test_struct_layout_and_defaults_match_the_header():
     e = CellDesc.default()
     for f, _ in CellDesc._fields_:
         assert getattr(d, f) == getattr(e, f), f
-    assert ctypes.sizeof(Params) == 96 and ctypes.sizeof(CellDesc) == 88
-    assert lib.ryr_abi_version() == 2
+    assert ctypes.sizeof(Params) == 104 and ctypes.sizeof(CellDesc) == 88
+    assert lib.ryr_abi_version() == 3
 
 
 def _create(p, cells):
