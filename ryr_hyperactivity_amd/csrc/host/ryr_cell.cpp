@@ -20,6 +20,10 @@
  *   --seed S           Philox seed (default 1)
  *   --ikr-scale X      I_Kr multiplier, svikr = 3X when not --lqt (default 1)
  *   --lqt / --no-lqt   (default on)   --iso2 / --no-iso2 (default on)   --tapsi
+ *   --perm             permeabilised cell; also ci_basal 0.3 and cjsr_basal 900 as
+ *                      the reference's perm build (:87-89)
+ *   --iso / --slowncxchange / --randomlcc   the other -D builds (ryr_cru.h RYR_VAR_*)
+ *   --nopacing         no stimulus (:694-696)
  *   --precision f64|f32   --scheme A|B
  *   --ic reference|baseline   ICs ci0/V0 0.0944/-80 (reference) or 0.1/-86 with cjsr0 = 750
  *   --cm X / --cm-fixed       Cm per 16120 CRUs (density-preserving) or a fixed Cm
@@ -83,6 +87,11 @@ int main(int argc, char **argv)
         else if (f == "--iso2") p.iso2 = 1;
         else if (f == "--no-iso2") p.iso2 = 0;
         else if (f == "--tapsi") p.tapsi = 1;
+        else if (f == "--perm") { p.variants |= RYR_VAR_PERM; p.ci0 = 0.3; d.cjsr0 = 900.0; }
+        else if (f == "--iso") p.variants |= RYR_VAR_ISO;
+        else if (f == "--slowncxchange") p.variants |= RYR_VAR_SLOWNCX;
+        else if (f == "--randomlcc") p.variants |= RYR_VAR_RANDOMLCC;
+        else if (f == "--nopacing") p.stim_amp = 0.0;
         else if (f == "--precision") {
             std::string v = next();
             p.precision = (v == "f32") ? RYR_PREC_F32 : RYR_PREC_F64;

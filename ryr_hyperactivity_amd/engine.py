@@ -267,6 +267,13 @@ def parse_argv(argv):
         elif f == "--iso2": p.iso2 = 1
         elif f == "--no-iso2": p.iso2 = 0
         elif f == "--tapsi": p.tapsi = 1
+        elif f == "--perm":
+            p.variants |= _abi.VAR_PERM
+            p.ci0, d.cjsr0 = 0.3, 900.0
+        elif f == "--iso": p.variants |= _abi.VAR_ISO
+        elif f == "--slowncxchange": p.variants |= _abi.VAR_SLOWNCX
+        elif f == "--randomlcc": p.variants |= _abi.VAR_RANDOMLCC
+        elif f == "--nopacing": p.stim_amp = 0.0
         elif f == "--precision": p.precision = _abi.PREC_F32 if next(it) == "f32" else _abi.PREC_F64
         elif f == "--scheme": p.scheme = _abi.SCHEME_B if next(it).upper() == "B" else _abi.SCHEME_A
         elif f == "--ic":

@@ -100,3 +100,15 @@ def test_argv_parser_matches_run_sh():
     assert (d.nx, d.ny, d.nz, d.cl, d.Ku2) == (100, 20, 10, 2000.0, 2.5)
     assert nsteps == 800000 and p.precision == 1 and p.scheme == 1
     assert (p.ci0, p.v0, d.cjsr0) == (0.1, -86.0, 750.0)
+
+
+def test_argv_variant_flags_match_the_makefile_builds():
+    from ryr_hyperactivity_amd.engine import parse_argv
+    from ryr_hyperactivity_amd._abi import VAR_PERM, VAR_ISO, VAR_SLOWNCX, VAR_RANDOMLCC
+    base = ["0", "x", "1.0", "740", "1.0", "10", "0.00", "1.", "0.7", "1"]
+    p, d, _, _, _ = parse_argv(base + ["--perm"])
+    assert p.variants == VAR_PERM and (p.ci0, d.cjsr0) == (0.3, 900.0)          # :87-89
+    p, _, _, _, _ = parse_argv(base + ["--iso", "--slowncxchange", "--randomlcc", "--nopacing"])
+    assert p.variants == VAR_ISO | VAR_SLOWNCX | VAR_RANDOMLCC and p.stim_amp == 0.0
+    with pytest.raises(ValueError):
+        parse_argv(base + ["--nosuchbuild"])
