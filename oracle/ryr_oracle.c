@@ -337,7 +337,28 @@ static void orc_slot(orc_rng *g, uint32_t *a, uint32_t *b)
     g->used++;
 }
 static double orc_u(orc_rng *g) { uint32_t a, b; orc_slot(g, &a, &b); return ryr_slot_uniform(a, b); }
-static uint32_t orc_i(orc_rng *g) { uint32_t a, b; orc_slot(g, &a, &b); return b; }
+
+/* Draw d (0, 1, ...) of RyR transition t under the addressing of philox.h:
+ * d = 0 is static slot 8 + t, d >= 1 is slot d - 1 of the transition's own
+ * substream.  A replay hands out its slots in consumption order instead. */
+static void orc_tslot(orc_rng *g, int t, uint32_t d, uint32_t *a, uint32_t *b)
+{
+    if (g->replay) {
+        orc_slot(g, a, b);
+        return;
+    }
+    const uint64_t seed = (uint64_t)g->ph.k0 | ((uint64_t)g->ph.k1 << 32);
+    const uint64_t step = (uint64_t)g->ph.step_lo | ((uint64_t)g->ph.step_hi << 32);
+    uint32_t c0, half;
+    if (d == 0) { c0 = (uint32_t)(RYR_SLOT_RYR0 + t) >> 1; half = (uint32_t)(RYR_SLOT_RYR0 + t) & 1u; }
+    else { c0 = ryr_sub_block(t, d - 1); half = (d - 1) & 1u; }
+    ryr_u32x4 o = ryr_block(seed, g->ph.cru, step, c0);
+    *a = o.v[2 * half];
+    *b = o.v[2 * half + 1];
+    g->used++;
+}
+static double orc_tu(orc_rng *g, int t, uint32_t d) { uint32_t a, b; orc_tslot(g, t, d, &a, &b); return ryr_slot_uniform(a, b); }
+static uint32_t orc_ti(orc_rng *g, int t, uint32_t d) { uint32_t a, b; orc_tslot(g, t, d, &a, &b); return b; }
 
 /* ---- reesica.h:14-116, with the uniform supplied ------------------------ */
 int orc_lcc_markov(int i, double cpx, double v, double dt, double ragg, int iso2)
@@ -372,28 +393,30 @@ int orc_lcc_markov(int i, double cpx, double v, double dt, double ragg, int iso2
 
 /* ---- subroutine_cuda.c:251-669 ------------------------------------------- */
 
-/* One of the eight population transitions: Knuth's product-of-uniforms
- * Poisson sampler (:336-347) or, outside its validity range, a dithered
- * Gaussian with rejection of negative values (:348-363). */
-static int orc_transition(orc_rng *g, int n, double p, int *fails)
+/* Transition t of the eight population transitions: Knuth's
+ * product-of-uniforms Poisson sampler (:336-347) or, outside its validity
+ * range, a dithered Gaussian with rejection of negative values (:348-363).
+ * Its uniforms come from its own draw sequence (orc_tslot). */
+static int orc_transition(orc_rng *g, int t, int n, double p, int *fails)
 {
     double lam = ryr_exp(-(n) * p);
     int out = -1;
+    uint32_t d = 0;
     if (n <= 1 || p < 0.2 || (n <= 5 && p < 0.3)) {
         int kk = 0;
         double puu = 1.0;
         while (puu >= lam && kk < 195) {
             kk++;
-            double re = orc_u(g);
+            double re = orc_tu(g, t, d++);
             puu = puu * (re);
         }
         out = kk - 1;
     } else {
         int kk = 0;
         while (out < 0) {
-            double u1 = orc_u(g);
-            double u2 = orc_u(g);
-            uint32_t r = orc_i(g);
+            double u1 = orc_tu(g, t, d++);
+            double u2 = orc_tu(g, t, d++);
+            uint32_t r = orc_ti(g, t, d++);
             out = (int)(floor(n * p + sqrt(n * p * (1.0 - p)) * sqrt(-2.0 * ryr_log(1.0 - u1))
                               * ryr_cos(2.0 * O_PI * u2)) + r % 2);
             kk++;
@@ -424,34 +447,34 @@ static void orc_ryrgating(orc_rng *g, double Ku, double Ku2, double Kb, double c
 
     /* OU -> CU (:332-364) */
     double pkuh = pkuminus * dt;
-    int n_ou_cu = orc_transition(g, *nou, pkuh, fails);
+    int n_ou_cu = orc_transition(g, 0, *nou, pkuh, fails);
     if (n_ou_cu > O_NRYR) n_ou_cu = O_NRYR;
     /* CU -> OU (:366-397) */
     double pkum = pku * dt;
-    int n_cu_ou = orc_transition(g, *ncu, pkum, fails);
+    int n_cu_ou = orc_transition(g, 1, *ncu, pkum, fails);
     if (n_cu_ou > O_NRYR) n_cu_ou = O_NRYR;
     /* OU -> OB (:401-436) */
     double pau = (pkb < 1e-16) ? 0 : aub * dt;
-    int n_ou_ob = orc_transition(g, *nou, pau, fails);
+    int n_ou_ob = orc_transition(g, 2, *nou, pau, fails);
     if (n_ou_ob > O_NRYR) n_ou_ob = O_NRYR;
     /* OB -> OU (:439-476) */
     double pbu = (pkb < 1e-16) ? abu * dt * 36 : abu * dt * (pku / pkb);
-    int n_ob_ou = orc_transition(g, *nob, pbu, fails);
+    int n_ob_ou = orc_transition(g, 3, *nob, pbu, fails);
     if (n_ob_ou > O_NRYR) n_ob_ou = O_NRYR;
     /* CB -> CU (:480-511) */
     double pcb = abu * dt;
-    int n_cb_cu = orc_transition(g, *ncb, pcb, fails);
+    int n_cb_cu = orc_transition(g, 4, *ncb, pcb, fails);
     if (n_cb_cu > O_NRYR) n_cb_cu = O_NRYR;
     /* CU -> CB (:513-544) */
     double pcub = aub * dt;
-    int n_cu_cb = orc_transition(g, *ncu, pcub, fails);
+    int n_cu_cb = orc_transition(g, 5, *ncu, pcub, fails);
     if (n_cu_cb > O_NRYR) n_cu_cb = O_NRYR;
     /* OB -> CB (:548-579), no cap */
     double pkbh = pkbminus * dt;
-    int n_ob_cb = orc_transition(g, *nob, pkbh, fails);
+    int n_ob_cb = orc_transition(g, 6, *nob, pkbh, fails);
     /* CB -> OB (:581-613) */
     double pkbm = pkb * dt;
-    int n_cb_ob = orc_transition(g, *ncb, pkbm, fails);
+    int n_cb_ob = orc_transition(g, 7, *ncb, pkbm, fails);
     if (n_cb_ob > O_NRYR) n_cb_ob = O_NRYR;
 
     /* outflow conflicts (:617-639) */

@@ -96,6 +96,34 @@ RYR_HD void ryr_stream_next_slot(ryr_stream *s, uint32_t *a, uint32_t *b)
     }
 }
 
+/* Draw addressing of the CRU update (one CRU-step; key = cell seed, counter =
+ * (c0, CRU index, step lo, step hi); a block's words 0,1 form its first slot,
+ * words 2,3 its second):
+ *   - static slots 0..15, blocks c0 = 0..7 (slot s: block s >> 1, half s & 1):
+ *     LCC channel l draws slot l; RyR transition t (0..7 in the order of the
+ *     reference's ryrgating) draws its first uniform from slot 8 + t;
+ *   - the further draws of transition t (the Knuth loop's uniforms, the
+ *     Gaussian tries) are slots j = 0, 1, ... of its own substream, blocks
+ *     c0 = ((t + 1) << RYR_SUB_SHIFT) | (j >> 1).
+ * Each draw's address is a function of (seed, CRU, step, site, j) alone, so
+ * the eight transitions are independent of each other's draw counts and can
+ * be sampled in any order.  The LCC slots coincide with the first slots of
+ * the sequential stream above. */
+#define RYR_SUB_SHIFT 24
+#define RYR_SLOT_RYR0 8
+
+RYR_HD ryr_u32x4 ryr_block(uint64_t seed, uint32_t cru, uint64_t step, uint32_t c0)
+{
+    ryr_u32x4 c;
+    c.v[0] = c0;
+    c.v[1] = cru;
+    c.v[2] = (uint32_t)step;
+    c.v[3] = (uint32_t)(step >> 32);
+    return ryr_philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+}
+
+RYR_HD uint32_t ryr_sub_block(int t, uint32_t j) { return ((uint32_t)(t + 1) << RYR_SUB_SHIFT) | (j >> 1); }
+
 /* One uniform on (0,1) (stands where curand_uniform_double was called). */
 RYR_HD double ryr_stream_uniform(ryr_stream *s)
 {

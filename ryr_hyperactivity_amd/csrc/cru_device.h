@@ -154,18 +154,53 @@ RYR_DEV RyrDD dd_add(RyrDD a, RyrDD b)
     return r;
 }
 
-/* Pairwise tree over the 64 lanes of a wave in index order (xor 1,2,...,32);
- * dd_add is commutative bit for bit, so every lane ends with the same value. */
+/* A double moved across lanes by a DPP control (a VALU move, no LDS trip). */
+template <int CTRL>
+RYR_DEV double dpp_f64(double x)
+{
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, true);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+/* Lane `src` (wave-uniform) of x, broadcast through a scalar register. */
+RYR_DEV double lanev(double x, int src)
+{
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, src);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), src);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+template <int CTRL>
+RYR_DEV RyrDD dd_dpp_add(RyrDD v)
+{
+    RyrDD w;
+    w.hi = dpp_f64<CTRL>(v.hi);
+    w.lo = dpp_f64<CTRL>(v.lo);
+    return dd_add(v, w);
+}
+
+/* Pairwise tree over the 64 lanes of a wave in index order: level k joins
+ * the aligned blocks of 2^(k-1) lanes pairwise.  After a level every lane of
+ * a block holds the block's sum, so any lane of the partner block serves as
+ * the partner: quad permutations for blocks of 1 and 2, the half-row mirror
+ * (lane ^ 7) for 4, the row mirror (lane ^ 15) for 8, and for the last two
+ * levels the four 16-lane sums read through scalar registers.  dd_add is
+ * commutative bit for bit, so every lane ends with the same value. */
 RYR_DEV RyrDD wave_tree(RyrDD v)
 {
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        RyrDD w;
-        w.hi = __shfl_xor(v.hi, o, 64);
-        w.lo = __shfl_xor(v.lo, o, 64);
-        v = dd_add(v, w);
-    }
-    return v;
+    v = dd_dpp_add<0xB1>(v);     /* quad_perm [1,0,3,2]: lane ^ 1 */
+    v = dd_dpp_add<0x4E>(v);     /* quad_perm [2,3,0,1]: lane ^ 2 */
+    v = dd_dpp_add<0x141>(v);    /* row_half_mirror */
+    v = dd_dpp_add<0x140>(v);    /* row_mirror */
+    RyrDD g0, g1, g2, g3;
+    g0.hi = lanev(v.hi, 0);  g0.lo = lanev(v.lo, 0);
+    g1.hi = lanev(v.hi, 16); g1.lo = lanev(v.lo, 16);
+    g2.hi = lanev(v.hi, 32); g2.lo = lanev(v.lo, 32);
+    g3.hi = lanev(v.hi, 48); g3.lo = lanev(v.lo, 48);
+    return dd_add(dd_add(g0, g1), dd_add(g2, g3));
 }
 
 /* ------------------------------------------------------------- buffers.c */
@@ -273,44 +308,52 @@ RYR_DEV int lcc_step(int i, double cpx, double dt, double ragg, const RyrMem &m,
     return i;
 }
 
-/* Knuth product-of-uniforms Poisson count, or the dithered Gaussian with
- * rejection outside Knuth's range (subroutine_cuda.c:332-363 and its seven
- * copies). */
-RYR_DEV int ryr_transition(ryr_stream *s, int n, double p, int *fails)
+/* Transition t's count once its first uniform u0 (static slot 8 + t) has
+ * failed the early-out: Knuth's product-of-uniforms Poisson count, or the
+ * dithered Gaussian with rejection outside Knuth's range
+ * (subroutine_cuda.c:332-363 and its seven copies).  The further uniforms
+ * come from t's own substream; *nd counts them. */
+RYR_DEV int ryr_transition_job(uint64_t seed, uint32_t cru, uint64_t step, int t, int n, double p, double u0,
+                               int *fails, int *nd)
 {
     RYR_STRICT_FP
     int out = -1;
+    uint32_t j = 0;
+    ryr_u32x4 w = {{0u, 0u, 0u, 0u}};
     if (n <= 1 || p < 0.2 || (n <= 5 && p < 0.3)) {
-        /* First draw of the product loop.  exp(-x) > 1 - x, and for x >= 2^-40
-         * the rounded 1 - 2x lies below the rounded exp(-x) by far more than
-         * the rounding of either, so a draw under 1 - 2x ends the loop with
-         * k = 0 exactly as the comparison with exp(-x) would: the exponential
-         * is only evaluated when the draw is within about 2x of 1. */
-        const double x = (double)(n) * p;
-        double puu = ryr_stream_uniform(s);
-        if (x >= 0x1p-40 && puu < 1.0 - 2.0 * x) return 0;
-        /* exp(-0) is exactly 1: skip the call for empty populations or zero rates */
-        const double lam = (n == 0 || p == 0.0) ? 1.0 : ryr_exp(-(n) * p);
+        const double lam = ryr_exp(-(n) * p);
         int kk = 1;
+        double puu = u0;
         while (puu >= lam && kk < 195) {
             kk++;
-            double re = ryr_stream_uniform(s);
+            if (!(j & 1u)) w = ryr_block(seed, cru, step, ryr_sub_block(t, j));
+            const double re = (j & 1u) ? ryr_slot_uniform(w.v[2], w.v[3]) : ryr_slot_uniform(w.v[0], w.v[1]);
+            ++j;
             puu = puu * (re);
         }
         out = kk - 1;
     } else {
         int kk = 0;
         while (out < 0) {
-            double u1 = ryr_stream_uniform(s);
-            double u2 = ryr_stream_uniform(s);
-            uint32_t r = ryr_stream_u32(s);
+            double u[3];
+            uint32_t r = 0;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                if (q == 0 && kk == 0) { u[0] = u0; continue; }
+                if (!(j & 1u)) w = ryr_block(seed, cru, step, ryr_sub_block(t, j));
+                const uint32_t wa = (j & 1u) ? w.v[2] : w.v[0], wb = (j & 1u) ? w.v[3] : w.v[1];
+                ++j;
+                u[q] = ryr_slot_uniform(wa, wb);
+                r = wb;
+            }
             out = (int)(__builtin_floor(n * p + __builtin_sqrt(n * p * (1.0 - p)) *
-                                        __builtin_sqrt(-2.0 * ryr_log(1.0 - u1)) *
-                                        ryr_cos(2.0 * 3.1415926 * u2)) + r % 2);
+                                        __builtin_sqrt(-2.0 * ryr_log(1.0 - u[0])) *
+                                        ryr_cos(2.0 * 3.1415926 * u[1])) + r % 2);
             kk++;
             if (kk > 200) { out = 0; (*fails)++; }
         }
     }
+    *nd += (int)j;
     return out;
 }
 
@@ -326,9 +369,21 @@ RYR_DEV double csqn_mono(double cjsr, double *roo2_out)
 }
 
 /* subroutine_cuda.c:251-669; counts in/out: nou, ncu, nob, ncb; mono =
- * csqn_mono(cjsr) */
-RYR_DEV void ryr_gating(ryr_stream *s, double Ku, double Ku2, double Kb, double cp, double cjsr,
-                        double mono, double dt, int &nou, int &ncu, int &nob, int &ncb, int *fails)
+ * csqn_mono(cjsr); *nd counts the draws.
+ *
+ * The eight transitions draw from their own addresses (philox.h), so they are
+ * sampled in two passes rather than in program order.  Pass 1 draws every
+ * first uniform (static slots 8..15, four Philox blocks for all lanes at
+ * once) and settles the transitions whose count is 0 without the
+ * exponential: an empty population or zero rate (exp(-0) = 1), or a first
+ * draw below 1 - x, x = n p (1 - 2x below x = 2^-20), which lies under the
+ * rounded exp(-x) by more than the rounding of either.  Pass 2 is a loop in
+ * which each lane takes its next unsettled transition, so one exponential and
+ * one Knuth loop per round serve every lane, whichever transitions they
+ * have left.  The counts equal those of the sequential reference loop. */
+RYR_DEV void ryr_gating(uint64_t seed, uint32_t cru, uint64_t step, double Ku, double Ku2, double Kb, double cp,
+                        double cjsr, double mono, double dt, int &nou, int &ncu, int &nob, int &ncb, int *fails,
+                        int *nd)
 {
     RYR_STRICT_FP
     double q1 = (5000 / cjsr) * (5000 / cjsr);
@@ -343,24 +398,63 @@ RYR_DEV void ryr_gating(ryr_stream *s, double Ku, double Ku2, double Kb, double 
     if (pkbminus * dt > 1.0) pkbminus = 1.0 / dt;
     double aub = mono / (1.0);
     double abu = 1.0 / (0.2 * 5 * 220.);
+    const double pau = (pkb < 1e-16) ? 0 : aub * dt;
+    const double pbu = (pkb < 1e-16) ? abu * dt * 36 : abu * dt * (pku / pkb);
 
-    int n_ou_cu = ryr_transition(s, nou, pkuminus * dt, fails);
-    if (n_ou_cu > RYR_NRYR) n_ou_cu = RYR_NRYR;
-    int n_cu_ou = ryr_transition(s, ncu, pku * dt, fails);
-    if (n_cu_ou > RYR_NRYR) n_cu_ou = RYR_NRYR;
-    double pau = (pkb < 1e-16) ? 0 : aub * dt;
-    int n_ou_ob = ryr_transition(s, nou, pau, fails);
-    if (n_ou_ob > RYR_NRYR) n_ou_ob = RYR_NRYR;
-    double pbu = (pkb < 1e-16) ? abu * dt * 36 : abu * dt * (pku / pkb);
-    int n_ob_ou = ryr_transition(s, nob, pbu, fails);
-    if (n_ob_ou > RYR_NRYR) n_ob_ou = RYR_NRYR;
-    int n_cb_cu = ryr_transition(s, ncb, abu * dt, fails);
-    if (n_cb_cu > RYR_NRYR) n_cb_cu = RYR_NRYR;
-    int n_cu_cb = ryr_transition(s, ncu, aub * dt, fails);
-    if (n_cu_cb > RYR_NRYR) n_cu_cb = RYR_NRYR;
-    int n_ob_cb = ryr_transition(s, nob, pkbminus * dt, fails);
-    int n_cb_ob = ryr_transition(s, ncb, pkb * dt, fails);
-    if (n_cb_ob > RYR_NRYR) n_cb_ob = RYR_NRYR;
+    /* OU->CU, CU->OU, OU->OB, OB->OU, CB->CU, CU->CB, OB->CB, CB->OB */
+    const double pt[8] = {pkuminus * dt, pku * dt, pau, pbu, abu * dt, aub * dt, pkbminus * dt, pkb * dt};
+    /* source population of transition t: 0 OU, 1 CU, 2 OB, 3 CB */
+    const int pop[4] = {nou, ncu, nob, ncb};
+    constexpr int src[8] = {0, 1, 0, 2, 3, 1, 2, 3};
+    double u0[8];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const ryr_u32x4 w = ryr_block(seed, cru, step, (uint32_t)(RYR_SLOT_RYR0 / 2 + b));
+        u0[2 * b] = ryr_slot_uniform(w.v[0], w.v[1]);
+        u0[2 * b + 1] = ryr_slot_uniform(w.v[2], w.v[3]);
+    }
+    uint32_t pend = 0u;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        const int n = pop[src[t]];
+        const double p = pt[t];
+        if (n <= 1 || p < 0.2 || (n <= 5 && p < 0.3)) {
+            const double x = (double)(n) * p;
+            const double thr = (x >= 0x1p-20) ? 1.0 - x : 1.0 - 2.0 * x;
+            const bool zero = (n == 0 || p == 0.0) || (x >= 0x1p-40 && u0[t] < thr);
+            if (!zero) pend |= 1u << t;
+        } else {
+            pend |= 1u << t;
+        }
+    }
+    /* counts in byte fields (Knuth counts stay below 195 and Gaussian ones
+     * below n p + 5 sqrt(-2 log 2^-53) + 1 < 150) */
+    uint32_t klo = 0u, khi = 0u;
+    int ndraw = 8;
+    while (pend) {
+        const int t = __builtin_ctz(pend);
+        pend &= pend - 1u;
+        int n = pop[0];
+        double p = pt[0], u = u0[0];
+#pragma unroll
+        for (int q = 1; q < 8; ++q)
+            if (t == q) { n = pop[src[q]]; p = pt[q]; u = u0[q]; }
+        const uint32_t k = (uint32_t)ryr_transition_job(seed, cru, step, t, n, p, u, fails, &ndraw) & 255u;
+        if (t < 4) klo |= k << (8 * t);
+        else khi |= k << (8 * (t - 4));
+    }
+    *nd += ndraw;
+
+#define RYR_KT(t) ((int)((((t) < 4 ? klo : khi) >> (8 * ((t) & 3))) & 255u))
+    int n_ou_cu = RYR_KT(0) > RYR_NRYR ? RYR_NRYR : RYR_KT(0);
+    int n_cu_ou = RYR_KT(1) > RYR_NRYR ? RYR_NRYR : RYR_KT(1);
+    int n_ou_ob = RYR_KT(2) > RYR_NRYR ? RYR_NRYR : RYR_KT(2);
+    int n_ob_ou = RYR_KT(3) > RYR_NRYR ? RYR_NRYR : RYR_KT(3);
+    int n_cb_cu = RYR_KT(4) > RYR_NRYR ? RYR_NRYR : RYR_KT(4);
+    int n_cu_cb = RYR_KT(5) > RYR_NRYR ? RYR_NRYR : RYR_KT(5);
+    int n_ob_cb = RYR_KT(6);
+    int n_cb_ob = RYR_KT(7) > RYR_NRYR ? RYR_NRYR : RYR_KT(7);
+#undef RYR_KT
 
     if (n_ou_ob + n_ou_cu > nou) {
         if (n_ou_cu >= n_ou_ob) n_ou_cu = 0; else n_ou_ob = 0;
@@ -387,41 +481,60 @@ RYR_DEV void ryr_gating(ryr_stream *s, double Ku, double Ku2, double Kb, double 
     ncb = ncb < 0 ? 0 : (ncb > RYR_NRYR ? RYR_NRYR : ncb);
 }
 
-/* ------------------------------------------ membrane ODE (FP64, one wave) */
+/* ------------------------------------ membrane ODE (FP64, up to four waves) */
 /* cuda_Cell_Detailed.cu:447-739: whole-cell currents from the three sums,
- * gates by Rush-Larsen, [Na]i and V by forward Euler.  One wavefront
- * evaluates it: the exponentials and logarithms whose arguments depend only
- * on V, [Na]i and constants are spread over the lanes and computed in one
- * SIMD pass, the Rush-Larsen factors exp(-dt/tau) in a second pass; every
- * lane then repeats the same scalar arithmetic on the broadcast values.  Each
- * argument is the reference's expression and the arithmetic keeps its
- * operand order, so the result does not depend on which lane computed what.
- * out[0..12]: new v xm xh xj xr xs1 xs2 qks xtos ytos xtof ytof xnai;
- * cur[0..9]: csubt xicatto xinacato xina xiks xikr xik1 xinak xitos xitof */
-RYR_DEV double lanev(double x, int src) { return __shfl(x, src, 64); }
+ * gates by Rush-Larsen, [Na]i and V by forward Euler.
+ *
+ * The currents fall into four independent groups: I_Na; I_Kr and I_Ks; I_K1
+ * and I_to; I_NaK with the scaling of the Ca-current sums.  Group g runs on
+ * wavefront g mod NW of the membrane workgroup, so with four waves the four
+ * groups run side by side on the four SIMDs.  Inside a group the work is
+ * spread over the lanes by dependency level: the divisions of one level (the
+ * exponent arguments, then the rates, then the time constants and steady
+ * states, then -dt/tau) are one division per lane, each lane picking its own
+ * numerator and denominator, and the exponentials of one level are one ryr_exp
+ * per lane.  Values are broadcast through scalar registers.  Each expression
+ * is the reference's, in its operand order, so no result depends on which
+ * wave or lane computed it.  Each group leaves its new gates and currents in
+ * shared slots RYR_MS_*; every wave then forms [Na]i and V from them. */
+#define RYR_MS_N 24
+enum {
+    RYR_MS_V = 0, RYR_MS_XM, RYR_MS_XH, RYR_MS_XJ, RYR_MS_XR, RYR_MS_XS1, RYR_MS_XS2, RYR_MS_QKS,
+    RYR_MS_XTOS, RYR_MS_YTOS, RYR_MS_XTOF, RYR_MS_YTOF, RYR_MS_NAI,
+    RYR_MS_CSUBT, RYR_MS_ICAT, RYR_MS_INACA, RYR_MS_INA, RYR_MS_IKS, RYR_MS_IKR, RYR_MS_IK1, RYR_MS_INAK,
+    RYR_MS_ITOS, RYR_MS_ITOF
+};
 
-RYR_DEV void membrane_wave(const double *s, double svikr, double Cm, double dN, double sum_icat,
-                           double sum_inaca, double sum_cs, double dt, double stim, int lane, double *out,
-                           double *cur, bool slowncx)
+struct MemIn {
+    const double *s;      /* v xm xh xj xr xs1 xs2 qks xtos ytos xtof ytof xnai */
+    double svikr, Cm, dN, sum_icat, sum_inaca, sum_cs, dt;
+    bool slowncx;
+};
+
+#define RYR_XKI 140.0
+#define RYR_XKO 5.40
+#define RYR_FRT (96.485 / 8.314 / 308.0)
+
+/* I_Na (:460-502): m, h, j */
+RYR_DEV void mem_ina(const MemIn &I, int lane, double *sh)
 {
     RYR_STRICT_FP
-    double v = s[0], xm = s[1], xh = s[2], xj = s[3], xr = s[4], xs1 = s[5], xs2 = s[6];
-    double qks = s[7], xtos = s[8], ytos = s[9], xtof = s[10], ytof = s[11], xnai = s[12];
-    const double xki = 140.0, xko = 5.40;
-    const double frt = (96.485 / 8.314 / 308.0);
-    const double ek = (1.00 / frt) * ryr_log(xko / xki);
-    const double rt1 = -(v + 3.0) / 15.00;
-    const double rt2 = (v + 33.5) / 10.00;
-    const double rt3 = (v + 60.00) / 10.00;
-    const double rt4 = -(v / 30.00) * (v / 30.00);
-    const double rt5 = (v + 33.50) / 10.00;
-
-    /* pass 1: exponentials of V, [Na]i and constants; logs of ENa and EKs */
+    const double v = I.s[0], xm = I.s[1], xh = I.s[2], xj = I.s[3], xnai = I.s[12], dt = I.dt;
+    const double frt = RYR_FRT;
+    const bool lo = v < -40.0;
+    double n = 1.0, d = 1.0;
+    switch (lane) {
+    case 1: n = -v; d = 11.0; break;
+    case 2: n = 80.0 + v; d = -6.8; break;
+    case 10: n = v + 10.66; d = -11.10; break;
+    case 13: n = 140.0; d = xnai; break;
+    default: break;
+    }
+    const double q1 = n / d;
     double arg = 0.0;
     switch (lane) {
     case 0: arg = -0.1 * (v + 47.13); break;
-    case 1: arg = -v / 11.0; break;
-    case 2: arg = (80.0 + v) / (-6.8); break;
+    case 1: case 2: case 10: arg = q1; break;
     case 3: arg = 0.079 * v; break;
     case 4: arg = 0.35 * v; break;
     case 5: arg = 0.2444 * v; break;
@@ -429,165 +542,315 @@ RYR_DEV void membrane_wave(const double *s, double svikr, double Cm, double dN, 
     case 7: arg = 0.311 * (v + 79.23); break;
     case 8: arg = -0.01052 * v; break;
     case 9: arg = -0.1378 * (v + 40.14); break;
-    case 10: arg = (v + 10.66) / (-11.10); break;
     case 11: arg = -0.00000025350 * v; break;
     case 12: arg = -0.10 * (v + 32.00); break;
-    case 13: arg = -0.123 * (v + 7.00); break;
-    case 14: arg = 0.1450 * (v + 10.00); break;
-    case 15: arg = -(v + 50.00) / 7.50; break;
-    case 16: arg = (v + 33.00) / 22.40; break;
-    case 17: arg = -(v - 1.500) / 16.700; break;
-    case 18: arg = -0.1480 * (v + 30.0); break;
-    case 19: arg = 0.06870 * (v + 30.00); break;
-    case 20: arg = 0.2385 * (v - ek - 59.215); break;
-    case 21: arg = 0.08032 * (v - ek + 5.476); break;
-    case 22: arg = 0.061750 * (v - ek - 594.31); break;
-    case 23: arg = -0.5143 * (v - ek + 4.753); break;
-    case 24: arg = rt1; break;
-    case 25: arg = rt2; break;
-    case 26: arg = -rt1; break;
-    case 27: arg = rt3; break;
-    case 28: arg = rt4; break;
-    case 29: arg = rt5; break;
-    case 30: arg = 140.0 / 67.30; break;
-    case 31: arg = -0.1 * v * frt; break;
-    case 32: arg = -v * frt; break;
     default: break;
     }
     const double ex = ryr_exp(arg);
-    double larg = 1.0;
-    if (lane == 0) larg = 140.0 / xnai;
-    else if (lane == 1) larg = (xko + 0.0183300 * 140.0) / (xki + 0.0183300 * xnai);
-    const double lg = ryr_log(larg);
+    const double lg = ryr_log(lane == 13 ? q1 : 1.0);
+    const double e0 = lanev(ex, 0), e1 = lanev(ex, 1);
 
-    const double csubt = sum_cs / dN;
-    const double xicatto = sum_icat / Cm * 0.0965 * (0.00126) * 2.0 * ((0.5) + 1.);
-    const double xinacato = sum_inaca / Cm * 0.0965 * (0.025);
-
-    /* I_Na (:460-502) */
-    const double ena = (1.0 / frt) * lanev(lg, 0);
-    const double am = 0.32 * (v + 47.13) / (1.0 - lanev(ex, 0));
-    const double bm = 0.08 * lanev(ex, 1);
-    double ah, bh, aj, bj;
-    if (v < -40.0) {
+    /* rates: am; bj; aj's V factor (V < -40) or bh */
+    n = 1.0; d = 1.0;
+    if (lane == 0) { n = 0.32 * (v + 47.13); d = 1.0 - e0; }
+    if (lo) {
+        const double e7 = lanev(ex, 7), e8 = lanev(ex, 8), e9 = lanev(ex, 9);
+        if (lane == 1) { n = 0.1212 * e8; d = 1.0 + e9; }
+        if (lane == 2) { n = v + 37.78; d = 1.0 + e7; }
+    } else {
+        const double e10 = lanev(ex, 10), e11 = lanev(ex, 11), e12 = lanev(ex, 12);
+        if (lane == 1) { n = 0.3 * e11; d = 1.0 + e12; }
+        if (lane == 2) { n = 1.00; d = 0.130 * (1.00 + e10); }
+    }
+    const double q2 = n / d;
+    const double am = lanev(q2, 0);
+    const double bm = 0.08 * e1;
+    const double bj = lanev(q2, 1);
+    double ah, bh, aj;
+    if (lo) {
         ah = 0.135 * lanev(ex, 2);
         bh = 3.56 * lanev(ex, 3) + 310000.0 * lanev(ex, 4);
-        aj = (-127140.0 * lanev(ex, 5) - 0.00003474 * lanev(ex, 6)) * ((v + 37.78) / (1.0 + lanev(ex, 7)));
-        bj = (0.1212 * lanev(ex, 8)) / (1.0 + lanev(ex, 9));
+        aj = (-127140.0 * lanev(ex, 5) - 0.00003474 * lanev(ex, 6)) * lanev(q2, 2);
     } else {
         ah = 0.00;
-        bh = 1.00 / (0.130 * (1.00 + lanev(ex, 10)));
+        bh = lanev(q2, 2);
         aj = 0.00;
-        bj = (0.3 * lanev(ex, 11)) / (1.0 + lanev(ex, 12));
     }
-    const double tauh = 1.00 / (ah + bh);
-    const double tauj = 1.00 / (aj + bj);
-    const double taum = 1.00 / (am + bm);
+    /* time constants (lanes 0-2) and steady states (lanes 3-5) */
+    n = 1.0; d = 1.0;
+    switch (lane) {
+    case 0: n = 1.00; d = ah + bh; break;
+    case 1: n = 1.00; d = aj + bj; break;
+    case 2: n = 1.00; d = am + bm; break;
+    case 3: n = ah; d = ah + bh; break;
+    case 4: n = aj; d = aj + bj; break;
+    case 5: n = am; d = am + bm; break;
+    default: break;
+    }
+    const double q3 = n / d;
+    const double a4 = -dt / q3;
+    const double ex2 = ryr_exp(lane < 3 ? a4 : 0.0);
+
+    const double ena = (1.0 / frt) * lanev(lg, 13);
     const double gna = 15;
     const double gnaleak = 0.3e-3 * 5;
     const double xina = gna * xh * xj * xm * xm * xm * (v - ena) + gnaleak * (v - ena);
+    const double hinf = lanev(q3, 3), jinf = lanev(q3, 4), minf = lanev(q3, 5);
+    const double xh1 = hinf - (hinf - xh) * lanev(ex2, 0);
+    const double xj1 = jinf - (jinf - xj) * lanev(ex2, 1);
+    const double xm1 = minf - (minf - xm) * lanev(ex2, 2);
+    if (lane == 0) {
+        sh[RYR_MS_XM] = xm1; sh[RYR_MS_XH] = xh1; sh[RYR_MS_XJ] = xj1; sh[RYR_MS_INA] = xina;
+    }
+}
 
-    /* I_Kr (:507-529) */
-    const double gss = __builtin_sqrt(xko / 5.40);
-    const double xkrv1 = 0.001380 * (v + 7.00) / (1.0 - lanev(ex, 13));
-    const double xkrv2 = 0.000610 * (v + 10.00) / (lanev(ex, 14) - 1.00);
-    const double taukr = 1.00 / (xkrv1 + xkrv2);
-    const double xkrinf = 1.00 / (1.00 + lanev(ex, 15));
-    const double rg = 1.00 / (1.00 + lanev(ex, 16));
-    const double gkr = 0.0078360;
-    const double xikr = svikr * gkr * gss * xr * rg * (v - ek);
+/* I_Kr (:507-529) and I_Ks (:534-559): xr, xs1, xs2, qks; the sub-membrane
+ * mean csubt */
+RYR_DEV void mem_ikr_iks(const MemIn &I, int lane, double *sh)
+{
+    RYR_STRICT_FP
+    const double v = I.s[0], xr = I.s[4], xs1 = I.s[5], xs2 = I.s[6], qks = I.s[7], xnai = I.s[12];
+    const double dt = I.dt;
+    const double xki = RYR_XKI, xko = RYR_XKO;
+    const double frt = RYR_FRT;
+    const double ek = (1.00 / frt) * ryr_log(xko / xki);
+    double n = 1.0, d = 1.0;
+    switch (lane) {
+    case 2: n = -(v + 50.00); d = 7.50; break;
+    case 3: n = v + 33.00; d = 22.40; break;
+    case 4: n = -(v - 1.500); d = 16.700; break;
+    case 7: n = xko + 0.0183300 * 140.0; d = xki + 0.0183300 * xnai; break;
+    case 8: n = I.sum_cs; d = I.dN; break;
+    default: break;
+    }
+    const double q1 = n / d;
+    double arg = 0.0;
+    switch (lane) {
+    case 0: arg = -0.123 * (v + 7.00); break;
+    case 1: arg = 0.1450 * (v + 10.00); break;
+    case 2: case 3: case 4: arg = q1; break;
+    case 5: arg = -0.1480 * (v + 30.0); break;
+    case 6: arg = 0.06870 * (v + 30.00); break;
+    default: break;
+    }
+    const double ex = ryr_exp(arg);
+    const double lg = ryr_log(lane == 7 ? q1 : 1.0);
+    const double csubt = lanev(q1, 8);
 
-    /* I_Ks (:534-559) */
-    const double prnak = 0.0183300;
-    (void)prnak;
+    /* rates and steady states */
+    const double e = ex;                         /* lane k uses its own exponential */
+    n = 1.0; d = 1.0;
+    switch (lane) {
+    case 0: n = 0.001380 * (v + 7.00); d = 1.0 - e; break;
+    case 1: n = 0.000610 * (v + 10.00); d = e - 1.00; break;
+    case 2: n = 1.00; d = 1.00 + e; break;
+    case 3: n = 1.00; d = 1.00 + e; break;
+    case 4: n = 1.0; d = 1.0 + e; break;
+    case 5: n = 0.0000719 * (v + 30.00); d = 1.00 - e; break;
+    case 6: n = 0.0001310 * (v + 30.00); d = e - 1.00; break;
+    case 7: n = 0.28; d = csubt; break;
+    default: break;
+    }
+    const double q2 = n / d;
+    const double xkrv1 = lanev(q2, 0), xkrv2 = lanev(q2, 1);
+    const double tx1 = lanev(q2, 5), tx2 = lanev(q2, 6);
     /* slowncxchange (:540-543) doubles the Ca dependence and the time constant */
-    const double qks_inf = slowncx ? 0.2 * (1 + 1.6 / (1 + ryr_pow((0.28 / csubt), 3)))
-                                   : 0.2 * (1 + 0.8 / (1 + ryr_pow((0.28 / csubt), 3)));
-    const double tauqks = slowncx ? 2000 : 1000.00;
-    const double eks = (1.00 / frt) * lanev(lg, 1);
-    const double xs1ss = 1.0 / (1.0 + lanev(ex, 17));
-    double tauxs = 1.00 / (0.0000719 * (v + 30.00) / (1.00 - lanev(ex, 18))
-                           + 0.0001310 * (v + 30.00) / (lanev(ex, 19) - 1.00));
+    const double qnum = I.slowncx ? 1.6 : 0.8;
+    n = 1.0; d = 1.0;
+    switch (lane) {
+    case 0: n = 1.00; d = xkrv1 + xkrv2; break;
+    case 1: n = 1.00; d = tx1 + tx2; break;
+    case 2: n = qnum; d = 1 + ryr_pow(lanev(q2, 7), 3); break;
+    default: break;
+    }
+    const double q3 = n / d;
+    const double taukr = lanev(q3, 0);
+    double tauxs = lanev(q3, 1);
     tauxs = 0.95 * tauxs;
+    const double qks_inf = 0.2 * (1 + lanev(q3, 2));
+    const double tauqks = I.slowncx ? 2000 : 1000.00;
+    n = 1.0; d = 1.0;
+    switch (lane) {
+    case 0: n = -dt; d = taukr; break;
+    case 1: n = -dt; d = tauxs; break;
+    case 2: n = dt * (qks_inf - qks); d = tauqks; break;
+    default: break;
+    }
+    const double q4 = n / d;
+    const double ex2 = ryr_exp(lane < 2 ? q4 : 0.0);
+
+    const double gss = __builtin_sqrt(xko / 5.40);
+    const double xkrinf = lanev(q2, 2);
+    const double rg = lanev(q2, 3);
+    const double gkr = 0.0078360;
+    const double xikr = I.svikr * gkr * gss * xr * rg * (v - ek);
+    const double eks = (1.00 / frt) * lanev(lg, 7);
+    const double xs1ss = lanev(q2, 4);
     const double gksx = 0.2000;
     const double sviks_iso = 1.;
     const double xiks = 1.0 * sviks_iso * gksx * qks * xs1 * xs2 * (v - eks);
+    const double xr1 = xkrinf - (xkrinf - xr) * lanev(ex2, 0);
+    const double xs11 = xs1ss - (xs1ss - xs1) * lanev(ex2, 1);
+    const double xs21 = xs1ss - (xs1ss - xs2) * lanev(ex2, 1);
+    const double qks1 = qks + lanev(q4, 2);
+    if (lane == 0) {
+        sh[RYR_MS_XR] = xr1; sh[RYR_MS_XS1] = xs11; sh[RYR_MS_XS2] = xs21; sh[RYR_MS_QKS] = qks1;
+        sh[RYR_MS_IKR] = xikr; sh[RYR_MS_IKS] = xiks; sh[RYR_MS_CSUBT] = csubt;
+    }
+}
 
-    /* I_K1 (:590-600) */
+/* I_K1 (:590-600), I_to,s and I_to,f (:604-659) */
+RYR_DEV void mem_ik1_ito(const MemIn &I, int lane, double *sh)
+{
+    RYR_STRICT_FP
+    const double v = I.s[0], xtos = I.s[8], ytos = I.s[9], xtof = I.s[10], ytof = I.s[11], dt = I.dt;
+    const double xki = RYR_XKI, xko = RYR_XKO;
+    const double frt = RYR_FRT;
+    const double ek = (1.00 / frt) * ryr_log(xko / xki);
+    /* rt1, rt2 (= rt5), rt3 and v/30 */
+    double n = 1.0, d = 1.0;
+    switch (lane) {
+    case 0: n = -(v + 3.0); d = 15.00; break;
+    case 1: n = v + 33.5; d = 10.00; break;
+    case 2: n = v + 60.00; d = 10.00; break;
+    case 3: n = v; d = 30.00; break;
+    default: break;
+    }
+    const double q1 = n / d;
+    const double rt1 = lanev(q1, 0), rt2 = lanev(q1, 1), rt3 = lanev(q1, 2), v30 = lanev(q1, 3);
+    const double rt4 = -(v30) * (v30);
+    double arg = 0.0;
+    switch (lane) {
+    case 0: arg = 0.2385 * (v - ek - 59.215); break;
+    case 1: arg = 0.08032 * (v - ek + 5.476); break;
+    case 2: arg = 0.061750 * (v - ek - 594.31); break;
+    case 3: arg = -0.5143 * (v - ek + 4.753); break;
+    case 4: arg = rt1; break;
+    case 5: arg = rt2; break;
+    case 6: arg = -rt1; break;
+    case 7: arg = rt3; break;
+    case 8: arg = rt4; break;
+    case 9: arg = rt2; break;            /* rt5 = (v + 33.50) / 10.00 = rt2 */
+    default: break;
+    }
+    const double ex = ryr_exp(arg);
+    const double e1 = lanev(ex, 1), e2 = lanev(ex, 2);
+    n = 1.0; d = 1.0;
+    switch (lane) {
+    case 3: n = 0.49124 * e1 + e2; d = 1.0 + ex; break;           /* bki */
+    case 0: n = 1.02; d = 1.0 + ex; break;                         /* aki */
+    case 4: n = 1.00; d = 1.0 + ex; break;                         /* xtos_inf */
+    case 5: n = 1.00; d = 1.00 + ex; break;                        /* ytos_inf = rs_inf */
+    case 6: n = 9.00; d = 1.00 + ex; break;                        /* txs */
+    case 7: n = 3000.00; d = 1.0 + ex; break;                      /* tys */
+    case 9: n = 20.0; d = 1.0 + ex; break;                         /* tyf */
+    default: break;
+    }
+    const double q2 = n / d;
+    const double aki = lanev(q2, 0), bki = lanev(q2, 3);
+    const double txs = lanev(q2, 6) + 0.50;
+    const double tys = lanev(q2, 7) + 30.00;
+    const double txf = 3.50 * lanev(ex, 8) + 1.50;
+    const double tyf = lanev(q2, 9) + 20.00;
+    n = 1.0; d = 1.0;
+    switch (lane) {
+    case 0: n = aki; d = aki + bki; break;
+    case 1: n = -dt; d = txs; break;
+    case 2: n = -dt; d = tys; break;
+    case 3: n = -dt; d = txf; break;
+    case 4: n = -dt; d = tyf; break;
+    default: break;
+    }
+    const double q3 = n / d;
+    const double ex2 = ryr_exp((lane >= 1 && lane <= 4) ? q3 : 0.0);
+
     const double gkix = 0.600;
     const double gki = gkix * (__builtin_sqrt(xko / 5.4));
-    const double aki = 1.02 / (1.0 + lanev(ex, 20));
-    const double bki = (0.49124 * lanev(ex, 21) + lanev(ex, 22)) / (1.0 + lanev(ex, 23));
-    const double xkin = aki / (aki + bki);
+    const double xkin = lanev(q3, 0);
     const double xik1 = (1.0) * gki * xkin * (v - ek);
-
-    /* I_to,s and I_to,f (:604-659) */
-    const double xtos_inf = 1.00 / (1.0 + lanev(ex, 24));
-    const double ytos_inf = 1.00 / (1.00 + lanev(ex, 25));
-    const double rs_inf = 1.00 / (1.00 + lanev(ex, 25));
-    const double txs = 9.00 / (1.00 + lanev(ex, 26)) + 0.50;
-    const double tys = 3000.00 / (1.0 + lanev(ex, 27)) + 30.00;
+    const double xtos_inf = lanev(q2, 4);
+    const double ytos_inf = lanev(q2, 5);
+    const double rs_inf = ytos_inf;
     const double gtos = 0.040;
     const double xitos = 1.0 * gtos * xtos * (ytos + 0.50 * rs_inf) * (v - ek);
     const double xtof_inf = xtos_inf;
     const double ytof_inf = ytos_inf;
-    const double txf = 3.50 * lanev(ex, 28) + 1.50;
-    const double tyf = 20.0 / (1.0 + lanev(ex, 29)) + 20.00;
     const double gtof = 0.10;
     const double xitof = 1.0 * gtof * xtof * ytof * (v - ek);
+    const double xtos1 = xtos_inf - (xtos_inf - xtos) * lanev(ex2, 1);
+    const double ytos1 = ytos_inf - (ytos_inf - ytos) * lanev(ex2, 2);
+    const double xtof1 = xtof_inf - (xtof_inf - xtof) * lanev(ex2, 3);
+    const double ytof1 = ytof_inf - (ytof_inf - ytof) * lanev(ex2, 4);
+    if (lane == 0) {
+        sh[RYR_MS_XTOS] = xtos1; sh[RYR_MS_YTOS] = ytos1; sh[RYR_MS_XTOF] = xtof1; sh[RYR_MS_YTOF] = ytof1;
+        sh[RYR_MS_IK1] = xik1; sh[RYR_MS_ITOS] = xitos; sh[RYR_MS_ITOF] = xitof;
+    }
+}
 
-    /* I_NaK (:665-676) */
+/* I_NaK (:665-676) and the whole-cell Ca currents from the CRU sums */
+RYR_DEV void mem_inak(const MemIn &I, int lane, double *sh)
+{
+    RYR_STRICT_FP
+    const double v = I.s[0], xnai = I.s[12];
+    const double xko = RYR_XKO;
+    const double frt = RYR_FRT;
     const double xkmko = 1.50, xkmnai = 12.00, xibarnak = 1.5000, hh = 1.00;
-    const double sigma = (lanev(ex, 30) - 1.00) / 7.00;
-    const double fnak = 1.00 / (1.0 + 0.1245 * lanev(ex, 31) + 0.0365 * sigma * lanev(ex, 32));
-    const double xinak = 1.0 * xibarnak * fnak * (1.0 / (1.0 + ryr_pow((xkmnai / xnai), hh))) * xko / (xko + xkmko);
-
-    /* pass 2: Rush-Larsen factors */
-    double arg2 = 0.0;
+    double n = 1.0, d = 1.0;
     switch (lane) {
-    case 0: arg2 = -dt / tauh; break;
-    case 1: arg2 = -dt / tauj; break;
-    case 2: arg2 = -dt / taum; break;
-    case 3: arg2 = -dt / taukr; break;
-    case 4: arg2 = -dt / tauxs; break;
-    case 5: arg2 = -dt / txs; break;
-    case 6: arg2 = -dt / tys; break;
-    case 7: arg2 = -dt / txf; break;
-    case 8: arg2 = -dt / tyf; break;
+    case 0: n = I.sum_icat; d = I.Cm; break;
+    case 1: n = I.sum_inaca; d = I.Cm; break;
+    case 2: n = xkmnai; d = xnai; break;
     default: break;
     }
-    const double ex2 = ryr_exp(arg2);
-    xh = ah / (ah + bh) - ((ah / (ah + bh)) - xh) * lanev(ex2, 0);
-    xj = aj / (aj + bj) - ((aj / (aj + bj)) - xj) * lanev(ex2, 1);
-    xm = am / (am + bm) - ((am / (am + bm)) - xm) * lanev(ex2, 2);
-    xr = xkrinf - (xkrinf - xr) * lanev(ex2, 3);
-    xs1 = xs1ss - (xs1ss - xs1) * lanev(ex2, 4);
-    xs2 = xs1ss - (xs1ss - xs2) * lanev(ex2, 4);
-    qks = qks + dt * (qks_inf - qks) / tauqks;
-    xtos = xtos_inf - (xtos_inf - xtos) * lanev(ex2, 5);
-    ytos = ytos_inf - (ytos_inf - ytos) * lanev(ex2, 6);
-    xtof = xtof_inf - (xtof_inf - xtof) * lanev(ex2, 7);
-    ytof = ytof_inf - (ytof_inf - ytof) * lanev(ex2, 8);
+    const double q1 = n / d;
+    double arg = 0.0;
+    switch (lane) {
+    case 0: arg = 140.0 / 67.30; break;
+    case 1: arg = -0.1 * v * frt; break;
+    case 2: arg = -v * frt; break;
+    default: break;
+    }
+    const double ex = ryr_exp(arg);
+    n = 1.0; d = 1.0;
+    switch (lane) {
+    case 0: n = ex - 1.00; d = 7.00; break;                                 /* sigma */
+    case 1: n = 1.0; d = 1.0 + ryr_pow(lanev(q1, 2), hh); break;
+    default: break;
+    }
+    const double q2 = n / d;
+    const double sigma = lanev(q2, 0);
+    const double fnak = 1.00 / (1.0 + 0.1245 * lanev(ex, 1) + 0.0365 * sigma * lanev(ex, 2));
+    const double xinak = 1.0 * xibarnak * fnak * lanev(q2, 1) * xko / (xko + xkmko);
+    const double xicatto = lanev(q1, 0) * 0.0965 * (0.00126) * 2.0 * ((0.5) + 1.);
+    const double xinacato = lanev(q1, 1) * 0.0965 * (0.025);
+    if (lane == 0) { sh[RYR_MS_ICAT] = xicatto; sh[RYR_MS_INACA] = xinacato; sh[RYR_MS_INAK] = xinak; }
+}
 
-    /* [Na]i and V (:701-738) */
+/* [Na]i and V (:701-738) from the groups' currents; fills slots V and NAI */
+RYR_DEV void mem_finish(const double *s, const double *sh, double dt, double stim, double *v_out,
+                        double *nai_out)
+{
+    RYR_STRICT_FP
+    const double xina = sh[RYR_MS_INA], xik1 = sh[RYR_MS_IK1], xikr = sh[RYR_MS_IKR], xiks = sh[RYR_MS_IKS];
+    const double xitos = sh[RYR_MS_ITOS], xitof = sh[RYR_MS_ITOF], xinacato = sh[RYR_MS_INACA];
+    const double xicatto = sh[RYR_MS_ICAT], xinak = sh[RYR_MS_INAK];
     const double wcainv = 1.0 / 50.0;
     const double conv = 0.18 * 12500;
     const double xinaca = xinacato;
     const double trick = 1.0;
     const double xrr = trick * (1.0 / wcainv / conv) / 1000.0;
     const double dnai = -xrr * (xina + 3.0 * xinak + 3.0 * xinaca);
+    double xnai = s[12];
     xnai += dnai * dt;
     const double dvh = -(xina + xik1 + 1 * xikr + 1 * xiks + xitos + 1 * xitof + 1. * xinacato + 1 * xicatto + xinak) + stim;
+    double v = s[0];
     v += dvh * dt;
-
-    out[0] = v; out[1] = xm; out[2] = xh; out[3] = xj; out[4] = xr; out[5] = xs1; out[6] = xs2;
-    out[7] = qks; out[8] = xtos; out[9] = ytos; out[10] = xtof; out[11] = ytof; out[12] = xnai;
-    cur[0] = csubt; cur[1] = xicatto; cur[2] = xinacato; cur[3] = xina; cur[4] = xiks;
-    cur[5] = xikr; cur[6] = xik1; cur[7] = xinak; cur[8] = xitos; cur[9] = xitof;
+    *v_out = v;
+    *nai_out = xnai;
 }
 
-/* The V-only factors of the next step (vfactors) with the exponentials
- * spread over the lanes; lane 0 stores them. */
+/* The V-only factors of the next step (vfactors): the exponentials and the
+ * divisions spread over the lanes; lane 0 stores them. */
 RYR_DEV void vfactors_wave(RyrMem *M, int iso2, int lane, const double v, const double xnai)
 {
     RYR_STRICT_FP
@@ -595,11 +858,18 @@ RYR_DEV void vfactors_wave(RyrMem *M, int iso2, int lane, const double v, const 
     if (iso2) { dv5 = 0; fv5 = -28; fvk = 8.5; }
     const double q = 0.0337 * (v + 10.5);
     const double za = v * 96.485 / 8.314 / 308;
+    double n = 1.0, dd = 1.0;
+    switch (lane) {
+    case 0: n = -(v - dv5); dd = dvk; break;
+    case 1: n = -(v - fv5); dd = fvk; break;
+    case 2: n = v - 60; dd = 12.; break;
+    case 3: n = xnai; dd = 12.3; break;
+    default: break;
+    }
+    const double q1 = n / dd;
     double arg = 0.0;
     switch (lane) {
-    case 0: arg = -(v - dv5) / dvk; break;
-    case 1: arg = -(v - fv5) / fvk; break;
-    case 2: arg = (v - 60) / 12.; break;
+    case 0: case 1: case 2: arg = q1; break;
     case 3: arg = -(q * q); break;
     case 4: arg = 2.0 * za; break;
     case 5: arg = 0.35 * za; break;
@@ -609,12 +879,29 @@ RYR_DEV void vfactors_wave(RyrMem *M, int iso2, int lane, const double v, const 
     const double ex = ryr_exp(arg);
     const double e0 = lanev(ex, 0), e1 = lanev(ex, 1), e2 = lanev(ex, 2), e3 = lanev(ex, 3);
     const double e4 = lanev(ex, 4), e5 = lanev(ex, 5), e6 = lanev(ex, 6);
+    n = 1.0; dd = 1.0;
+    switch (lane) {
+    case 0: n = 1.0; dd = 1.0 + e0; break;                          /* dinf */
+    case 1: n = 1.0 - e0; dd = 0.035 * (v - dv5); break;
+    case 2: n = 1.0; dd = 1.0 + e1; break;
+    default: break;
+    }
+    const double q2 = n / dd;
+    const double dinf = lanev(q2, 0);
+    n = 1.0; dd = 1.0;
+    switch (lane) {
+    case 0: n = 1.0; dd = lanev(q2, 1) * dinf; break;               /* taudin */
+    case 1: n = 0.035 * dvk; dd = dinf; break;                      /* taudin near V = 0 */
+    case 2: n = lanev(q2, 2); dd = 1. + e2; break;
+    default: break;
+    }
+    const double q3 = n / dd;
+    double taudin = lanev(q3, 0);
+    if ((v < 0.0001) && (v > -0.0001)) taudin = lanev(q3, 1);
+    const double finf = 1. - lanev(q3, 2);
+    const double taufin = (0.02 - 0.007 * e3);
+    const double r = lanev(q1, 3);
     if (lane != 0) return;
-    double dinf = 1.0 / (1.0 + e0);
-    double taudin = 1.0 / ((1.0 - e0) / (0.035 * (v - dv5)) * dinf);
-    if ((v < 0.0001) && (v > -0.0001)) taudin = 0.035 * dvk / dinf;
-    double finf = 1. - 1.0 / (1.0 + e1) / (1. + e2);
-    double taufin = (0.02 - 0.007 * e3);
     M->alphad = dinf * taudin;
     M->betad = (1 - dinf) * taudin;
     M->alphaf = (finf) * taufin;
@@ -623,7 +910,6 @@ RYR_DEV void vfactors_wave(RyrMem *M, int iso2, int lane, const double v, const 
     M->ghk_e2 = e4;
     M->ghk_A = 4.0 * 11.9 * za * 96.485 * 0.341;
     M->ghk_D = M->ghk_e2 - 1.0;
-    double r = xnai / 12.3;
     M->ncx_t1 = 0.00359 * ((140.0) * (140.0) * (140.0)) * (1.0 + r * r * r);
     double p3 = xnai * xnai * xnai;
     M->ncx_t3a = (1.3 + (1 * 1.8)) * p3;
@@ -983,8 +1269,13 @@ RYR_DEV void fast_cleft(float dt, float cif0, float cif1, float csf0, float csu,
 }
 
 /* ------------------------------------------------------------ step kernel */
+/* The 64- and 128-lane scheme-A instances, which the shape tuner picks for
+ * batches (throughput), are held to 168 VGPRs: three waves per SIMD, without
+ * spilling.  The 256-lane instance, which it picks for single cells (one
+ * wave per SIMD anyway), keeps the registers the allocator wants. */
 template <typename TF, typename TS, int WG, bool SCHB>
-__global__ __launch_bounds__(WG) void k_cru_step(RyrKArgs a, int kc)
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu((WG >= 256 || SCHB) ? 1 : 3)))
+void k_cru_step(RyrKArgs a, int kc)
 {
     constexpr int SUB = RYR_BLOCK / WG;
     const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
@@ -1047,11 +1338,9 @@ __global__ __launch_bounds__(WG) void k_cru_step(RyrKArgs a, int kc)
                 }
         }
 
-        ryr_stream rs;
-        ryr_stream_init(&rs, C.seed, (uint32_t)n, (uint64_t)nstep);
-
         /* LCC gating and I_CaL (cuda_Cell_Detailed.cu:1248-1272): the slots
-         * up to the first absent one, as the reference's loop */
+         * up to the first absent one, as the reference's loop; channel l
+         * draws static slot l */
         const double cp_d = (double)cpv, cs_d = (double)csv;
         double icat1 = 0.0, xicat, bcgcur = 0.0;
         {
@@ -1061,9 +1350,12 @@ __global__ __launch_bounds__(WG) void k_cru_step(RyrKArgs a, int kc)
                 const double betac = a.iso2 ? 2.0 : 4.0;
                 int nlcp = 0;
                 uint32_t nw = lw;
+                ryr_u32x4 lb = {{0u, 0u, 0u, 0u}};
                 for (int ll = 0; ll < 8 && ((lmask >> ll) & 1u); ++ll) {
                     int st = (int)((lw >> (4 * ll)) & 15u);
-                    int jj = lcc_step(st, cpx, dt, ryr_stream_uniform(&rs), *M, betac);
+                    if (!(ll & 1)) lb = ryr_block(C.seed, (uint32_t)n, (uint64_t)nstep, (uint32_t)(ll >> 1));
+                    const double ragg = (ll & 1) ? ryr_slot_uniform(lb.v[2], lb.v[3]) : ryr_slot_uniform(lb.v[0], lb.v[1]);
+                    int jj = lcc_step(st, cpx, dt, ragg, *M, betac);
                     nw = (nw & ~(15u << (4 * ll))) | ((uint32_t)jj << (4 * ll));
                     if (jj == 0) { ++nl; ++nlcp; }
                 }
@@ -1085,7 +1377,9 @@ __global__ __launch_bounds__(WG) void k_cru_step(RyrKArgs a, int kc)
         const int prevsp = nou + nob;
         double roo2;
         const double mono = csqn_mono((double)cjsrv, &roo2);
-        ryr_gating(&rs, C.Ku, C.Ku2, C.Kb, cp_d, (double)cjsrv, mono, dt, nou, ncu, nob, ncb, &nfail);
+        int ndraw = 0;
+        ryr_gating(C.seed, (uint32_t)n, (uint64_t)nstep, C.Ku, C.Ku2, C.Kb, cp_d, (double)cjsrv, mono, dt, nou, ncu,
+                   nob, ncb, &nfail, &ndraw);
         if (prevsp < 5 && (nou + nob) >= 5) nspark = 1;
         double xire;
         {
@@ -1310,9 +1604,10 @@ __global__ __launch_bounds__(WG) void k_cru_step(RyrKArgs a, int kc)
 /* ------------------------------------------------------------ membrane kernel */
 /* One workgroup per cell, after k_cru_step: the chunk partials of the three
  * whole-cell sums (pairwise trees over groups of 256 chunks, groups in order),
- * then the membrane ODE on wavefront 0, the clamp protocol, the beat tracker,
- * the membrane columns of an output row and the V-only factors of the next
- * step (cuda_Cell_Detailed.cu:430-739). */
+ * then the membrane ODE's four current groups on the workgroup's waves, and on
+ * wavefront 0 [Na]i and V, the clamp protocol, the beat tracker, the membrane
+ * columns of an output row and the V-only factors of the next step
+ * (cuda_Cell_Detailed.cu:430-739). */
 template <typename TF, typename TS, int WG>
 RYR_DEV void membrane_block(const RyrKArgs &a, int c, long long nstep, double stim_amp)
 {
@@ -1349,12 +1644,33 @@ RYR_DEV void membrane_block(const RyrKArgs &a, int c, long long nstep, double st
             acc[q] = dd_add(acc[q], dd_add(dd_add(s_red[q][0], s_red[q][1]), dd_add(s_red[q][2], s_red[q][3])));
         __syncthreads();
     }
+
+    /* the four current groups, group g on wave g mod NW */
+    __shared__ double s_ms[RYR_MS_N];
+    {
+        MemIn I;
+        I.s = M->m; I.svikr = C.svikr; I.Cm = C.Cm; I.dN = C.dN;
+        I.sum_icat = acc[0].hi; I.sum_inaca = acc[1].hi; I.sum_cs = acc[2].hi; I.dt = dt;
+        I.slowncx = (C.variants & RYR_VAR_SLOWNCX) != 0;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            if (g % NW != wv) continue;
+            if (g == 0) mem_ina(I, lane, s_ms);
+            else if (g == 1) mem_ikr_iks(I, lane, s_ms);
+            else if (g == 2) mem_ik1_ito(I, lane, s_ms);
+            else mem_inak(I, lane, s_ms);
+        }
+    }
+    __syncthreads();
     if (wv != 0) return;
 
     const double stim = stim_of(C, nstep, stim_amp);
     double mo[13], cu[10];
-    membrane_wave(M->m, C.svikr, C.Cm, C.dN, acc[0].hi, acc[1].hi, acc[2].hi, dt, stim, lane, mo, cu,
-                  (C.variants & RYR_VAR_SLOWNCX) != 0);
+    mem_finish(M->m, s_ms, dt, stim, &mo[0], &mo[12]);
+#pragma unroll
+    for (int q = 1; q < 12; ++q) mo[q] = s_ms[q];
+#pragma unroll
+    for (int q = 0; q < 10; ++q) cu[q] = s_ms[RYR_MS_CSUBT + q];
     double vnew = mo[0];
     if (C.vclamp_n > 0) {
         /* clamp protocols (cuda_Cell_Detailed.cu:733-773) */
@@ -1780,31 +2096,34 @@ __global__ void k_gating_probe(long long nq, unsigned long long seed, const uint
 {
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nq) return;
-    ryr_stream rs;
-    ryr_stream_init(&rs, seed, cru[q], step);
     const double cp = conc[4 * q], cjsr = conc[4 * q + 1], cs = conc[4 * q + 2], v = conc[4 * q + 3];
     RyrMem m;
     m.m[0] = v;
     m.m[12] = 10.0;
     vfactors(&m, iso2);
-    int no = 0;
+    int no = 0, nlc = 0;
     {
         RYR_STRICT_FP
         const double cpx = (cp + cs * (0.5)) / (1 + (0.5));
         const double betac = iso2 ? 2.0 : 4.0;
+        ryr_u32x4 lb = {{0u, 0u, 0u, 0u}};
         for (int ll = 0; ll < 8 && lcc[8 * q + ll] < 16; ll++) {
-            int jj = lcc_step(lcc[8 * q + ll], cpx, dt, ryr_stream_uniform(&rs), m, betac);
+            if (!(ll & 1)) lb = ryr_block(seed, cru[q], step, (uint32_t)(ll >> 1));
+            const double ragg = (ll & 1) ? ryr_slot_uniform(lb.v[2], lb.v[3]) : ryr_slot_uniform(lb.v[0], lb.v[1]);
+            int jj = lcc_step(lcc[8 * q + ll], cpx, dt, ragg, m, betac);
             lcc[8 * q + ll] = jj;
             if (jj == 0) ++no;
+            ++nlc;
         }
     }
     int nou = counts[4 * q], ncu = counts[4 * q + 1], nob = counts[4 * q + 2], ncb = counts[4 * q + 3];
     int f = 0;
     double roo2;
     const double mono = csqn_mono(cjsr, &roo2);
-    ryr_gating(&rs, Ku, Ku2, Kb, cp, cjsr, mono, dt, nou, ncu, nob, ncb, &f);
+    int nd = 0;
+    ryr_gating(seed, cru[q], step, Ku, Ku2, Kb, cp, cjsr, mono, dt, nou, ncu, nob, ncb, &f, &nd);
     counts[4 * q] = nou; counts[4 * q + 1] = ncu; counts[4 * q + 2] = nob; counts[4 * q + 3] = ncb;
-    draws[q] = (int32_t)rs.slot;
+    draws[q] = (int32_t)(nlc + nd);
     nopen[q] = no;
     fails[q] = f;
 }

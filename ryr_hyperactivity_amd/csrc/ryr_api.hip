@@ -573,6 +573,9 @@ static void sched_candidates(const ryr_handle *h, std::vector<std::array<int, 3>
     };
     add(h->wg_cyto, h->wg_cru, RYR_SCHED_SEQ);
     add(h->wg_cyto, h->wg_cru, RYR_SCHED_FUSED);
+    /* fused, the membrane's four current groups on four waves: the sub-voxel
+     * size timed alone does not see that */
+    add(256, h->wg_cru, RYR_SCHED_FUSED);
     if (nsched(h) > 2) {
         add(h->wg_cyto, h->wg_cru, RYR_SCHED_CONCURRENT);
         add(128, 64, RYR_SCHED_CONCURRENT);

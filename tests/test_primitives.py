@@ -55,6 +55,52 @@ def test_stream_statistics_and_addressing():
     assert u[3] == (x + 0.5) * 2.0 ** -52
 
 
+def _slot_u(blk, half):
+    a, b = blk[2 * half], blk[2 * half + 1]
+    return (((a << 20) | (b >> 12)) + 0.5) * 2.0 ** -52
+
+
+def test_transition_draw_addressing():
+    """RyR transition t draws its first uniform from static slot 8 + t (block
+    4 + t//2 of counter (block, CRU, step)) and its further ones from its own
+    substream, block ((t + 1) << 24) | (j >> 1).  Only OU->CU (t = 0) moves
+    here, so its count is Knuth's product rule applied to those uniforms,
+    recomputed from the known-answer-pinned Philox blocks."""
+    n = 300
+    seed, step = 0x1234_5678_9ABC, 41
+    counts = np.zeros((n, 4), np.int32)
+    counts[:, 0] = 100
+    conc = np.tile([1.0, 740.0, 0.1, -80.0], (n, 1))
+    lcc = np.full((n, 8), 16, np.int32)
+    out, _, draws, _, _ = orc.gating_probe(seed, np.arange(n, dtype=np.uint32), step, 10.0, 1.0, 0.0, 0.025, 1,
+                                           counts, lcc, conc)
+    key = (seed & 0xFFFFFFFF, seed >> 32)
+    lam = orc.lib().orc_dm_exp(-(100) * 0.025)
+    for q in range(n):
+        puu = _slot_u(orc.philox((4, q, step, 0), key), 0)
+        kk, j = 1, 0
+        while puu >= lam and kk < 195:
+            kk += 1
+            puu = puu * _slot_u(orc.philox(((1 << 24) | (j >> 1), q, step, 0), key), j & 1)
+            j += 1
+        assert out[q, 1] == kk - 1
+        assert draws[q] == 8 + j
+
+
+def test_knuth_early_out_bound():
+    """The kernels settle a Knuth transition at count 0 without the
+    exponential when the first uniform lies below 1 - x (x >= 2^-20) or
+    1 - 2x (2^-40 <= x < 2^-20), x = n p.  That needs the rounded bound to lie
+    at or below the rounded exp(-x) that the loop compares against."""
+    L = orc.lib()
+    rng = np.random.default_rng(5)
+    xs = np.concatenate([2.0 ** rng.uniform(-40, 0, 60000), 2.0 ** -np.arange(0, 41, dtype=float),
+                         np.nextafter(2.0 ** -20, 0) * np.ones(1), np.linspace(0.5, 1.0, 2001)])
+    for x in xs:
+        thr = 1.0 - x if x >= 2.0 ** -20 else 1.0 - 2.0 * x
+        assert thr <= L.orc_dm_exp(-x)
+
+
 def _ulps(a, b):
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)

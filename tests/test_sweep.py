@@ -65,7 +65,7 @@ def test_steered_map_spends_less_than_the_plain_map():
     from ryr_hyperactivity_amd.sweep import Cost, simulate_points, steered_map
     print("case steered map 16x6x4", flush=True)
     lat, R, NB = (16, 6, 4), 16, 5
-    ikr = [0, 0.125, 0.25, 0.375, 0.5, 0.625, 0.75, 0.875, 1.0]
+    ikr = [0.5, 0.75, 1.0, 1.25, 1.5, 1.75, 2.0, 2.25, 2.5]       # EADs fade out as I_Kr grows
     grid = grid_points([1.0, 4.0], ikr, [1000.0])
     plain, pcost = simulate_points(grid, R, NB, lat, progress=lambda s: None)
     train = grid_points([1.0, 4.0], ikr[::2], [1000.0])
@@ -82,6 +82,11 @@ def test_steered_map_spends_less_than_the_plain_map():
     err = [abs(sm.values[p] - plain[p].ead) for p in grid if sm.source[p] == "surrogate"]
     print(f"surrogate points {n_sur}/{len(grid)}, cell-step ratio {ratio:.2f}, "
           f"surrogate-point error mean {np.mean(err) if err else 0:.3f} max {max(err) if err else 0:.3f}", flush=True)
+    for p in grid:
+        inc, spr = sm.predicted.get(p, (float("nan"), float("nan")))
+        print(f"  ku2 {p.ku2:g} ikr {p.ikr:.3f}: plain {plain[p].ead:.3f} "
+              f"[{plain[p].ead_lo:.2f}, {plain[p].ead_hi:.2f}] predicted {inc:.3f} spread {spr:.3f} "
+              f"-> {sm.source[p]}", flush=True)
     assert calls[0] == len(train) and srcs.count("train") == len(train)
     assert n_sur >= 3 and ratio < 0.85
     assert np.mean(err) < 0.1 and max(err) < 0.25
