@@ -253,7 +253,9 @@ int ryr_kernel_time(ryr_handle *h, double *ms, int64_t *launches);
 
 /* Launch shape of a step: the workgroup sizes (64, 128 or 256 lanes) of the
  * two CRU-parallel kernels (the sub-voxel cytosol/NSR kernel and the CRU
- * kernel), and the schedule of a step's kernels:
+ * kernel; for the CRU kernel also 512: 256-lane workgroups with two lanes per
+ * CRU, which split the RyR transitions and the LCC channels between them, for
+ * grids too small to fill the GPU), and the schedule of a step's kernels:
  *   RYR_SCHED_SEQ         sub-voxel kernel, CRU kernel, membrane step
  *   RYR_SCHED_FUSED       the membrane step of step n rides in the sub-voxel
  *                         launch of step n+1
@@ -262,8 +264,9 @@ int ryr_kernel_time(ryr_handle *h, double *ms, int64_t *launches);
  * No reference counterpart (the reference hard-codes one 48x448 launch,
  * :1701).  A new handle tunes the shape itself: its first 9 steps alternate
  * the sizes with each kernel timed by its own events (fastest median kept);
- * then each schedule candidate (the tuned sizes with every schedule, and the
- * concurrent schedule with smaller sizes) runs 12 steps as three replays of a
+ * then each schedule candidate (the tuned sizes with every schedule, the fused
+ * schedule with 256-lane sub-voxel workgroups, the two-lanes-per-CRU kernel,
+ * and the concurrent schedule with smaller sizes) runs 12 steps as three replays of a
  * captured 4-step graph, timed by events, and the fastest is kept.  Output
  * steps are stepped over eagerly, so tuning takes about 100 steps.  Every
  * shape computes identical values, so tuning never changes a result.  Passing

@@ -291,21 +291,32 @@ RYR_DEV void vfactors(RyrMem *m, int iso2)
 }
 
 /* ---------------------------------------------------------- gating (FP64) */
+/* reesica.h:14-116 with the V-only rates of the step precomputed (vfactors).
+ * The reference walks the four gates in order: a gate flips if the scaled
+ * uniform lies below its rate, else the rate is subtracted and the next gate
+ * is tried.  Here the same three subtractions are always formed and the
+ * first gate whose comparison holds is picked by selects, so every lane of a
+ * wave runs one path whatever its channel's state; the values compared are
+ * those of the sequential walk. */
 RYR_DEV int lcc_step(int i, double cpx, double dt, double ragg, const RyrMem &m, double betac)
 {
     RYR_STRICT_FP
     const double alphac = 0.22, alphafca = 0.012 / 2.;
     double betafca = 0.175 / (1 + (25 / cpx) * (25 / cpx));
+    const double r1 = (i & 1) ? alphac : betac;
+    const double r2 = (i & 2) ? m.alphad : m.betad;
+    const double r4 = (i & 4) ? m.alphaf : m.betaf;
+    const double r8 = (i & 8) ? alphafca : betafca;
     double rig = ragg / dt;
-    if ((i % 2)) { if (rig < alphac) return i - 1; else rig -= alphac; }
-    else         { if (rig < betac) return i + 1; else rig -= betac; }
-    if (((i / 2) % 2)) { if (rig < m.alphad) return i - 2; else rig -= m.alphad; }
-    else               { if (rig < m.betad) return i + 2; else rig -= m.betad; }
-    if (((i / 4) % 2)) { if (rig < m.alphaf) return i - 4; else rig -= m.alphaf; }
-    else               { if (rig < m.betaf) return i + 4; else rig -= m.betaf; }
-    if (((i / 8) % 2)) { if (rig < alphafca) return i - 8; else rig -= alphafca; }
-    else               { if (rig < betafca) return i + 8; else rig -= betafca; }
-    return i;
+    const bool c1 = rig < r1;
+    rig -= r1;
+    const bool c2 = rig < r2;
+    rig -= r2;
+    const bool c4 = rig < r4;
+    rig -= r4;
+    const bool c8 = rig < r8;
+    const int d = c1 ? 1 : c2 ? 2 : c4 ? 4 : c8 ? 8 : 0;
+    return (i & d) ? i - d : i + d;
 }
 
 /* Transition t's count once its first uniform u0 (static slot 8 + t) has
@@ -368,8 +379,17 @@ RYR_DEV double csqn_mono(double cjsr, double *roo2_out)
     return (-1.0 + __builtin_sqrt(1.0 + 8.0 * 460.0 * roo2)) / (4.0 * roo2 * 460.0);
 }
 
+/* The partner lane of a CRU that two lanes share (lane ^ 1, a quad
+ * permutation). */
+RYR_DEV uint32_t lane_pair_u32(uint32_t x)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, true);
+}
+
 /* subroutine_cuda.c:251-669; counts in/out: nou, ncu, nob, ncb; mono =
- * csqn_mono(cjsr); *nd counts the draws.
+ * csqn_mono(cjsr); *nd counts the draws.  With LPC = 2 two adjacent lanes
+ * share the CRU: lane half h samples the transitions t = h, h + 2, ... and
+ * the counts are exchanged before they are combined.
  *
  * The eight transitions draw from their own addresses (philox.h), so they are
  * sampled in two passes rather than in program order.  Pass 1 draws every
@@ -381,9 +401,10 @@ RYR_DEV double csqn_mono(double cjsr, double *roo2_out)
  * which each lane takes its next unsettled transition, so one exponential and
  * one Knuth loop per round serve every lane, whichever transitions they
  * have left.  The counts equal those of the sequential reference loop. */
+template <int LPC = 1>
 RYR_DEV void ryr_gating(uint64_t seed, uint32_t cru, uint64_t step, double Ku, double Ku2, double Kb, double cp,
                         double cjsr, double mono, double dt, int &nou, int &ncu, int &nob, int &ncb, int *fails,
-                        int *nd)
+                        int *nd, int h = 0)
 {
     RYR_STRICT_FP
     double q1 = (5000 / cjsr) * (5000 / cjsr);
@@ -416,6 +437,7 @@ RYR_DEV void ryr_gating(uint64_t seed, uint32_t cru, uint64_t step, double Ku, d
     uint32_t pend = 0u;
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
+        if (LPC > 1 && (t & (LPC - 1)) != h) continue;       /* the partner lane's transition */
         const int n = pop[src[t]];
         const double p = pt[t];
         if (n <= 1 || p < 0.2 || (n <= 5 && p < 0.3)) {
@@ -444,6 +466,11 @@ RYR_DEV void ryr_gating(uint64_t seed, uint32_t cru, uint64_t step, double Ku, d
         else khi |= k << (8 * (t - 4));
     }
     *nd += ndraw;
+    if constexpr (LPC == 2) {
+        /* the two lanes of a CRU hold disjoint byte fields */
+        klo |= lane_pair_u32(klo);
+        khi |= lane_pair_u32(khi);
+    }
 
 #define RYR_KT(t) ((int)((((t) < 4 ? klo : khi) >> (8 * ((t) & 3))) & 255u))
     int n_ou_cu = RYR_KT(0) > RYR_NRYR ? RYR_NRYR : RYR_KT(0);
@@ -604,8 +631,8 @@ RYR_DEV void mem_ina(const MemIn &I, int lane, double *sh)
     }
 }
 
-/* I_Kr (:507-529) and I_Ks (:534-559): xr, xs1, xs2, qks; the sub-membrane
- * mean csubt */
+/* I_Kr (:507-529) and I_Ks (:534-559): xr, xs1, xs2 (qks, which needs the
+ * sub-membrane mean, in mem_sums) */
 RYR_DEV void mem_ikr_iks(const MemIn &I, int lane, double *sh)
 {
     RYR_STRICT_FP
@@ -620,7 +647,6 @@ RYR_DEV void mem_ikr_iks(const MemIn &I, int lane, double *sh)
     case 3: n = v + 33.00; d = 22.40; break;
     case 4: n = -(v - 1.500); d = 16.700; break;
     case 7: n = xko + 0.0183300 * 140.0; d = xki + 0.0183300 * xnai; break;
-    case 8: n = I.sum_cs; d = I.dN; break;
     default: break;
     }
     const double q1 = n / d;
@@ -635,7 +661,6 @@ RYR_DEV void mem_ikr_iks(const MemIn &I, int lane, double *sh)
     }
     const double ex = ryr_exp(arg);
     const double lg = ryr_log(lane == 7 ? q1 : 1.0);
-    const double csubt = lanev(q1, 8);
 
     /* rates and steady states */
     const double e = ex;                         /* lane k uses its own exponential */
@@ -648,32 +673,25 @@ RYR_DEV void mem_ikr_iks(const MemIn &I, int lane, double *sh)
     case 4: n = 1.0; d = 1.0 + e; break;
     case 5: n = 0.0000719 * (v + 30.00); d = 1.00 - e; break;
     case 6: n = 0.0001310 * (v + 30.00); d = e - 1.00; break;
-    case 7: n = 0.28; d = csubt; break;
     default: break;
     }
     const double q2 = n / d;
     const double xkrv1 = lanev(q2, 0), xkrv2 = lanev(q2, 1);
     const double tx1 = lanev(q2, 5), tx2 = lanev(q2, 6);
-    /* slowncxchange (:540-543) doubles the Ca dependence and the time constant */
-    const double qnum = I.slowncx ? 1.6 : 0.8;
     n = 1.0; d = 1.0;
     switch (lane) {
     case 0: n = 1.00; d = xkrv1 + xkrv2; break;
     case 1: n = 1.00; d = tx1 + tx2; break;
-    case 2: n = qnum; d = 1 + ryr_pow(lanev(q2, 7), 3); break;
     default: break;
     }
     const double q3 = n / d;
     const double taukr = lanev(q3, 0);
     double tauxs = lanev(q3, 1);
     tauxs = 0.95 * tauxs;
-    const double qks_inf = 0.2 * (1 + lanev(q3, 2));
-    const double tauqks = I.slowncx ? 2000 : 1000.00;
     n = 1.0; d = 1.0;
     switch (lane) {
     case 0: n = -dt; d = taukr; break;
     case 1: n = -dt; d = tauxs; break;
-    case 2: n = dt * (qks_inf - qks); d = tauqks; break;
     default: break;
     }
     const double q4 = n / d;
@@ -692,10 +710,9 @@ RYR_DEV void mem_ikr_iks(const MemIn &I, int lane, double *sh)
     const double xr1 = xkrinf - (xkrinf - xr) * lanev(ex2, 0);
     const double xs11 = xs1ss - (xs1ss - xs1) * lanev(ex2, 1);
     const double xs21 = xs1ss - (xs1ss - xs2) * lanev(ex2, 1);
-    const double qks1 = qks + lanev(q4, 2);
     if (lane == 0) {
-        sh[RYR_MS_XR] = xr1; sh[RYR_MS_XS1] = xs11; sh[RYR_MS_XS2] = xs21; sh[RYR_MS_QKS] = qks1;
-        sh[RYR_MS_IKR] = xikr; sh[RYR_MS_IKS] = xiks; sh[RYR_MS_CSUBT] = csubt;
+        sh[RYR_MS_XR] = xr1; sh[RYR_MS_XS1] = xs11; sh[RYR_MS_XS2] = xs21;
+        sh[RYR_MS_IKR] = xikr; sh[RYR_MS_IKS] = xiks;
     }
 }
 
@@ -795,14 +812,7 @@ RYR_DEV void mem_inak(const MemIn &I, int lane, double *sh)
     const double xko = RYR_XKO;
     const double frt = RYR_FRT;
     const double xkmko = 1.50, xkmnai = 12.00, xibarnak = 1.5000, hh = 1.00;
-    double n = 1.0, d = 1.0;
-    switch (lane) {
-    case 0: n = I.sum_icat; d = I.Cm; break;
-    case 1: n = I.sum_inaca; d = I.Cm; break;
-    case 2: n = xkmnai; d = xnai; break;
-    default: break;
-    }
-    const double q1 = n / d;
+    const double q1 = xkmnai / xnai;
     double arg = 0.0;
     switch (lane) {
     case 0: arg = 140.0 / 67.30; break;
@@ -811,19 +821,45 @@ RYR_DEV void mem_inak(const MemIn &I, int lane, double *sh)
     default: break;
     }
     const double ex = ryr_exp(arg);
-    n = 1.0; d = 1.0;
+    double n = 1.0, d = 1.0;
     switch (lane) {
     case 0: n = ex - 1.00; d = 7.00; break;                                 /* sigma */
-    case 1: n = 1.0; d = 1.0 + ryr_pow(lanev(q1, 2), hh); break;
+    case 1: n = 1.0; d = 1.0 + ryr_pow(q1, hh); break;
     default: break;
     }
     const double q2 = n / d;
     const double sigma = lanev(q2, 0);
     const double fnak = 1.00 / (1.0 + 0.1245 * lanev(ex, 1) + 0.0365 * sigma * lanev(ex, 2));
     const double xinak = 1.0 * xibarnak * fnak * lanev(q2, 1) * xko / (xko + xkmko);
-    const double xicatto = lanev(q1, 0) * 0.0965 * (0.00126) * 2.0 * ((0.5) + 1.);
-    const double xinacato = lanev(q1, 1) * 0.0965 * (0.025);
-    if (lane == 0) { sh[RYR_MS_ICAT] = xicatto; sh[RYR_MS_INACA] = xinacato; sh[RYR_MS_INAK] = xinak; }
+    if (lane == 0) sh[RYR_MS_INAK] = xinak;
+}
+
+/* The parts that need the whole-cell sums of the CRU kernel: the
+ * sub-membrane mean csubt and the I_Ks Ca gate qks (:534-559), and the
+ * whole-cell Ca currents I_CaL and I_NCX (:447-456) */
+RYR_DEV void mem_sums(const MemIn &I, int lane, double *sh)
+{
+    RYR_STRICT_FP
+    const double qks = I.s[7], dt = I.dt;
+    double n = 1.0, d = 1.0;
+    switch (lane) {
+    case 0: n = I.sum_cs; d = I.dN; break;
+    case 1: n = I.sum_icat; d = I.Cm; break;
+    case 2: n = I.sum_inaca; d = I.Cm; break;
+    default: break;
+    }
+    const double q1 = n / d;
+    const double csubt = lanev(q1, 0);
+    /* slowncxchange (:540-543) doubles the Ca dependence and the time constant */
+    const double qks_inf = I.slowncx ? 0.2 * (1 + 1.6 / (1 + ryr_pow((0.28 / csubt), 3)))
+                                     : 0.2 * (1 + 0.8 / (1 + ryr_pow((0.28 / csubt), 3)));
+    const double tauqks = I.slowncx ? 2000 : 1000.00;
+    const double qks1 = qks + dt * (qks_inf - qks) / tauqks;
+    const double xicatto = lanev(q1, 1) * 0.0965 * (0.00126) * 2.0 * ((0.5) + 1.);
+    const double xinacato = lanev(q1, 2) * 0.0965 * (0.025);
+    if (lane == 0) {
+        sh[RYR_MS_CSUBT] = csubt; sh[RYR_MS_QKS] = qks1; sh[RYR_MS_ICAT] = xicatto; sh[RYR_MS_INACA] = xinacato;
+    }
 }
 
 /* [Na]i and V (:701-738) from the groups' currents; fills slots V and NAI */
@@ -1273,11 +1309,13 @@ RYR_DEV void fast_cleft(float dt, float cif0, float cif1, float csf0, float csu,
  * batches (throughput), are held to 168 VGPRs: three waves per SIMD, without
  * spilling.  The 256-lane instance, which it picks for single cells (one
  * wave per SIMD anyway), keeps the registers the allocator wants. */
-template <typename TF, typename TS, int WG, bool SCHB>
+template <typename TF, typename TS, int WG, bool SCHB, int LPC = 1>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu((WG >= 256 || SCHB) ? 1 : 3)))
 void k_cru_step(RyrKArgs a, int kc)
 {
-    constexpr int SUB = RYR_BLOCK / WG;
+    static_assert(LPC == 1 || (LPC == 2 && WG >= 128), "paired lanes need two waves per workgroup");
+    constexpr int CPW = WG / LPC;          /* CRUs per workgroup */
+    constexpr int SUB = RYR_BLOCK / CPW;
     const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     const int bid = L / SUB;               /* 256-CRU chunk */
     const int part = L - bid * SUB;
@@ -1289,7 +1327,9 @@ void k_cru_step(RyrKArgs a, int kc)
     const int spo = a.spo;
     const bool reset = (nstep == 0) || ((nstep - 1) % spo == 0);
     const bool diag = (nstep % spo) == 0;
-    const int n = (bid - C.first_block) * RYR_BLOCK + part * WG + (int)threadIdx.x;
+    const int n = (bid - C.first_block) * RYR_BLOCK + part * CPW + (int)threadIdx.x / LPC;
+    const int hl = LPC == 1 ? 0 : (int)threadIdx.x & (LPC - 1);   /* lane half of a shared CRU */
+    const bool lead = hl == 0;                                     /* the lane that stores */
     const bool active = n < C.N;
     const long long P = a.P;
     const double dt = a.dt;
@@ -1351,14 +1391,40 @@ void k_cru_step(RyrKArgs a, int kc)
                 int nlcp = 0;
                 uint32_t nw = lw;
                 ryr_u32x4 lb = {{0u, 0u, 0u, 0u}};
-                for (int ll = 0; ll < 8 && ((lmask >> ll) & 1u); ++ll) {
-                    int st = (int)((lw >> (4 * ll)) & 15u);
-                    if (!(ll & 1)) lb = ryr_block(C.seed, (uint32_t)n, (uint64_t)nstep, (uint32_t)(ll >> 1));
-                    const double ragg = (ll & 1) ? ryr_slot_uniform(lb.v[2], lb.v[3]) : ryr_slot_uniform(lb.v[0], lb.v[1]);
-                    int jj = lcc_step(st, cpx, dt, ragg, *M, betac);
-                    nw = (nw & ~(15u << (4 * ll))) | ((uint32_t)jj << (4 * ll));
-                    if (jj == 0) { ++nl; ++nlcp; }
+                if constexpr (LPC == 1) {
+                    const int npre = __builtin_ctz(~lmask);          /* present slots before the first absent */
+#pragma unroll
+                    for (int ll = 0; ll < 8; ++ll) {
+                        if (ll < npre) {
+                            int st = (int)((lw >> (4 * ll)) & 15u);
+                            if (!(ll & 1)) lb = ryr_block(C.seed, (uint32_t)n, (uint64_t)nstep, (uint32_t)(ll >> 1));
+                            const double ragg = (ll & 1) ? ryr_slot_uniform(lb.v[2], lb.v[3]) : ryr_slot_uniform(lb.v[0], lb.v[1]);
+                            int jj = lcc_step(st, cpx, dt, ragg, *M, betac);
+                            nw = (nw & ~(15u << (4 * ll))) | ((uint32_t)jj << (4 * ll));
+                            if (jj == 0) ++nlcp;
+                        }
+                    }
+                } else {
+                    /* channels of this lane's parity among the present slots
+                     * before the first absent one */
+                    const int npre = __builtin_ctz(~lmask);
+                    for (int ll = hl; ll < npre; ll += 2) {
+                        int st = (int)((lw >> (4 * ll)) & 15u);
+                        lb = ryr_block(C.seed, (uint32_t)n, (uint64_t)nstep, (uint32_t)(ll >> 1));
+                        const double ragg = (ll & 1) ? ryr_slot_uniform(lb.v[2], lb.v[3]) : ryr_slot_uniform(lb.v[0], lb.v[1]);
+                        int jj = lcc_step(st, cpx, dt, ragg, *M, betac);
+                        nw = (nw & ~(15u << (4 * ll))) | ((uint32_t)jj << (4 * ll));
+                        if (jj == 0) ++nlcp;
+                    }
+                    /* the partner lane gated the other parity's channels */
+                    uint32_t pm = 0u;
+#pragma unroll
+                    for (int ll = 0; ll < 8; ++ll)
+                        if (ll < npre && (ll & 1) != hl) pm |= 15u << (4 * ll);
+                    nw = (nw & ~pm) | (lane_pair_u32(nw) & pm);
+                    nlcp += (int)lane_pair_u32((uint32_t)nlcp);
                 }
+                nl += nlcp;
                 lw = nw;
                 double X_ = cpx / 1000.0 * M->ghk_e2 - (1 * 1.8);
                 double ica = M->ghk_small ? 2.0 * 11.9 * 96.485 * 0.341 * X_ : M->ghk_A * X_ / M->ghk_D;
@@ -1378,8 +1444,9 @@ void k_cru_step(RyrKArgs a, int kc)
         double roo2;
         const double mono = csqn_mono((double)cjsrv, &roo2);
         int ndraw = 0;
-        ryr_gating(C.seed, (uint32_t)n, (uint64_t)nstep, C.Ku, C.Ku2, C.Kb, cp_d, (double)cjsrv, mono, dt, nou, ncu,
-                   nob, ncb, &nfail, &ndraw);
+        ryr_gating<LPC>(C.seed, (uint32_t)n, (uint64_t)nstep, C.Ku, C.Ku2, C.Kb, cp_d, (double)cjsrv, mono, dt,
+                        nou, ncu, nob, ncb, &nfail, &ndraw, hl);
+        if constexpr (LPC == 2) nfail += (int)lane_pair_u32((uint32_t)nfail);
         if (prevsp < 5 && (nou + nob) >= 5) nspark = 1;
         double xire;
         {
@@ -1422,7 +1489,7 @@ void k_cru_step(RyrKArgs a, int kc)
             ancx_new = (TF)(Ka + Ancxdot * dt);
             xinaca = C.svncx * 1.0 * Inaca;
         }
-        ancx[g] = ancx_new;
+        if (lead) ancx[g] = ancx_new;
 
         /* exchange fluxes and JSR (:1309-1318) */
         const TF cpT = cpv, csT = csv;
@@ -1467,7 +1534,7 @@ void k_cru_step(RyrKArgs a, int kc)
             fast_cleft<!SCHB>((float)dt, (float)cil[0], (float)cil[1], (float)csv, (float)cs_up,
                        (float)cs_dn, (float)icat1, (float)xinaca, (float)xire, (float)bcgcur, perm, (float)C.ci_bath,
                        cpf, csf, sar, sarh, sarj, sarhj, dci[0], dci[1]);
-            if (SCHB) {
+            if (SCHB && lead) {
                 ci1[g] = (TF)fmaxf((float)cil[0] + dci[0] * (float)dt, 0.0f);
                 ci1[P + g] = (TF)fmaxf((float)cil[1] + dci[1] * (float)dt, 0.0f);
             }
@@ -1555,48 +1622,64 @@ void k_cru_step(RyrKArgs a, int kc)
             for (int ii = 0; ii < 2; ++ii) {
                 TF cin = cil[ii] + dotci01[ii] * dtT;
                 if (cin < TF(0)) cin = TF(0);
-                ci1[ii * P + g] = cin;
+                if (lead) ci1[ii * P + g] = cin;
             }
         }
         }   /* exact path */
         if (cpv < TF(0)) { cpv = TF(0); ++nclampcp; }
         if (csn < TF(0)) { csn = TF(1e-6); ++nclampcs; }
-        casar[g] = v_casar; casarh[g] = v_casarh; casarj[g] = v_casarj; casarhj[g] = v_casarhj;
-        cp1[g] = cpv;
-        cs1[g] = csn;
-        cjsr1[g] = cjsrv + dotcjsr * (TS)dtT;
-        cjsrbp[g] = cjsrb_new;
-        a.lccw[g] = lw;
-        a.ryrw[g] = (uint32_t)nou | ((uint32_t)ncu << 8) | ((uint32_t)nob << 16) | ((uint32_t)ncb << 24);
-        a.aux[g] = (uint32_t)(nl & 0xffff) | ((uint32_t)nspark << 16) | (lmask << RYR_AUX_MASK_SHIFT);
-        if (diag) {
-            a.d_xire[g] = xire;
-            a.d_xinaca[g] = xinaca;
+        if (lead) {
+            casar[g] = v_casar; casarh[g] = v_casarh; casarj[g] = v_casarj; casarhj[g] = v_casarhj;
+            cp1[g] = cpv;
+            cs1[g] = csn;
+            cjsr1[g] = cjsrv + dotcjsr * (TS)dtT;
+            cjsrbp[g] = cjsrb_new;
+            a.lccw[g] = lw;
+            a.ryrw[g] = (uint32_t)nou | ((uint32_t)ncu << 8) | ((uint32_t)nob << 16) | ((uint32_t)ncb << 24);
+            a.aux[g] = (uint32_t)(nl & 0xffff) | ((uint32_t)nspark << 16) | (lmask << RYR_AUX_MASK_SHIFT);
+            if (diag) {
+                a.d_xire[g] = xire;
+                a.d_xinaca[g] = xinaca;
+            }
+            if (nfail | nclampcp | nclampcs) {
+                unsigned long long *cnt = a.counters + (long long)c * RYR_NCNT;
+                if (nfail) atomicAdd(cnt + 0, (unsigned long long)nfail);
+                if (nclampcp) atomicAdd(cnt + 1, 1ull);
+                if (nclampcs) atomicAdd(cnt + 2, 1ull);
+            }
+            r_icat = xicat;
+            r_inaca = xinaca;
+            r_cs = (double)csn;
         }
-        if (nfail | nclampcp | nclampcs) {
-            unsigned long long *cnt = a.counters + (long long)c * RYR_NCNT;
-            if (nfail) atomicAdd(cnt + 0, (unsigned long long)nfail);
-            if (nclampcp) atomicAdd(cnt + 1, 1ull);
-            if (nclampcs) atomicAdd(cnt + 2, 1ull);
-        }
-        r_icat = xicat;
-        r_inaca = xinaca;
-        r_cs = (double)csn;
     }
 
-    /* whole-cell sums: pairwise double-double tree over each wave's 64 CRUs
-     * (index order), one partial per wave (4 per chunk, whatever WG is);
-     * k_membrane pairs them into chunk sums and adds those */
+    /* whole-cell sums: pairwise double-double tree over each group of 64
+     * CRUs (index order), one partial per group (4 per chunk, whatever the
+     * shape); k_membrane pairs them into chunk sums and adds those.  With
+     * paired lanes a wave holds 32 CRUs at its even lanes and zeros at the odd
+     * ones (adding an exact zero changes no double-double), so its tree is the
+     * group tree's lower or upper half, and the two halves meet in LDS. */
     {
         const int lane = threadIdx.x & 63;
-        const int w = part * (WG / 64) + ((int)threadIdx.x >> 6);
+        const int wave = (int)threadIdx.x >> 6;
         RyrDD t0 = {r_icat, 0.0}, t1 = {r_inaca, 0.0}, t2 = {r_cs, 0.0};
         t0 = wave_tree(t0);
         t1 = wave_tree(t1);
         t2 = wave_tree(t2);
-        if (lane == 0) {
-            RyrDD *pp = a.partial + 12LL * bid + 3 * w;
-            pp[0] = t0; pp[1] = t1; pp[2] = t2;
+        if constexpr (LPC == 1) {
+            if (lane == 0) {
+                RyrDD *pp = a.partial + 12LL * bid + 3 * (part * (WG / 64) + wave);
+                pp[0] = t0; pp[1] = t1; pp[2] = t2;
+            }
+        } else {
+            __shared__ RyrDD s_half[WG / 64][3];
+            if (lane == 0) { s_half[wave][0] = t0; s_half[wave][1] = t1; s_half[wave][2] = t2; }
+            __syncthreads();
+            if (lane == 0 && !(wave & 1)) {
+                RyrDD *pp = a.partial + 12LL * bid + 3 * ((part * (WG / 64) + wave) / 2);
+#pragma unroll
+                for (int q = 0; q < 3; ++q) pp[q] = dd_add(s_half[wave][q], s_half[wave + 1][q]);
+            }
         }
     }
 }
@@ -1621,6 +1704,36 @@ RYR_DEV void membrane_block(const RyrKArgs &a, int c, long long nstep, double st
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     RyrDD acc[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
     const RyrDD *pp = a.partial + 12LL * C.first_block;
+    __shared__ double s_ms[RYR_MS_N];
+    MemIn I;
+    I.s = M->m; I.svikr = C.svikr; I.Cm = C.Cm; I.dN = C.dN; I.dt = dt;
+    I.slowncx = (C.variants & RYR_VAR_SLOWNCX) != 0;
+
+    /* this wave's first slice of wave partials is loaded before the current
+     * groups run, so that the loads' latency hides behind them */
+    RyrDD w0[3][4];
+    {
+        const int bb = wv * 64 + lane;
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (bb < C.nblocks) w0[q][k] = pp[12LL * bb + q + 3 * k];
+                else { w0[q][k].hi = 0.0; w0[q][k].lo = 0.0; }
+            }
+    }
+
+    /* the four current groups (none needs the whole-cell sums), group g on
+     * wave g mod NW */
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        if (g % NW != wv) continue;
+        if (g == 0) mem_ina(I, lane, s_ms);
+        else if (g == 1) mem_ikr_iks(I, lane, s_ms);
+        else if (g == 2) mem_ik1_ito(I, lane, s_ms);
+        else mem_inak(I, lane, s_ms);
+    }
+
     /* groups of 256 chunk sums, each a pairwise tree of four 64-chunk wave
      * trees (the same tree whatever the number of waves WG gives) */
     for (int g0 = 0; g0 < C.nblocks; g0 += 256) {
@@ -1629,8 +1742,10 @@ RYR_DEV void membrane_block(const RyrKArgs &a, int c, long long nstep, double st
             RyrDD t[3];
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
-                if (bb < C.nblocks) {
-                    /* chunk sum = pairwise tree over its four wave partials */
+                /* chunk sum = pairwise tree over its four wave partials */
+                if (g0 == 0 && sl == wv) {
+                    t[q] = dd_add(dd_add(w0[q][0], w0[q][1]), dd_add(w0[q][2], w0[q][3]));
+                } else if (bb < C.nblocks) {
                     const RyrDD *w = pp + 12LL * bb + q;
                     t[q] = dd_add(dd_add(w[0], w[3]), dd_add(w[6], w[9]));
                 } else { t[q].hi = 0.0; t[q].lo = 0.0; }
@@ -1644,25 +1759,11 @@ RYR_DEV void membrane_block(const RyrKArgs &a, int c, long long nstep, double st
             acc[q] = dd_add(acc[q], dd_add(dd_add(s_red[q][0], s_red[q][1]), dd_add(s_red[q][2], s_red[q][3])));
         __syncthreads();
     }
-
-    /* the four current groups, group g on wave g mod NW */
-    __shared__ double s_ms[RYR_MS_N];
-    {
-        MemIn I;
-        I.s = M->m; I.svikr = C.svikr; I.Cm = C.Cm; I.dN = C.dN;
-        I.sum_icat = acc[0].hi; I.sum_inaca = acc[1].hi; I.sum_cs = acc[2].hi; I.dt = dt;
-        I.slowncx = (C.variants & RYR_VAR_SLOWNCX) != 0;
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            if (g % NW != wv) continue;
-            if (g == 0) mem_ina(I, lane, s_ms);
-            else if (g == 1) mem_ikr_iks(I, lane, s_ms);
-            else if (g == 2) mem_ik1_ito(I, lane, s_ms);
-            else mem_inak(I, lane, s_ms);
-        }
-    }
-    __syncthreads();
     if (wv != 0) return;
+
+    /* wave 0: the parts that need the sums, then [Na]i and V */
+    I.sum_icat = acc[0].hi; I.sum_inaca = acc[1].hi; I.sum_cs = acc[2].hi;
+    mem_sums(I, lane, s_ms);
 
     const double stim = stim_of(C, nstep, stim_amp);
     double mo[13], cu[10];

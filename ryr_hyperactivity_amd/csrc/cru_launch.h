@@ -73,6 +73,8 @@ static hipError_t ryr_go_cru_s(const RyrKArgs &a, int nchunks, int wg, int kc, h
     case 64: RYR_GO((k_cru_step<TF, TS, 64, SCHB>), dim3((unsigned)nchunks * 4u), dim3(64), s, ev, a, kc); break;
     case 128: RYR_GO((k_cru_step<TF, TS, 128, SCHB>), dim3((unsigned)nchunks * 2u), dim3(128), s, ev, a, kc); break;
     case 256: RYR_GO((k_cru_step<TF, TS, 256, SCHB>), dim3((unsigned)nchunks), dim3(256), s, ev, a, kc); break;
+    /* 512 lanes per 256-CRU chunk: two 256-lane workgroups, two lanes per CRU */
+    case 512: RYR_GO((k_cru_step<TF, TS, 256, SCHB, 2>), dim3((unsigned)nchunks * 2u), dim3(256), s, ev, a, kc); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -576,6 +576,10 @@ static void sched_candidates(const ryr_handle *h, std::vector<std::array<int, 3>
     /* fused, the membrane's four current groups on four waves: the sub-voxel
      * size timed alone does not see that */
     add(256, h->wg_cru, RYR_SCHED_FUSED);
+    /* two lanes per CRU: the gating and LCC loops split, for latency-bound
+     * (small) grids */
+    add(256, 512, RYR_SCHED_FUSED);
+    add(h->wg_cyto, 512, RYR_SCHED_SEQ);
     if (nsched(h) > 2) {
         add(h->wg_cyto, h->wg_cru, RYR_SCHED_CONCURRENT);
         add(128, 64, RYR_SCHED_CONCURRENT);
@@ -765,13 +769,14 @@ extern "C" int ryr_set_launch_shape(ryr_handle *h, int wg_cyto, int wg_cru, int 
 {
     if (!h) return RYR_E_ARG;
     auto ok = [](int w) { return w == 64 || w == 128 || w == 256; };
+    auto ok_cru = [&](int w) { return ok(w) || w == 512; };
     if (wg_cyto == 0 && wg_cru == 0) {
         h->wg_cyto = h->wg_cru = 0;
         h->sched = -1;
         h->tune_i = 0;
         h->cand.clear();
         for (auto &k : h->tune_ms) for (auto &q : k) q.clear();
-    } else if (ok(wg_cyto) && ok(wg_cru) && sched >= 0 && sched < nsched(h)) {
+    } else if (ok(wg_cyto) && ok_cru(wg_cru) && sched >= 0 && sched < nsched(h)) {
         h->wg_cyto = wg_cyto;
         h->wg_cru = wg_cru;
         h->sched = sched;

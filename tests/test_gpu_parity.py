@@ -195,11 +195,12 @@ def test_beats_and_clamp_match_the_oracle():
 
 
 @pytest.mark.parametrize("shape", [(64, 64, 1), (128, 256, 0), (256, 128, 1), (256, 256, 0), (64, 128, 2),
-                                   (256, 64, 2)])
+                                   (256, 64, 2), (256, 512, 1), (128, 512, 0), (64, 512, 2)])
 def test_every_launch_shape_is_bitexact(shape):
     """Workgroup sizes and step schedules change no value: FP64 against the
     oracle, and FP32 against the FP32 run at another shape (whole-cell sums
-    are formed per wave, so the reduction order is the same for every size;
+    are formed per 64-CRU group, so the reduction order is the same for every
+    size, two lanes per CRU (CRU size 512) included;
     the fused membrane step reads only the CRU kernel's sums; the concurrent
     kernels read only step-start values of each other's fields)."""
     print(f"case shape={shape}", flush=True)
