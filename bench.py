@@ -125,7 +125,7 @@ def run_gpu(p, cells, steps, warmup, prof_steps, shape=None):
         "ncru": ncru,
         "step_bytes": ncru * alg_bytes_per_cru_step(p.precision),
         "step_GBs": ncru * alg_bytes_per_cru_step(p.precision) / step_s / 1e9,
-        "launch_shape": {"k_cyto": shape[0], "k_cru_step": shape[1], "schedule": ["sequential", "membrane-fused", "concurrent"][shape[2]]},
+        "launch_shape": {"k_cyto": shape[0], "k_cru_step": shape[1], "schedule": ["sequential", "membrane-fused", "concurrent", "membrane-and-gating-fused"][shape[2]]},
         "kernels": kernels,
         "dominant": dom,
     }

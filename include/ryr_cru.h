@@ -261,6 +261,9 @@ int ryr_kernel_time(ryr_handle *h, double *ms, int64_t *launches);
  *                         launch of step n+1
  *   RYR_SCHED_CONCURRENT  the sub-voxel and CRU kernels of a step run side by
  *                         side on two streams (cleft scheme A only)
+ *   RYR_SCHED_FUSED_GATING as FUSED, and the RyR gating of step n (which reads
+ *                         only step-start values) rides in the sub-voxel launch
+ *                         of step n too, off the CRU kernel (CRU size 128/256)
  * No reference counterpart (the reference hard-codes one 48x448 launch,
  * :1701).  A new handle tunes the shape itself: its first 9 steps alternate
  * the sizes with each kernel timed by its own events (fastest median kept);
@@ -268,13 +271,14 @@ int ryr_kernel_time(ryr_handle *h, double *ms, int64_t *launches);
  * schedule with 256-lane sub-voxel workgroups, the two-lanes-per-CRU kernel,
  * and the concurrent schedule with smaller sizes) runs 12 steps as three replays of a
  * captured 4-step graph, timed by events, and the fastest is kept.  Output
- * steps are stepped over eagerly, so tuning takes about 100 steps.  Every
+ * steps are stepped over eagerly, so tuning takes about 150 steps.  Every
  * shape computes identical values, so tuning never changes a result.  Passing
  * sizes (0, 0) re-arms the tuning; the getter reports sizes 0 and schedule -1
  * while it is pending. */
 #define RYR_SCHED_SEQ        0
 #define RYR_SCHED_FUSED      1
 #define RYR_SCHED_CONCURRENT 2
+#define RYR_SCHED_FUSED_GATING 3
 int ryr_set_launch_shape(ryr_handle *h, int wg_cyto, int wg_cru, int sched);
 int ryr_get_launch_shape(const ryr_handle *h, int *wg_cyto, int *wg_cru, int *sched);
 

@@ -388,8 +388,8 @@ RYR_DEV uint32_t lane_pair_u32(uint32_t x)
 
 /* subroutine_cuda.c:251-669; counts in/out: nou, ncu, nob, ncb; mono =
  * csqn_mono(cjsr); *nd counts the draws.  With LPC = 2 two adjacent lanes
- * share the CRU: lane half h samples the transitions t = h, h + 2, ... and
- * the counts are exchanged before they are combined.
+ * share the CRU: lane h samples the transitions 4h .. 4h + 3 and the counts
+ * are exchanged before they are combined.
  *
  * The eight transitions draw from their own addresses (philox.h), so they are
  * sampled in two passes rather than in program order.  Pass 1 draws every
@@ -427,49 +427,63 @@ RYR_DEV void ryr_gating(uint64_t seed, uint32_t cru, uint64_t step, double Ku, d
     /* source population of transition t: 0 OU, 1 CU, 2 OB, 3 CB */
     const int pop[4] = {nou, ncu, nob, ncb};
     constexpr int src[8] = {0, 1, 0, 2, 3, 1, 2, 3};
-    double u0[8];
+    /* this lane's transitions t = TPL h + l, l < TPL: all eight, or with two
+     * lanes per CRU one half each (two of the four first-uniform blocks) */
+    constexpr int TPL = 8 / LPC;
+    double u0[TPL], pl[TPL];
+    int nl[TPL];
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-        const ryr_u32x4 w = ryr_block(seed, cru, step, (uint32_t)(RYR_SLOT_RYR0 / 2 + b));
+    for (int l = 0; l < TPL; ++l) {
+        pl[l] = (LPC == 1 || h == 0) ? pt[l] : pt[(TPL + l) & 7];
+        nl[l] = (LPC == 1 || h == 0) ? pop[src[l]] : pop[src[(TPL + l) & 7]];
+    }
+#pragma unroll
+    for (int b = 0; b < TPL / 2; ++b) {
+        const ryr_u32x4 w = ryr_block(seed, cru, step, (uint32_t)(RYR_SLOT_RYR0 / 2 + h * (TPL / 2) + b));
         u0[2 * b] = ryr_slot_uniform(w.v[0], w.v[1]);
         u0[2 * b + 1] = ryr_slot_uniform(w.v[2], w.v[3]);
     }
     uint32_t pend = 0u;
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
-        if (LPC > 1 && (t & (LPC - 1)) != h) continue;       /* the partner lane's transition */
-        const int n = pop[src[t]];
-        const double p = pt[t];
+    for (int l = 0; l < TPL; ++l) {
+        const int n = nl[l];
+        const double p = pl[l];
         if (n <= 1 || p < 0.2 || (n <= 5 && p < 0.3)) {
             const double x = (double)(n) * p;
             const double thr = (x >= 0x1p-20) ? 1.0 - x : 1.0 - 2.0 * x;
-            const bool zero = (n == 0 || p == 0.0) || (x >= 0x1p-40 && u0[t] < thr);
-            if (!zero) pend |= 1u << t;
+            const bool zero = (n == 0 || p == 0.0) || (x >= 0x1p-40 && u0[l] < thr);
+            if (!zero) pend |= 1u << l;
         } else {
-            pend |= 1u << t;
+            pend |= 1u << l;
         }
     }
     /* counts in byte fields (Knuth counts stay below 195 and Gaussian ones
      * below n p + 5 sqrt(-2 log 2^-53) + 1 < 150) */
-    uint32_t klo = 0u, khi = 0u;
-    int ndraw = 8;
+    uint32_t kw[LPC == 1 ? 2 : 1] = {};
+    int ndraw = TPL;
     while (pend) {
-        const int t = __builtin_ctz(pend);
+        const int l = __builtin_ctz(pend);
         pend &= pend - 1u;
-        int n = pop[0];
-        double p = pt[0], u = u0[0];
+        int n = nl[0];
+        double p = pl[0], u = u0[0];
 #pragma unroll
-        for (int q = 1; q < 8; ++q)
-            if (t == q) { n = pop[src[q]]; p = pt[q]; u = u0[q]; }
-        const uint32_t k = (uint32_t)ryr_transition_job(seed, cru, step, t, n, p, u, fails, &ndraw) & 255u;
-        if (t < 4) klo |= k << (8 * t);
-        else khi |= k << (8 * (t - 4));
+        for (int q = 1; q < TPL; ++q)
+            if (l == q) { n = nl[q]; p = pl[q]; u = u0[q]; }
+        const uint32_t k =
+            (uint32_t)ryr_transition_job(seed, cru, step, TPL * h + l, n, p, u, fails, &ndraw) & 255u;
+        if (LPC == 1 && l >= 4) kw[LPC == 1 ? 1 : 0] |= k << (8 * (l - 4));
+        else kw[0] |= k << (8 * (l & 3));
     }
     *nd += ndraw;
-    if constexpr (LPC == 2) {
-        /* the two lanes of a CRU hold disjoint byte fields */
-        klo |= lane_pair_u32(klo);
-        khi |= lane_pair_u32(khi);
+    uint32_t klo, khi;
+    if constexpr (LPC == 1) {
+        klo = kw[0];
+        khi = kw[LPC == 1 ? 1 : 0];
+    } else {
+        /* lane 0 holds transitions 0-3, lane 1 transitions 4-7 */
+        const uint32_t other = lane_pair_u32(kw[0]);
+        klo = h ? other : kw[0];
+        khi = h ? kw[0] : other;
     }
 
 #define RYR_KT(t) ((int)((((t) < 4 ? klo : khi) >> (8 * ((t) & 3))) & 255u))
@@ -1055,21 +1069,36 @@ RYR_DEV float fbuf(float b, float c, float dt, float sat, float kon, float koff,
  * membrane code's registers, hence a separate instance. */
 template <typename TF, typename TS, int WG>
 RYR_DEV void membrane_block(const RyrKArgs &a, int c, long long nstep, double stim_amp);
+#define RYR_GATE_LPC 2   /* lanes per CRU in the gating part: each takes four transitions */
+template <typename TF, typename TS, int WG>
+RYR_DEV void gating_block(const RyrKArgs &a, int L, long long nstep);
 
-template <typename TF, typename TS, int WG, bool FUSE>
+/* MODE bit 0: the first ncells logical workgroups run the membrane step
+ * gstep + km; bit 1: the next nchunks * 256/WG * RYR_GATE_LPC run the RyR
+ * gating of step gstep + kc (gating_block).  The sub-voxel sweep follows. */
+template <typename TF, typename TS, int WG, int MODE>
 __global__ __launch_bounds__(WG) void k_cyto(RyrKArgs a, int kc, int km, double stim_amp)
 {
     constexpr int SUB = RYR_BLOCK / WG;
     const long long base = *a.gstep;
-    const int nmem = FUSE ? a.ncells : 0;
+    const int nmem = (MODE & 1) ? a.ncells : 0;
     const int Lg = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-    if constexpr (FUSE) {
+    if constexpr ((MODE & 1) != 0) {
         if (Lg < nmem) {
             membrane_block<TF, TS, WG>(a, Lg, base + km, stim_amp);
             return;
         }
     }
-    const int L = Lg - nmem;
+    int L = Lg - nmem;
+    if constexpr ((MODE & 2) != 0) {
+        /* RYR_GATE_LPC gating workgroups per 8 sub-voxel ones */
+        const int ngate = ((int)gridDim.x - nmem) / (8 + RYR_GATE_LPC) * RYR_GATE_LPC;
+        if (L < ngate) {
+            gating_block<TF, TS, WG>(a, L, base + kc);
+            return;
+        }
+        L -= ngate;
+    }
     const int b = L / (8 * SUB);
     const int r = L - b * (8 * SUB);
     const int ii = r & 7;
@@ -1304,16 +1333,61 @@ RYR_DEV void fast_cleft(float dt, float cif0, float cif1, float csf0, float csu,
     }
 }
 
+/* ------------------------------------------------------------ RyR gating */
+/* The RyR gating of step nstep for WG CRUs (cuda_Cell_Detailed.cu:1274-1290,
+ * subroutine_cuda.c:251-669), as a part of the sub-voxel launch of the same
+ * step (schedule RYR_SCHED_FUSED_GATING).  It reads only step-start values
+ * that the CRU kernel of the previous step wrote last (cp, cjsr, the RyR
+ * counts), so it runs beside the sub-voxel sweep and the previous membrane
+ * step, and the CRU kernel of the step (GSPLIT) takes the new counts.  It
+ * also clears the output interval's LCC-open count and spark flag at the
+ * interval's first step, as the CRU kernel does otherwise, then records a
+ * spark (open channels rising through 5). */
+template <typename TF, typename TS, int WG>
+RYR_DEV void gating_block(const RyrKArgs &a, int L, long long nstep)
+{
+    constexpr int LPC = RYR_GATE_LPC;
+    constexpr int CPW = WG / LPC;
+    constexpr int SUBG = RYR_BLOCK / CPW;
+    const int bid = L / SUBG;
+    const int part = L - bid * SUBG;
+    const RyrCellInfo C = a.chunks[bid];
+    const int n = (bid - C.first_block) * RYR_BLOCK + part * CPW + (int)threadIdx.x / LPC;
+    const int h = (int)threadIdx.x & (LPC - 1);
+    if (n >= C.N) return;                  /* both lanes of a CRU together */
+    const long long g = C.off + n;
+    const int par = (int)(nstep & 1);
+    const double cp_d = (double)((const TF *)a.cp[par])[g];
+    const double cj = (double)((const TS *)a.cjsr[par])[g];
+    const uint32_t rw = a.ryrw[g];
+    uint32_t ax = a.aux[g];
+    if ((nstep == 0) || ((nstep - 1) % a.spo == 0)) ax &= ~0x1ffffu;
+    int nou = (int)(rw & 255u), ncu = (int)((rw >> 8) & 255u), nob = (int)((rw >> 16) & 255u), ncb = (int)(rw >> 24);
+    const int prevsp = nou + nob;
+    double roo2;
+    const double mono = csqn_mono(cj, &roo2);
+    int nfail = 0, ndraw = 0;
+    ryr_gating<LPC>(C.seed, (uint32_t)n, (uint64_t)nstep, C.Ku, C.Ku2, C.Kb, cp_d, cj, mono, a.dt, nou, ncu, nob,
+                    ncb, &nfail, &ndraw, h);
+    if constexpr (LPC == 2) nfail += (int)lane_pair_u32((uint32_t)nfail);
+    if (h != 0) return;
+    if (prevsp < 5 && (nou + nob) >= 5) ax |= 1u << 16;
+    a.ryrw[g] = (uint32_t)nou | ((uint32_t)ncu << 8) | ((uint32_t)nob << 16) | ((uint32_t)ncb << 24);
+    a.aux[g] = ax;
+    if (nfail) atomicAdd(a.counters + (long long)C.cell * RYR_NCNT + 0, (unsigned long long)nfail);
+}
+
 /* ------------------------------------------------------------ step kernel */
 /* The 64- and 128-lane scheme-A instances, which the shape tuner picks for
  * batches (throughput), are held to 168 VGPRs: three waves per SIMD, without
  * spilling.  The 256-lane instance, which it picks for single cells (one
  * wave per SIMD anyway), keeps the registers the allocator wants. */
-template <typename TF, typename TS, int WG, bool SCHB, int LPC = 1>
+template <typename TF, typename TS, int WG, bool SCHB, int LPC = 1, bool GSPLIT = false>
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu((WG >= 256 || SCHB) ? 1 : 3)))
 void k_cru_step(RyrKArgs a, int kc)
 {
     static_assert(LPC == 1 || (LPC == 2 && WG >= 128), "paired lanes need two waves per workgroup");
+    static_assert(!(GSPLIT && LPC > 1), "with the gating split off, one lane per CRU");
     constexpr int CPW = WG / LPC;          /* CRUs per workgroup */
     constexpr int SUB = RYR_BLOCK / CPW;
     const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
@@ -1364,7 +1438,7 @@ void k_cru_step(RyrKArgs a, int kc)
         TF cpv = cp0[g];
         const TS cjsrv = cjsr0[g];
         uint32_t lw = a.lccw[g], rw = a.ryrw[g], ax = a.aux[g];
-        if (reset) ax &= ~0x1ffffu;                 /* nl and spark flag; the LCC slots stay */
+        if (!GSPLIT && reset) ax &= ~0x1ffffu;      /* nl and spark flag; the LCC slots stay (GSPLIT: done by the gating) */
         int nl = (int)(ax & 0xffffu), nspark = (int)((ax >> 16) & 1u);
         uint32_t lmask = (ax >> RYR_AUX_MASK_SHIFT) & 0xffu;
         int nfail = 0, nclampcp = 0, nclampcs = 0;
@@ -1443,11 +1517,13 @@ void k_cru_step(RyrKArgs a, int kc)
         const int prevsp = nou + nob;
         double roo2;
         const double mono = csqn_mono((double)cjsrv, &roo2);
-        int ndraw = 0;
-        ryr_gating<LPC>(C.seed, (uint32_t)n, (uint64_t)nstep, C.Ku, C.Ku2, C.Kb, cp_d, (double)cjsrv, mono, dt,
-                        nou, ncu, nob, ncb, &nfail, &ndraw, hl);
-        if constexpr (LPC == 2) nfail += (int)lane_pair_u32((uint32_t)nfail);
-        if (prevsp < 5 && (nou + nob) >= 5) nspark = 1;
+        if constexpr (!GSPLIT) {                    /* GSPLIT: the counts are this step's already */
+            int ndraw = 0;
+            ryr_gating<LPC>(C.seed, (uint32_t)n, (uint64_t)nstep, C.Ku, C.Ku2, C.Kb, cp_d, (double)cjsrv, mono, dt,
+                            nou, ncu, nob, ncb, &nfail, &ndraw, hl);
+            if constexpr (LPC == 2) nfail += (int)lane_pair_u32((uint32_t)nfail);
+            if (prevsp < 5 && (nou + nob) >= 5) nspark = 1;
+        }
         double xire;
         {
             RYR_STRICT_FP
@@ -1635,7 +1711,8 @@ void k_cru_step(RyrKArgs a, int kc)
             cjsr1[g] = cjsrv + dotcjsr * (TS)dtT;
             cjsrbp[g] = cjsrb_new;
             a.lccw[g] = lw;
-            a.ryrw[g] = (uint32_t)nou | ((uint32_t)ncu << 8) | ((uint32_t)nob << 16) | ((uint32_t)ncb << 24);
+            if (!GSPLIT)
+                a.ryrw[g] = (uint32_t)nou | ((uint32_t)ncu << 8) | ((uint32_t)nob << 16) | ((uint32_t)ncb << 24);
             a.aux[g] = (uint32_t)(nl & 0xffff) | ((uint32_t)nspark << 16) | (lmask << RYR_AUX_MASK_SHIFT);
             if (diag) {
                 a.d_xire[g] = xire;

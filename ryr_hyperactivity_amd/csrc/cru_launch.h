@@ -17,8 +17,8 @@
 
 #define RYR_DECLARE_LAUNCHERS(sfx)                                                               \
     hipError_t ryr_launch_cyto_##sfx(const RyrKArgs &a, int nchunks, int wg, int kc, int km,     \
-                                     double amp, hipStream_t s, hipEvent_t *ev);                \
-    hipError_t ryr_launch_cru_##sfx(const RyrKArgs &a, int nchunks, int wg, int kc,              \
+                                     bool gate, double amp, hipStream_t s, hipEvent_t *ev);     \
+    hipError_t ryr_launch_cru_##sfx(const RyrKArgs &a, int nchunks, int wg, int kc, bool split,  \
                                     hipStream_t s, hipEvent_t *ev);                             \
     hipError_t ryr_launch_mem_##sfx(const RyrKArgs &a, int ncells, int km, double amp,           \
                                     hipStream_t s, hipEvent_t *ev);                             \
@@ -42,28 +42,40 @@ RYR_DECLARE_LAUNCHERS(f32)
     } while (0)
 
 /* Sub-voxel kernel of step gstep+kc; with km >= 0 it also runs the membrane
- * step gstep+km of every cell in its first ncells workgroups. */
+ * step gstep+km of every cell in its first ncells workgroups, and with gate
+ * the RyR gating of step gstep+kc in the next nchunks * 256/wg * RYR_GATE_LPC. */
+template <typename TF, typename TS, int WG, int MODE>
+static hipError_t ryr_go_cyto_m(const RyrKArgs &a, int nchunks, int kc, int km, double amp, hipStream_t s,
+                                hipEvent_t *ev)
+{
+    const unsigned per = (unsigned)(RYR_BLOCK / WG) * (unsigned)nchunks;
+    const unsigned grid = ((MODE & 1) ? (unsigned)a.ncells : 0u) + per * ((MODE & 2) ? 8u + RYR_GATE_LPC : 8u);
+    RYR_GO((k_cyto<TF, TS, WG, MODE>), dim3(grid), dim3(WG), s, ev, a, kc, km, amp);
+    return hipGetLastError();
+}
+
+template <typename TF, typename TS, int MODE>
+static hipError_t ryr_go_cyto_w(const RyrKArgs &a, int nchunks, int wg, int kc, int km, double amp,
+                                hipStream_t s, hipEvent_t *ev)
+{
+    switch (wg) {
+    case 64: return ryr_go_cyto_m<TF, TS, 64, MODE>(a, nchunks, kc, km, amp, s, ev);
+    case 128: return ryr_go_cyto_m<TF, TS, 128, MODE>(a, nchunks, kc, km, amp, s, ev);
+    case 256: return ryr_go_cyto_m<TF, TS, 256, MODE>(a, nchunks, kc, km, amp, s, ev);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 template <typename TF, typename TS>
-static hipError_t ryr_go_cyto(const RyrKArgs &a, int nchunks, int wg, int kc, int km, double amp,
+static hipError_t ryr_go_cyto(const RyrKArgs &a, int nchunks, int wg, int kc, int km, bool gate, double amp,
                               hipStream_t s, hipEvent_t *ev)
 {
-    const unsigned nmem = km >= 0 ? (unsigned)a.ncells : 0u;
-    if (km >= 0) {
-        switch (wg) {
-        case 64: RYR_GO((k_cyto<TF, TS, 64, true>), dim3(nmem + (unsigned)nchunks * 32u), dim3(64), s, ev, a, kc, km, amp); break;
-        case 128: RYR_GO((k_cyto<TF, TS, 128, true>), dim3(nmem + (unsigned)nchunks * 16u), dim3(128), s, ev, a, kc, km, amp); break;
-        case 256: RYR_GO((k_cyto<TF, TS, 256, true>), dim3(nmem + (unsigned)nchunks * 8u), dim3(256), s, ev, a, kc, km, amp); break;
-        default: return hipErrorInvalidValue;
-        }
-    } else {
-        switch (wg) {
-        case 64: RYR_GO((k_cyto<TF, TS, 64, false>), dim3((unsigned)nchunks * 32u), dim3(64), s, ev, a, kc, km, amp); break;
-        case 128: RYR_GO((k_cyto<TF, TS, 128, false>), dim3((unsigned)nchunks * 16u), dim3(128), s, ev, a, kc, km, amp); break;
-        case 256: RYR_GO((k_cyto<TF, TS, 256, false>), dim3((unsigned)nchunks * 8u), dim3(256), s, ev, a, kc, km, amp); break;
-        default: return hipErrorInvalidValue;
-        }
+    switch ((km >= 0 ? 1 : 0) | (gate ? 2 : 0)) {
+    case 0: return ryr_go_cyto_w<TF, TS, 0>(a, nchunks, wg, kc, km, amp, s, ev);
+    case 1: return ryr_go_cyto_w<TF, TS, 1>(a, nchunks, wg, kc, km, amp, s, ev);
+    case 2: return ryr_go_cyto_w<TF, TS, 2>(a, nchunks, wg, kc, km, amp, s, ev);
+    default: return ryr_go_cyto_w<TF, TS, 3>(a, nchunks, wg, kc, km, amp, s, ev);
     }
-    return hipGetLastError();
 }
 
 template <typename TF, typename TS, bool SCHB>
@@ -80,23 +92,39 @@ static hipError_t ryr_go_cru_s(const RyrKArgs &a, int nchunks, int wg, int kc, h
     return hipGetLastError();
 }
 
-template <typename TF, typename TS>
-static hipError_t ryr_go_cru(const RyrKArgs &a, int nchunks, int wg, int kc, hipStream_t s, hipEvent_t *ev)
+/* the CRU kernel without the RyR gating (run by the sub-voxel launch) */
+template <typename TF, typename TS, bool SCHB>
+static hipError_t ryr_go_cru_split(const RyrKArgs &a, int nchunks, int wg, int kc, hipStream_t s, hipEvent_t *ev)
 {
+    switch (wg) {
+    case 128: RYR_GO((k_cru_step<TF, TS, 128, SCHB, 1, true>), dim3((unsigned)nchunks * 2u), dim3(128), s, ev, a, kc); break;
+    case 256: RYR_GO((k_cru_step<TF, TS, 256, SCHB, 1, true>), dim3((unsigned)nchunks), dim3(256), s, ev, a, kc); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+template <typename TF, typename TS>
+static hipError_t ryr_go_cru(const RyrKArgs &a, int nchunks, int wg, int kc, bool split, hipStream_t s,
+                             hipEvent_t *ev)
+{
+    if (split)
+        return a.scheme ? ryr_go_cru_split<TF, TS, true>(a, nchunks, wg, kc, s, ev)
+                        : ryr_go_cru_split<TF, TS, false>(a, nchunks, wg, kc, s, ev);
     return a.scheme ? ryr_go_cru_s<TF, TS, true>(a, nchunks, wg, kc, s, ev)
                     : ryr_go_cru_s<TF, TS, false>(a, nchunks, wg, kc, s, ev);
 }
 
 #define RYR_DEFINE_LAUNCHERS(sfx, TF, TS)                                                        \
     hipError_t ryr_launch_cyto_##sfx(const RyrKArgs &a, int nchunks, int wg, int kc, int km,     \
-                                     double amp, hipStream_t s, hipEvent_t *ev)                 \
+                                     bool gate, double amp, hipStream_t s, hipEvent_t *ev)      \
     {                                                                                            \
-        return ryr_go_cyto<TF, TS>(a, nchunks, wg, kc, km, amp, s, ev);                          \
+        return ryr_go_cyto<TF, TS>(a, nchunks, wg, kc, km, gate, amp, s, ev);                    \
     }                                                                                            \
-    hipError_t ryr_launch_cru_##sfx(const RyrKArgs &a, int nchunks, int wg, int kc,              \
+    hipError_t ryr_launch_cru_##sfx(const RyrKArgs &a, int nchunks, int wg, int kc, bool split,  \
                                     hipStream_t s, hipEvent_t *ev)                              \
     {                                                                                            \
-        return ryr_go_cru<TF, TS>(a, nchunks, wg, kc, s, ev);                                    \
+        return ryr_go_cru<TF, TS>(a, nchunks, wg, kc, split, s, ev);                             \
     }                                                                                            \
     hipError_t ryr_launch_mem_##sfx(const RyrKArgs &a, int ncells, int km, double amp,           \
                                     hipStream_t s, hipEvent_t *ev)                              \

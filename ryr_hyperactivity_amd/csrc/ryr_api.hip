@@ -423,16 +423,16 @@ static hipError_t ensure_base(ryr_handle *h, long long nahead)
 }
 
 #define RYR_F32(h) ((h)->p.precision == RYR_PREC_F32)
-static hipError_t go_cyto(ryr_handle *h, int wg, int kc, int km, hipEvent_t *ev)
+static hipError_t go_cyto(ryr_handle *h, int wg, int kc, int km, hipEvent_t *ev, bool gate = false)
 {
-    return RYR_F32(h) ? ryr_launch_cyto_f32(h->a, h->nblocks, wg, kc, km, h->p.stim_amp, h->stream, ev)
-                      : ryr_launch_cyto_f64(h->a, h->nblocks, wg, kc, km, h->p.stim_amp, h->stream, ev);
+    return RYR_F32(h) ? ryr_launch_cyto_f32(h->a, h->nblocks, wg, kc, km, gate, h->p.stim_amp, h->stream, ev)
+                      : ryr_launch_cyto_f64(h->a, h->nblocks, wg, kc, km, gate, h->p.stim_amp, h->stream, ev);
 }
-static hipError_t go_cru(ryr_handle *h, int wg, int kc, hipEvent_t *ev, hipStream_t st = nullptr)
+static hipError_t go_cru(ryr_handle *h, int wg, int kc, hipEvent_t *ev, hipStream_t st = nullptr, bool split = false)
 {
     if (!st) st = h->stream;
-    return RYR_F32(h) ? ryr_launch_cru_f32(h->a, h->nblocks, wg, kc, st, ev)
-                      : ryr_launch_cru_f64(h->a, h->nblocks, wg, kc, st, ev);
+    return RYR_F32(h) ? ryr_launch_cru_f32(h->a, h->nblocks, wg, kc, split, st, ev)
+                      : ryr_launch_cru_f64(h->a, h->nblocks, wg, kc, split, st, ev);
 }
 static hipError_t go_mem(ryr_handle *h, int km, hipEvent_t *ev)
 {
@@ -446,12 +446,15 @@ static hipError_t go_diag(ryr_handle *h, hipEvent_t *ev)
 }
 
 /* Steps h->step .. h->step+n-1 (h->step itself is not advanced here), in
- * one of three schedules (every schedule computes identical values):
+ * one of four schedules (every schedule computes identical values):
  *  RYR_SCHED_SEQ: sub-voxel kernel, CRU kernel, membrane step;
  *  RYR_SCHED_FUSED: the sub-voxel kernel of step m carries the membrane step
  *    of m-1; the membrane step runs on its own only at an output step (the
  *    output-row kernel follows it) and at the last step of the run, so every
  *    run ends with the whole state advanced;
+ *  RYR_SCHED_FUSED_GATING: as FUSED, and the sub-voxel launch of step m also
+ *    carries the RyR gating of step m, which reads only step-start values;
+ *    the CRU kernel of m then runs without it;
  *  RYR_SCHED_CONCURRENT (cleft scheme A): the sub-voxel and CRU kernels of a
  *    step read only step-start values of each other's fields (cp, cs, cjsr
  *    and ci/cnsr are double-buffered) and write disjoint fields, so the CRU
@@ -477,9 +480,10 @@ static hipError_t enqueue_run(ryr_handle *h, long long n, int sched)
             if (e == hipSuccess && out) e = go_diag(h, nullptr);
             continue;
         }
-        const bool fuse = sched == RYR_SCHED_FUSED;
-        e = go_cyto(h, h->wg_cyto, k, pending ? k - 1 : -1, nullptr);
-        if (e == hipSuccess) e = go_cru(h, h->wg_cru, k, nullptr);
+        const bool gate = sched == RYR_SCHED_FUSED_GATING;
+        const bool fuse = sched == RYR_SCHED_FUSED || gate;
+        e = go_cyto(h, h->wg_cyto, k, pending ? k - 1 : -1, nullptr, gate);
+        if (e == hipSuccess) e = go_cru(h, h->wg_cru, k, nullptr, nullptr, gate);
         if (e == hipSuccess && (out || i == n - 1 || !fuse)) {
             e = go_mem(h, k, nullptr);
             if (e == hipSuccess && out) e = go_diag(h, nullptr);
@@ -498,8 +502,11 @@ static hipError_t enqueue_timed_step(ryr_handle *h, int wg_cyto, int wg_cru, boo
 {
     const int k = (int)(h->step - h->gbase);
     *out = h->step % h->p.steps_per_output == 0;
-    hipError_t e = go_cyto(h, wg_cyto, k, -1, h->kev);
-    if (e == hipSuccess) e = go_cru(h, wg_cru, k, h->kev + 2);
+    /* under the gating schedule the sub-voxel launch carries the gating and
+     * the CRU kernel runs without it, as in production */
+    const bool gate = h->sched == RYR_SCHED_FUSED_GATING;
+    hipError_t e = go_cyto(h, wg_cyto, k, -1, h->kev, gate);
+    if (e == hipSuccess) e = go_cru(h, wg_cru, k, h->kev + 2, nullptr, gate);
     if (e == hipSuccess) e = go_mem(h, k, h->kev + 4);
     if (e == hipSuccess && *out) e = go_diag(h, h->kev + 6);
     return e;
@@ -562,7 +569,14 @@ static int tune_step(ryr_handle *h)
  * faster by more than 3%.  Each candidate advances 3 * RYR_TUNE_GSTEPS
  * steps; candidates run while a call has that many steps left. */
 #define RYR_TUNE_GSTEPS 4
-static int nsched(const ryr_handle *h) { return h->p.scheme == RYR_SCHEME_A ? 3 : 2; }
+/* whether a shape can run: the concurrent schedule needs cleft scheme A,
+ * the gating schedule a 128- or 256-lane CRU kernel */
+static bool shape_ok(const ryr_handle *h, int wg_cru, int sched)
+{
+    if (sched == RYR_SCHED_CONCURRENT) return h->p.scheme == RYR_SCHEME_A;
+    if (sched == RYR_SCHED_FUSED_GATING) return wg_cru == 128 || wg_cru == 256;
+    return sched == RYR_SCHED_SEQ || sched == RYR_SCHED_FUSED;
+}
 
 static void sched_candidates(const ryr_handle *h, std::vector<std::array<int, 3>> &c)
 {
@@ -580,7 +594,10 @@ static void sched_candidates(const ryr_handle *h, std::vector<std::array<int, 3>
      * (small) grids */
     add(256, 512, RYR_SCHED_FUSED);
     add(h->wg_cyto, 512, RYR_SCHED_SEQ);
-    if (nsched(h) > 2) {
+    /* the RyR gating beside the sub-voxel sweep and the membrane step */
+    add(256, 256, RYR_SCHED_FUSED_GATING);
+    add(h->wg_cyto, 256, RYR_SCHED_FUSED_GATING);
+    if (h->p.scheme == RYR_SCHEME_A) {
         add(h->wg_cyto, h->wg_cru, RYR_SCHED_CONCURRENT);
         add(128, 64, RYR_SCHED_CONCURRENT);
         add(256, 64, RYR_SCHED_CONCURRENT);
@@ -776,13 +793,14 @@ extern "C" int ryr_set_launch_shape(ryr_handle *h, int wg_cyto, int wg_cru, int 
         h->tune_i = 0;
         h->cand.clear();
         for (auto &k : h->tune_ms) for (auto &q : k) q.clear();
-    } else if (ok(wg_cyto) && ok_cru(wg_cru) && sched >= 0 && sched < nsched(h)) {
+    } else if (ok(wg_cyto) && ok_cru(wg_cru) && shape_ok(h, wg_cru, sched)) {
         h->wg_cyto = wg_cyto;
         h->wg_cru = wg_cru;
         h->sched = sched;
     } else {
-        h->err = "workgroup sizes must be 64, 128 or 256 and the schedule one of RYR_SCHED_* "
-                 "(concurrent needs cleft scheme A), or sizes 0, 0 to tune";
+        h->err = "workgroup sizes must be 64, 128 or 256 (CRU kernel also 512) and the schedule one of "
+                 "RYR_SCHED_* (concurrent needs cleft scheme A, gating a 128- or 256-lane CRU kernel), "
+                 "or sizes 0, 0 to tune";
         return RYR_E_ARG;
     }
     drop_graph(h);

@@ -82,7 +82,8 @@ class CellEngine:
         """Workgroup sizes (64/128/256) of the sub-voxel and CRU kernels (CRU
         kernel also 512: two lanes per CRU) and the step schedule (0 sequential,
         1 membrane fused into the next sub-voxel launch, 2 sub-voxel and CRU
-        kernels concurrent); sizes (0, 0) re-tune."""
+        kernels concurrent, 3 as 1 with the RyR gating in the sub-voxel launch
+        too); sizes (0, 0) re-tune."""
         self._check(self.lib.ryr_set_launch_shape(self.h, int(wg_cyto), int(wg_cru), int(sched)),
                     "ryr_set_launch_shape")
 

@@ -137,19 +137,21 @@ def test_variant_fp64_bitexact_against_the_oracle(name, pk, dk):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("scheme", [0, 1])
-def test_paired_lanes_bitexact_with_gapped_lcc_slots(scheme):
+@pytest.mark.parametrize("shape", [(256, 512, 1), (256, 256, 3)])
+def test_split_shapes_bitexact_with_gapped_lcc_slots(scheme, shape):
     """Two lanes per CRU (CRU kernel size 512) split the RyR transitions and
-    the LCC channels by parity.  randomlcc + iso leaves gaps in the slot mask
+    the LCC channels by parity; the gating schedule (3) moves the RyR gating
+    into the sub-voxel launch.  randomlcc + iso leaves gaps in the slot mask
     (0..count-1 and, from 5.5 CL, 4..7): only the slots before the first gap
     gate, as in the reference's loop.  FP64 against the oracle, both cleft
     schemes, with the shape forced."""
     from ryr_hyperactivity_amd import CellEngine
-    print(f"case paired lanes scheme {scheme}", flush=True)
+    print(f"case shape {shape} scheme {scheme}", flush=True)
     p = Params.default(stim_offset=1.0, variants=VAR_RANDOMLCC | VAR_ISO, scheme=scheme)
     d = CellDesc.default(nx=9, ny=5, nz=3, cl=10.0, seed=9)
     o = OracleCell(p, d)
     with CellEngine(p, d) as eng:
-        eng.set_launch_shape(256, 512, 1)
+        eng.set_launch_shape(*shape)
         for chunk in (1, 99, 2100, 300):
             eng.step(chunk)
             o.step(chunk)
