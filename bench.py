@@ -42,7 +42,7 @@ def alg_bytes_per_cru_step(precision):
     return 2 * (73 * 8 + 3 * 4)
 
 
-def alg_bytes_per_kernel(precision):
+def alg_bytes_per_kernel(precision, sched=0):
     """Algorithmic HBM bytes per interior CRU of each step kernel (scheme A),
     every word the kernel must touch counted once.
     k_cyto (sub-voxel kernel): reads 8 ci, 8 cnsr, 5x8 fast buffers, 8 slow
@@ -53,10 +53,18 @@ def alg_bytes_per_kernel(precision):
     buffers, cp, cs, cjsr, cjsrb, the packed words.  FP32: 80 + 56 = 136 B;
     FP64: 116 + 84 = 200 B.  The two add up to the whole-step figure plus the
     40 (FP64: 56) bytes both kernels read.
-    k_membrane: per cell, not per CRU (a few hundred bytes): 0 here."""
+    k_membrane: per cell, not per CRU (a few hundred bytes): 0 here.
+    Schedule 3 (RyR gating in the sub-voxel launch): the sub-voxel launch also
+    reads and writes the RyR-count and flag words (cp and cjsr it reads
+    anyway), +16 B; the CRU kernel no longer writes the RyR-count word, -4 B."""
     if precision == 1:
-        return {"k_cyto": 656, "k_cru_step": 136, "k_membrane": 0}
-    return {"k_cyto": 1048, "k_cru_step": 200, "k_membrane": 0}
+        b = {"k_cyto": 656, "k_cru_step": 136, "k_membrane": 0}
+    else:
+        b = {"k_cyto": 1048, "k_cru_step": 200, "k_membrane": 0}
+    if sched == 3:
+        b["k_cyto"] += 16
+        b["k_cru_step"] -= 4
+    return b
 
 
 def make_cell(prec):
@@ -105,7 +113,7 @@ def run_gpu(p, cells, steps, warmup, prof_steps, shape=None):
         prof = eng.profile_kernels(prof_steps)
     sim_ms = len(cells) * steps * p.dt
     step_s = ev_ms * 1e-3 / max(launches, 1)
-    bpc = alg_bytes_per_kernel(p.precision)
+    bpc = alg_bytes_per_kernel(p.precision, shape[2])
     kernels = {}
     for name, (us, cnt) in prof.items():
         d = {"avg_us": us, "launches": cnt}

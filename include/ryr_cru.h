@@ -263,7 +263,7 @@ int ryr_kernel_time(ryr_handle *h, double *ms, int64_t *launches);
  *                         side on two streams (cleft scheme A only)
  *   RYR_SCHED_FUSED_GATING as FUSED, and the RyR gating of step n (which reads
  *                         only step-start values) rides in the sub-voxel launch
- *                         of step n too, off the CRU kernel (CRU size 128/256)
+ *                         of step n too, off the CRU kernel (CRU size 128/256/512)
  * No reference counterpart (the reference hard-codes one 48x448 launch,
  * :1701).  A new handle tunes the shape itself: its first 9 steps alternate
  * the sizes with each kernel timed by its own events (fastest median kept);

@@ -105,6 +105,26 @@ struct RyrMem {
 
 struct RyrDD { double hi, lo; };
 
+/* The V-only factors of the step that the CRU kernel reads (copied from
+ * RyrMem at the kernel's start, so that their loads issue with the state
+ * loads instead of inside the LCC and NCX branches) */
+struct RyrVF {
+    double alphad, betad, alphaf, betaf;
+    double ghk_A, ghk_e2, ghk_D;
+    double ncx_n1, ncx_n2, ncx_t1, ncx_t3a, ncx_den2;
+    int ghk_small;
+};
+
+RYR_DEV RyrVF load_vf(const RyrMem *M)
+{
+    RyrVF f;
+    f.alphad = M->alphad; f.betad = M->betad; f.alphaf = M->alphaf; f.betaf = M->betaf;
+    f.ghk_A = M->ghk_A; f.ghk_e2 = M->ghk_e2; f.ghk_D = M->ghk_D; f.ghk_small = M->ghk_small;
+    f.ncx_n1 = M->ncx_n1; f.ncx_n2 = M->ncx_n2; f.ncx_t1 = M->ncx_t1; f.ncx_t3a = M->ncx_t3a;
+    f.ncx_den2 = M->ncx_den2;
+    return f;
+}
+
 /* kernel arguments (identical for every step: the step index lives in RyrMem,
  * so a captured graph replays without new arguments) */
 struct RyrKArgs {
@@ -180,6 +200,18 @@ RYR_DEV RyrDD dd_dpp_add(RyrDD v)
     w.hi = dpp_f64<CTRL>(v.hi);
     w.lo = dpp_f64<CTRL>(v.lo);
     return dd_add(v, w);
+}
+
+/* Sum over the 64 lanes of a wave of a small unsigned count (exact in any
+ * order); every lane gets the total. */
+RYR_DEV uint32_t wave_sum_u32(uint32_t x)
+{
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, true);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, true);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x141, 0xF, 0xF, true);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x140, 0xF, 0xF, true);
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, 0) + (uint32_t)__builtin_amdgcn_readlane((int)x, 16) +
+           (uint32_t)__builtin_amdgcn_readlane((int)x, 32) + (uint32_t)__builtin_amdgcn_readlane((int)x, 48);
 }
 
 /* Pairwise tree over the 64 lanes of a wave in index order: level k joins
@@ -298,7 +330,8 @@ RYR_DEV void vfactors(RyrMem *m, int iso2)
  * first gate whose comparison holds is picked by selects, so every lane of a
  * wave runs one path whatever its channel's state; the values compared are
  * those of the sequential walk. */
-RYR_DEV int lcc_step(int i, double cpx, double dt, double ragg, const RyrMem &m, double betac)
+template <typename MV>
+RYR_DEV int lcc_step(int i, double cpx, double dt, double ragg, const MV &m, double betac)
 {
     RYR_STRICT_FP
     const double alphac = 0.22, alphafca = 0.012 / 2.;
@@ -386,10 +419,17 @@ RYR_DEV uint32_t lane_pair_u32(uint32_t x)
     return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, true);
 }
 
+/* Lane Q of this lane's quad (quad_perm [Q,Q,Q,Q]). */
+template <int Q>
+RYR_DEV uint32_t quad_lane_u32(uint32_t x)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, Q * 0x55, 0xF, 0xF, true);
+}
+
 /* subroutine_cuda.c:251-669; counts in/out: nou, ncu, nob, ncb; mono =
- * csqn_mono(cjsr); *nd counts the draws.  With LPC = 2 two adjacent lanes
- * share the CRU: lane h samples the transitions 4h .. 4h + 3 and the counts
- * are exchanged before they are combined.
+ * csqn_mono(cjsr); *nd counts the draws.  With LPC = 2 or 4 adjacent lanes
+ * share the CRU: lane h samples the transitions 8h/LPC .. 8(h+1)/LPC - 1 and
+ * the counts are gathered before they are combined.
  *
  * The eight transitions draw from their own addresses (philox.h), so they are
  * sampled in two passes rather than in program order.  Pass 1 draws every
@@ -434,8 +474,11 @@ RYR_DEV void ryr_gating(uint64_t seed, uint32_t cru, uint64_t step, double Ku, d
     int nl[TPL];
 #pragma unroll
     for (int l = 0; l < TPL; ++l) {
-        pl[l] = (LPC == 1 || h == 0) ? pt[l] : pt[(TPL + l) & 7];
-        nl[l] = (LPC == 1 || h == 0) ? pop[src[l]] : pop[src[(TPL + l) & 7]];
+        pl[l] = pt[l];
+        nl[l] = pop[src[l]];
+#pragma unroll
+        for (int hh = 1; hh < LPC; ++hh)
+            if (h == hh) { pl[l] = pt[TPL * hh + l]; nl[l] = pop[src[TPL * hh + l]]; }
     }
 #pragma unroll
     for (int b = 0; b < TPL / 2; ++b) {
@@ -479,11 +522,15 @@ RYR_DEV void ryr_gating(uint64_t seed, uint32_t cru, uint64_t step, double Ku, d
     if constexpr (LPC == 1) {
         klo = kw[0];
         khi = kw[LPC == 1 ? 1 : 0];
-    } else {
+    } else if constexpr (LPC == 2) {
         /* lane 0 holds transitions 0-3, lane 1 transitions 4-7 */
         const uint32_t other = lane_pair_u32(kw[0]);
         klo = h ? other : kw[0];
         khi = h ? kw[0] : other;
+    } else {
+        /* lane q of the quad holds transitions 2q, 2q + 1 in its low 16 bits */
+        klo = quad_lane_u32<0>(kw[0]) | (quad_lane_u32<1>(kw[0]) << 16);
+        khi = quad_lane_u32<2>(kw[0]) | (quad_lane_u32<3>(kw[0]) << 16);
     }
 
 #define RYR_KT(t) ((int)((((t) < 4 ? klo : khi) >> (8 * ((t) & 3))) & 255u))
@@ -1069,7 +1116,7 @@ RYR_DEV float fbuf(float b, float c, float dt, float sat, float kon, float koff,
  * membrane code's registers, hence a separate instance. */
 template <typename TF, typename TS, int WG>
 RYR_DEV void membrane_block(const RyrKArgs &a, int c, long long nstep, double stim_amp);
-#define RYR_GATE_LPC 2   /* lanes per CRU in the gating part: each takes four transitions */
+#define RYR_GATE_LPC 4   /* lanes per CRU in the gating part: each takes two transitions */
 template <typename TF, typename TS, int WG>
 RYR_DEV void gating_block(const RyrKArgs &a, int L, long long nstep);
 
@@ -1370,6 +1417,9 @@ RYR_DEV void gating_block(const RyrKArgs &a, int L, long long nstep)
     ryr_gating<LPC>(C.seed, (uint32_t)n, (uint64_t)nstep, C.Ku, C.Ku2, C.Kb, cp_d, cj, mono, a.dt, nou, ncu, nob,
                     ncb, &nfail, &ndraw, h);
     if constexpr (LPC == 2) nfail += (int)lane_pair_u32((uint32_t)nfail);
+    if constexpr (LPC == 4)
+        nfail = (int)(quad_lane_u32<0>((uint32_t)nfail) + quad_lane_u32<1>((uint32_t)nfail) +
+                      quad_lane_u32<2>((uint32_t)nfail) + quad_lane_u32<3>((uint32_t)nfail));
     if (h != 0) return;
     if (prevsp < 5 && (nou + nob) >= 5) ax |= 1u << 16;
     a.ryrw[g] = (uint32_t)nou | ((uint32_t)ncu << 8) | ((uint32_t)nob << 16) | ((uint32_t)ncb << 24);
@@ -1387,7 +1437,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu((WG >= 256 |
 void k_cru_step(RyrKArgs a, int kc)
 {
     static_assert(LPC == 1 || (LPC == 2 && WG >= 128), "paired lanes need two waves per workgroup");
-    static_assert(!(GSPLIT && LPC > 1), "with the gating split off, one lane per CRU");
+
     constexpr int CPW = WG / LPC;          /* CRUs per workgroup */
     constexpr int SUB = RYR_BLOCK / CPW;
     const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
@@ -1430,6 +1480,7 @@ void k_cru_step(RyrKArgs a, int kc)
         TS *cjsr1 = (TS *)a.cjsr[par ^ 1], *cjsrbp = (TS *)a.cjsrb;
 
         const double v = M->m[0];
+        const RyrVF vf = load_vf(M);
         TF cil[2];
         TS cnl[2];
 #pragma unroll
@@ -1473,7 +1524,7 @@ void k_cru_step(RyrKArgs a, int kc)
                             int st = (int)((lw >> (4 * ll)) & 15u);
                             if (!(ll & 1)) lb = ryr_block(C.seed, (uint32_t)n, (uint64_t)nstep, (uint32_t)(ll >> 1));
                             const double ragg = (ll & 1) ? ryr_slot_uniform(lb.v[2], lb.v[3]) : ryr_slot_uniform(lb.v[0], lb.v[1]);
-                            int jj = lcc_step(st, cpx, dt, ragg, *M, betac);
+                            int jj = lcc_step(st, cpx, dt, ragg, vf, betac);
                             nw = (nw & ~(15u << (4 * ll))) | ((uint32_t)jj << (4 * ll));
                             if (jj == 0) ++nlcp;
                         }
@@ -1486,7 +1537,7 @@ void k_cru_step(RyrKArgs a, int kc)
                         int st = (int)((lw >> (4 * ll)) & 15u);
                         lb = ryr_block(C.seed, (uint32_t)n, (uint64_t)nstep, (uint32_t)(ll >> 1));
                         const double ragg = (ll & 1) ? ryr_slot_uniform(lb.v[2], lb.v[3]) : ryr_slot_uniform(lb.v[0], lb.v[1]);
-                        int jj = lcc_step(st, cpx, dt, ragg, *M, betac);
+                        int jj = lcc_step(st, cpx, dt, ragg, vf, betac);
                         nw = (nw & ~(15u << (4 * ll))) | ((uint32_t)jj << (4 * ll));
                         if (jj == 0) ++nlcp;
                     }
@@ -1500,8 +1551,8 @@ void k_cru_step(RyrKArgs a, int kc)
                 }
                 nl += nlcp;
                 lw = nw;
-                double X_ = cpx / 1000.0 * M->ghk_e2 - (1 * 1.8);
-                double ica = M->ghk_small ? 2.0 * 11.9 * 96.485 * 0.341 * X_ : M->ghk_A * X_ / M->ghk_D;
+                double X_ = cpx / 1000.0 * vf.ghk_e2 - (1 * 1.8);
+                double ica = vf.ghk_small ? 2.0 * 11.9 * 96.485 * 0.341 * X_ : vf.ghk_A * X_ / vf.ghk_D;
                 if (ica > 0.0) ica = 0.0;
                 icat1 = C.svica * (double)(nlcp) * ica;
             }
@@ -1545,9 +1596,9 @@ void k_cru_step(RyrKArgs a, int kc)
             const float csm = (float)cs_d * 1e-3f;
             const float Ka = (float)ancx[g];
             const float t2 = 669921.875f * csm * (1.0f + csm * (float)(1.0 / 0.00359));
-            const float t3 = (float)M->ncx_t3a + csm * 2744000.0f;
-            const float Inaca = Ka * 21.0f * ((float)M->ncx_n1 - (float)M->ncx_n2 * csm) *
-                                __builtin_amdgcn_rcpf(((float)M->ncx_t1 + t2 + t3) * (float)M->ncx_den2);
+            const float t3 = (float)vf.ncx_t3a + csm * 2744000.0f;
+            const float Inaca = Ka * 21.0f * ((float)vf.ncx_n1 - (float)vf.ncx_n2 * csm) *
+                                __builtin_amdgcn_rcpf(((float)vf.ncx_t1 + t2 + t3) * (float)vf.ncx_den2);
             const float r = 3e-4f * __builtin_amdgcn_rcpf(csm);
             const float Ancxdot = (__builtin_amdgcn_rcpf(1.0f + r * r * r) - Ka) * (1.0f / 150.0f);
             ancx_new = (TF)(Ka + Ancxdot * (float)dt);
@@ -1557,9 +1608,9 @@ void k_cru_step(RyrKArgs a, int kc)
             double cs = cs_d / 1000.0;
             double Ka = (double)ancx[g];
             double t2 = (87.5) * (87.5) * (87.5) * cs * (1.0 + cs / 0.00359);
-            double t3 = M->ncx_t3a + cs * ((140.0) * (140.0) * (140.0));
-            double Inaca = Ka * (21.0 / 1.0) * (M->ncx_n1 - M->ncx_n2 * cs)
-                           / ((M->ncx_t1 + t2 + t3) * M->ncx_den2);
+            double t3 = vf.ncx_t3a + cs * ((140.0) * (140.0) * (140.0));
+            double Inaca = Ka * (21.0 / 1.0) * (vf.ncx_n1 - vf.ncx_n2 * cs)
+                           / ((vf.ncx_t1 + t2 + t3) * vf.ncx_den2);
             double r = 0.0003 / cs;
             double Ancxdot = (1.0 / (1.0 + r * r * r) - Ka) / 150.;
             ancx_new = (TF)(Ka + Ancxdot * dt);
@@ -2002,9 +2053,12 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_diag(RyrKArgs a)
         t = wave_tree(t);
         if (lane == 0) s_red[q][wv] = t;
     }
+    /* the counts: per-wave sums (each below 2^32), one LDS atomic per wave */
 #pragma unroll
-    for (int q = 0; q < RYR_DIAG_I; ++q)
-        if (ic[q]) atomicAdd(&s_cnt[q], ic[q]);
+    for (int q = 0; q < RYR_DIAG_I; ++q) {
+        const uint32_t w = wave_sum_u32((uint32_t)ic[q]);
+        if (lane == 0 && w) atomicAdd(&s_cnt[q], (unsigned long long)w);
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         RyrDD *pp = a.dpartial + (long long)b * RYR_DIAG_Q;

@@ -99,6 +99,7 @@ static hipError_t ryr_go_cru_split(const RyrKArgs &a, int nchunks, int wg, int k
     switch (wg) {
     case 128: RYR_GO((k_cru_step<TF, TS, 128, SCHB, 1, true>), dim3((unsigned)nchunks * 2u), dim3(128), s, ev, a, kc); break;
     case 256: RYR_GO((k_cru_step<TF, TS, 256, SCHB, 1, true>), dim3((unsigned)nchunks), dim3(256), s, ev, a, kc); break;
+    case 512: RYR_GO((k_cru_step<TF, TS, 256, SCHB, 2, true>), dim3((unsigned)nchunks * 2u), dim3(256), s, ev, a, kc); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

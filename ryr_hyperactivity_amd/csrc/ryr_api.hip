@@ -570,11 +570,11 @@ static int tune_step(ryr_handle *h)
  * steps; candidates run while a call has that many steps left. */
 #define RYR_TUNE_GSTEPS 4
 /* whether a shape can run: the concurrent schedule needs cleft scheme A,
- * the gating schedule a 128- or 256-lane CRU kernel */
+ * the gating schedule a CRU size of 128, 256 or 512 */
 static bool shape_ok(const ryr_handle *h, int wg_cru, int sched)
 {
     if (sched == RYR_SCHED_CONCURRENT) return h->p.scheme == RYR_SCHEME_A;
-    if (sched == RYR_SCHED_FUSED_GATING) return wg_cru == 128 || wg_cru == 256;
+    if (sched == RYR_SCHED_FUSED_GATING) return wg_cru == 128 || wg_cru == 256 || wg_cru == 512;
     return sched == RYR_SCHED_SEQ || sched == RYR_SCHED_FUSED;
 }
 
@@ -596,6 +596,7 @@ static void sched_candidates(const ryr_handle *h, std::vector<std::array<int, 3>
     add(h->wg_cyto, 512, RYR_SCHED_SEQ);
     /* the RyR gating beside the sub-voxel sweep and the membrane step */
     add(256, 256, RYR_SCHED_FUSED_GATING);
+    add(256, 512, RYR_SCHED_FUSED_GATING);
     add(h->wg_cyto, 256, RYR_SCHED_FUSED_GATING);
     if (h->p.scheme == RYR_SCHEME_A) {
         add(h->wg_cyto, h->wg_cru, RYR_SCHED_CONCURRENT);
@@ -799,7 +800,7 @@ extern "C" int ryr_set_launch_shape(ryr_handle *h, int wg_cyto, int wg_cru, int 
         h->sched = sched;
     } else {
         h->err = "workgroup sizes must be 64, 128 or 256 (CRU kernel also 512) and the schedule one of "
-                 "RYR_SCHED_* (concurrent needs cleft scheme A, gating a 128- or 256-lane CRU kernel), "
+                 "RYR_SCHED_* (concurrent needs cleft scheme A, gating a CRU size of 128, 256 or 512), "
                  "or sizes 0, 0 to tune";
         return RYR_E_ARG;
     }

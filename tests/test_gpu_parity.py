@@ -196,7 +196,7 @@ def test_beats_and_clamp_match_the_oracle():
 
 @pytest.mark.parametrize("shape", [(64, 64, 1), (128, 256, 0), (256, 128, 1), (256, 256, 0), (64, 128, 2),
                                    (256, 64, 2), (256, 512, 1), (128, 512, 0), (64, 512, 2), (256, 256, 3),
-                                   (64, 128, 3)])
+                                   (64, 128, 3), (128, 512, 3)])
 def test_every_launch_shape_is_bitexact(shape):
     """Workgroup sizes and step schedules change no value: FP64 against the
     oracle, and FP32 against the FP32 run at another shape (whole-cell sums
