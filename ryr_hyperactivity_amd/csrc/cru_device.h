@@ -1123,35 +1123,17 @@ RYR_DEV void gating_block(const RyrKArgs &a, int L, long long nstep);
 /* MODE bit 0: the first ncells logical workgroups run the membrane step
  * gstep + km; bit 1: the next nchunks * 256/WG * RYR_GATE_LPC run the RyR
  * gating of step gstep + kc (gating_block).  The sub-voxel sweep follows. */
-template <typename TF, typename TS, int WG, int MODE>
-__global__ __launch_bounds__(WG) void k_cyto(RyrKArgs a, int kc, int km, double stim_amp)
+/* The sub-voxel sweep for logical workgroup L of a launch: one lane per
+ * sub-voxel of plane ii of one part of one 256-CRU chunk. */
+template <typename TF, typename TS, int WG>
+RYR_DEV void cyto_block(const RyrKArgs &a, int L, long long nstep)
 {
     constexpr int SUB = RYR_BLOCK / WG;
-    const long long base = *a.gstep;
-    const int nmem = (MODE & 1) ? a.ncells : 0;
-    const int Lg = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-    if constexpr ((MODE & 1) != 0) {
-        if (Lg < nmem) {
-            membrane_block<TF, TS, WG>(a, Lg, base + km, stim_amp);
-            return;
-        }
-    }
-    int L = Lg - nmem;
-    if constexpr ((MODE & 2) != 0) {
-        /* RYR_GATE_LPC gating workgroups per 8 sub-voxel ones */
-        const int ngate = ((int)gridDim.x - nmem) / (8 + RYR_GATE_LPC) * RYR_GATE_LPC;
-        if (L < ngate) {
-            gating_block<TF, TS, WG>(a, L, base + kc);
-            return;
-        }
-        L -= ngate;
-    }
     const int b = L / (8 * SUB);
     const int r = L - b * (8 * SUB);
     const int ii = r & 7;
     const int part = r >> 3;
     const RyrCellInfo C = a.chunks[b];
-    const long long nstep = base + kc;
     const int n = (b - C.first_block) * RYR_BLOCK + part * WG + (int)threadIdx.x;
     if (n >= C.N) return;
     const int par = (int)(nstep & 1);
@@ -1318,6 +1300,32 @@ __global__ __launch_bounds__(WG) void k_cyto(RyrKArgs a, int kc, int km, double 
     }
 }
 
+template <typename TF, typename TS, int WG, int MODE>
+__global__ __launch_bounds__(WG) void k_cyto(RyrKArgs a, int kc, int km, double stim_amp)
+{
+    constexpr int SUB = RYR_BLOCK / WG;
+    const long long base = *a.gstep;
+    const int nmem = (MODE & 1) ? a.ncells : 0;
+    const int Lg = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    if constexpr ((MODE & 1) != 0) {
+        if (Lg < nmem) {
+            membrane_block<TF, TS, WG>(a, Lg, base + km, stim_amp);
+            return;
+        }
+    }
+    int L = Lg - nmem;
+    if constexpr ((MODE & 2) != 0) {
+        /* RYR_GATE_LPC gating workgroups per 8 sub-voxel ones */
+        const int ngate = ((int)gridDim.x - nmem) / (8 + RYR_GATE_LPC) * RYR_GATE_LPC;
+        if (L < ngate) {
+            gating_block<TF, TS, WG>(a, L, base + kc);
+            return;
+        }
+        L -= ngate;
+    }
+    cyto_block<TF, TS, WG>(a, L, base + kc);
+}
+
 /* Cleft/sub-membrane sub-steps of the FP32 production path (scheme A
  * cuda_Cell_Detailed.cu:1396-1464, scheme B :1479-1553), reciprocal
  * constants and float arithmetic.  Under scheme B dotci01 enters with the
@@ -1428,19 +1436,14 @@ RYR_DEV void gating_block(const RyrKArgs &a, int L, long long nstep)
 }
 
 /* ------------------------------------------------------------ step kernel */
-/* The 64- and 128-lane scheme-A instances, which the shape tuner picks for
- * batches (throughput), are held to 168 VGPRs: three waves per SIMD, without
- * spilling.  The 256-lane instance, which it picks for single cells (one
- * wave per SIMD anyway), keeps the registers the allocator wants. */
-template <typename TF, typename TS, int WG, bool SCHB, int LPC = 1, bool GSPLIT = false>
-__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu((WG >= 256 || SCHB) ? 1 : 3)))
-void k_cru_step(RyrKArgs a, int kc)
+/* The CRU update for logical workgroup L of a launch (step gstep + kc). */
+template <typename TF, typename TS, int WG, bool SCHB, int LPC, bool GSPLIT>
+RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L)
 {
     static_assert(LPC == 1 || (LPC == 2 && WG >= 128), "paired lanes need two waves per workgroup");
 
     constexpr int CPW = WG / LPC;          /* CRUs per workgroup */
     constexpr int SUB = RYR_BLOCK / CPW;
-    const int L = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     const int bid = L / SUB;               /* 256-CRU chunk */
     const int part = L - bid * SUB;
     const RyrCellInfo C = a.chunks[bid];
@@ -1810,6 +1813,17 @@ void k_cru_step(RyrKArgs a, int kc)
             }
         }
     }
+}
+
+/* The 64- and 128-lane scheme-A instances, which the shape tuner picks for
+ * batches (throughput), are held to 168 VGPRs: three waves per SIMD, without
+ * spilling.  The 256-lane instance, which it picks for single cells (one
+ * wave per SIMD anyway), keeps the registers the allocator wants. */
+template <typename TF, typename TS, int WG, bool SCHB, int LPC = 1, bool GSPLIT = false>
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu((WG >= 256 || SCHB) ? 1 : 3)))
+void k_cru_step(RyrKArgs a, int kc)
+{
+    cru_block<TF, TS, WG, SCHB, LPC, GSPLIT>(a, kc, xcd_remap((int)blockIdx.x, (int)gridDim.x));
 }
 
 /* ------------------------------------------------------------ membrane kernel */
