@@ -1301,7 +1301,8 @@ RYR_DEV void cyto_block(const RyrKArgs &a, int L, long long nstep)
 }
 
 template <typename TF, typename TS, int WG, int MODE>
-__global__ __launch_bounds__(WG) void k_cyto(RyrKArgs a, int kc, int km, double stim_amp)
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu((MODE & 2) ? 4 : 1)))
+void k_cyto(RyrKArgs a, int kc, int km, double stim_amp)
 {
     constexpr int SUB = RYR_BLOCK / WG;
     const long long base = *a.gstep;
