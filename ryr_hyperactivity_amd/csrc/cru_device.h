@@ -1484,11 +1484,20 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L)
         TS *cjsr1 = (TS *)a.cjsr[par ^ 1], *cjsrbp = (TS *)a.cjsrb;
 
         const double v = M->m[0];
-        const RyrVF vf = load_vf(M);
+        /* The V-only factors and the NSR values of sub-voxels 0-1 are loaded
+         * with the rest of the state where registers allow (the instances that
+         * run one wave per SIMD anyway), and at their use sites in the
+         * instances held to 168 VGPRs, where keeping them live spills */
+        constexpr bool vf_early = WG >= 256 || SCHB;
+        RyrVF vf0{};
+        if constexpr (vf_early) vf0 = load_vf(M);
         TF cil[2];
-        TS cnl[2];
+        TS cnl[2] = {TS(0), TS(0)};
 #pragma unroll
-        for (int ii = 0; ii < 2; ++ii) { cil[ii] = ci0[ii * P + g]; cnl[ii] = cn0[ii * P + g]; }
+        for (int ii = 0; ii < 2; ++ii) {
+            cil[ii] = ci0[ii * P + g];
+            if constexpr (vf_early) cnl[ii] = cn0[ii * P + g];
+        }
         const TF csv = cs0[g];
         TF cpv = cp0[g];
         const TS cjsrv = cjsr0[g];
@@ -1515,6 +1524,7 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L)
         {
             RYR_STRICT_FP
             if (!perm) {
+                const RyrVF vf = vf_early ? vf0 : load_vf(M);
                 const double cpx = (cp_d + cs_d * (0.5)) / (1 + (0.5));
                 const double betac = a.iso2 ? 2.0 : 4.0;
                 int nlcp = 0;
@@ -1597,6 +1607,7 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L)
             xinaca = 0.0;
             ancx_new = ancx[g];
         } else if constexpr (sizeof(TF) != 8) {
+            const RyrVF vf = vf_early ? vf0 : load_vf(M);
             const float csm = (float)cs_d * 1e-3f;
             const float Ka = (float)ancx[g];
             const float t2 = 669921.875f * csm * (1.0f + csm * (float)(1.0 / 0.00359));
@@ -1609,6 +1620,7 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L)
             xinaca = (double)((float)C.svncx * Inaca);
         } else {
             RYR_STRICT_FP
+            const RyrVF vf = vf_early ? vf0 : load_vf(M);
             double cs = cs_d / 1000.0;
             double Ka = (double)ancx[g];
             double t2 = (87.5) * (87.5) * (87.5) * cs * (1.0 + cs / 0.00359);
@@ -1626,6 +1638,7 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L)
         const TF cpT = cpv, csT = csv;
         const TF dtT = (TF)dt;
         TS diffjn0, diffjn1;
+        if constexpr (!vf_early) { cnl[0] = cn0[g]; cnl[1] = cn0[P + g]; }
         if constexpr (sizeof(TF) == 8) {
             diffjn0 = (cjsrv - cnl[0]) / TS(25.0 * 1.0 / 4.) / TS(2.);
             diffjn1 = (cjsrv - cnl[1]) / TS(25.0 * 1.0 / 4.) / TS(2.);
@@ -1768,7 +1781,12 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L)
             a.lccw[g] = lw;
             if (!GSPLIT)
                 a.ryrw[g] = (uint32_t)nou | ((uint32_t)ncu << 8) | ((uint32_t)nob << 16) | ((uint32_t)ncb << 24);
-            a.aux[g] = (uint32_t)(nl & 0xffff) | ((uint32_t)nspark << 16) | (lmask << RYR_AUX_MASK_SHIFT);
+            /* the base passes through an empty asm so that the store forms its
+             * address here instead of keeping the load's 64-bit address live
+             * across the kernel (which spills in the 168-VGPR instances) */
+            uint32_t *auxp = a.aux;
+            __asm__ volatile("" : "+s"(auxp));
+            auxp[g] = (uint32_t)(nl & 0xffff) | ((uint32_t)nspark << 16) | (lmask << RYR_AUX_MASK_SHIFT);
             if (diag) {
                 a.d_xire[g] = xire;
                 a.d_xinaca[g] = xinaca;

@@ -243,6 +243,35 @@ def steered_map(grid, train_points, replicas: int, nbeats: int, lattice=(20, 8, 
     return SteeredMap(values, source, cost, predicted)
 
 
+def holdout_points(grid, n: int, seed: int = 1):
+    """`n` distinct grid points drawn at random (seeded), in grid order."""
+    rng = np.random.default_rng(seed)
+    idx = np.sort(rng.choice(len(grid), size=min(n, len(grid)), replace=False))
+    return [grid[int(i)] for i in idx]
+
+
+def holdout_summary(grid, sm: SteeredMap, hold: dict, hold_cost: Cost, replicas: int, nbeats: int) -> dict:
+    """Map error of the steered map against a plain-method hold-out, and its
+    GPU time against the plain map's.  The plain map is not run: its GPU time
+    is the hold-out's GPU time per cell-step times the plain map's cell-steps
+    (the hold-out runs in batches of the same kind).  Points the steered map
+    simulated itself carry the same seeds, so their values must equal the
+    hold-out's exactly; `resimulated_equal` reports it."""
+    plain_cells = sum(point_cost(p, replicas, nbeats) for p in grid)
+    plain_gpu_s = hold_cost.gpu_s / hold_cost.cell_steps * plain_cells
+    err = np.array([abs(sm.values[p] - hold[p].ead) for p in hold])
+    sur = np.array([sm.source[p] == "surrogate" for p in hold])
+    same = [sm.values[p] == hold[p].ead for p in hold if sm.source[p] != "surrogate"]
+    return dict(holdout=len(hold), holdout_surrogate=int(sur.sum()),
+                map_err_mean=float(err.mean()), map_err_max=float(err.max()),
+                map_err_surrogate_mean=float(err[sur].mean()) if sur.any() else 0.0,
+                map_err_surrogate_max=float(err[sur].max()) if sur.any() else 0.0,
+                resimulated_equal=bool(all(same)), plain_cell_steps=plain_cells,
+                plain_gpu_s_estimate=plain_gpu_s,
+                cell_step_ratio=sm.cost.cell_steps / plain_cells,
+                gpu_time_ratio=(sm.cost.gpu_s + sm.cost.train_s) / plain_gpu_s)
+
+
 def grid_points(ku2s, ikrs, cls):
     return [Point(float(k), float(i), float(c)) for k in ku2s for i in ikrs for c in cls]
 
@@ -255,8 +284,11 @@ def main(argv=None):
     ap.add_argument("--cl", default="1000,2000,3000,4000")
     ap.add_argument("--replicas", type=int, default=16)
     ap.add_argument("--nbeats", type=int, default=6)
-    ap.add_argument("--mode", choices=["plain", "steered", "compare"], default="compare")
-    ap.add_argument("--train-every", type=int, default=2, help="training subset: every k-th value on each axis")
+    ap.add_argument("--mode", choices=["plain", "steered", "compare", "holdout"], default="compare",
+                    help="compare: the plain method on every grid point; holdout: on --holdout random points")
+    ap.add_argument("--holdout", type=int, default=200, help="hold-out size of --mode holdout")
+    ap.add_argument("--train-every", default="2",
+                    help="training subset: every k-th value on each axis (one k, or k_ku2,k_ikr,k_cl)")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--out", default=None)
     args = ap.parse_args(argv)
@@ -271,7 +303,9 @@ def main(argv=None):
                                  apd90=r.apd90, apd90_ci=r.apd90_ci) for p, r in res.items()]
         report["cost"] = cost.__dict__
     else:
-        sub = [a[::args.train_every] + ([a[-1]] if (len(a) - 1) % args.train_every else []) for a in ax]
+        ks = [int(v) for v in args.train_every.split(",")]
+        ks = ks * 3 if len(ks) == 1 else ks
+        sub = [a[::k] + ([a[-1]] if (len(a) - 1) % k else []) for a, k in zip(ax, ks)]
         train = grid_points(*sub)
         sm = steered_map(grid, train, args.replicas, args.nbeats, lat, seed0=args.seed, progress=log)
         n_sur = sum(1 for s in sm.source.values() if s == "surrogate")
@@ -279,6 +313,14 @@ def main(argv=None):
         report.update(steered_cost=sm.cost.__dict__, plain_cell_steps=plain_cells,
                       cell_step_ratio=sm.cost.cell_steps / plain_cells, n_surrogate=n_sur,
                       n_plain=sum(1 for s in sm.source.values() if s == "plain"), n_train=len(train))
+        if args.mode == "holdout":
+            hp = holdout_points(grid, args.holdout, args.seed)
+            hold, hcost = simulate_points(hp, args.replicas, args.nbeats, lat, seed0=args.seed, progress=log)
+            report.update(holdout_cost=hcost.__dict__,
+                          **holdout_summary(grid, sm, hold, hcost, args.replicas, args.nbeats),
+                          points=[dict(ku2=p.ku2, ikr=p.ikr, cl=p.cl, steered=sm.values[p], source=sm.source[p],
+                                       predicted=sm.predicted.get(p, (None, None))[0],
+                                       plain=hold[p].ead if p in hold else None) for p in grid])
         if args.mode == "compare":
             # the judge: the plain method on every grid point (same seeds), timed the same way
             res, cost = simulate_points(grid, args.replicas, args.nbeats, lat, seed0=args.seed, progress=log)

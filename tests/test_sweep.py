@@ -5,8 +5,8 @@ import math
 import numpy as np
 import pytest
 
-from ryr_hyperactivity_amd.sweep import (Point, PointResult, grid_points, plan_batches, point_cost, point_seed,
-                                         summarize, wilson)
+from ryr_hyperactivity_amd.sweep import (Cost, Point, PointResult, SteeredMap, grid_points, holdout_points,
+                                         holdout_summary, plan_batches, point_cost, point_seed, summarize, wilson)
 
 
 def test_wilson_interval():
@@ -52,6 +52,30 @@ def test_batch_plan_merges_cycle_lengths_only_when_latency_bound():
     big = plan_batches(pts, replicas=512, nbeats=5, ncru=20000)        # each length fills the chip
     assert [len(g) for g, _ in big] == [4, 4, 4] and [s for _, s in big] == sorted([s for _, s in big], reverse=True)
     assert [p for g, _ in big for p in g] and {p for g, _ in big for p in g} == set(pts)
+
+
+def test_holdout_draw_and_summary():
+    grid = grid_points([1, 2, 3, 4], [0, 0.5, 1], [1000.0, 2000.0])
+    h = holdout_points(grid, 5, seed=3)
+    assert len(set(h)) == 5 and set(h) <= set(grid) and h == holdout_points(grid, 5, seed=3)
+    assert [grid.index(p) for p in h] == sorted(grid.index(p) for p in h)
+    assert len(holdout_points(grid, 100)) == len(grid)
+    # steered values: two simulated points (exact), two predictions
+    a, b, c, d = grid[0], grid[1], grid[2], grid[3]
+    sm = SteeredMap(values={a: 0.5, b: 0.0, c: 0.02, d: 0.9}, source={a: "train", b: "plain", c: "surrogate",
+                                                                      d: "surrogate"},
+                    cost=Cost(cell_steps=1000, gpu_s=3.0, train_s=1.0))
+    hold = {p: PointResult(p, [], ead=v) for p, v in ((a, 0.5), (b, 0.0), (c, 0.1), (d, 0.7))}
+    r = holdout_summary(grid, sm, hold, Cost(cell_steps=500, gpu_s=2.0), replicas=4, nbeats=3)
+    plain = sum(point_cost(p, 4, 3) for p in grid)
+    assert r["plain_cell_steps"] == plain
+    assert r["plain_gpu_s_estimate"] == pytest.approx(2.0 / 500 * plain)
+    assert r["gpu_time_ratio"] == pytest.approx(4.0 / (2.0 / 500 * plain))
+    assert r["map_err_mean"] == pytest.approx((0.08 + 0.2) / 4) and r["map_err_max"] == pytest.approx(0.2)
+    assert r["map_err_surrogate_mean"] == pytest.approx(0.14) and r["holdout_surrogate"] == 2
+    assert r["resimulated_equal"]
+    hold[b] = PointResult(b, [], ead=0.05)
+    assert not holdout_summary(grid, sm, hold, Cost(cell_steps=500, gpu_s=2.0), 4, 3)["resimulated_equal"]
 
 
 @pytest.mark.gpu
