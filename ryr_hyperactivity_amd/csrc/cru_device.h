@@ -38,6 +38,17 @@
 #define RYR_NRYR 100
 #define RYR_DEV __device__ __forceinline__
 
+/* A uniform base pointer passed through an empty asm: an access through the
+ * result forms its 64-bit address where it stands, rather than reusing one
+ * computed (and kept live in two VGPRs) for an earlier access to the same
+ * element. */
+template <typename T>
+RYR_DEV T *late_base(T *p)
+{
+    __asm__ volatile("" : "+s"(p));
+    return p;
+}
+
 /* ---------------------------------------------------------------- structs */
 struct RyrCellInfo {
     int X, Y, Z, N;
@@ -1778,15 +1789,14 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L)
             cs1[g] = csn;
             cjsr1[g] = cjsrv + dotcjsr * (TS)dtT;
             cjsrbp[g] = cjsrb_new;
-            a.lccw[g] = lw;
+            /* the gating words' bases pass through late_base, so that these
+             * stores form their addresses here instead of keeping the loads'
+             * 64-bit addresses live across the kernel (which spills in the
+             * 168-VGPR instances) */
+            late_base(a.lccw)[g] = lw;
             if (!GSPLIT)
-                a.ryrw[g] = (uint32_t)nou | ((uint32_t)ncu << 8) | ((uint32_t)nob << 16) | ((uint32_t)ncb << 24);
-            /* the base passes through an empty asm so that the store forms its
-             * address here instead of keeping the load's 64-bit address live
-             * across the kernel (which spills in the 168-VGPR instances) */
-            uint32_t *auxp = a.aux;
-            __asm__ volatile("" : "+s"(auxp));
-            auxp[g] = (uint32_t)(nl & 0xffff) | ((uint32_t)nspark << 16) | (lmask << RYR_AUX_MASK_SHIFT);
+                late_base(a.ryrw)[g] = (uint32_t)nou | ((uint32_t)ncu << 8) | ((uint32_t)nob << 16) | ((uint32_t)ncb << 24);
+            late_base(a.aux)[g] = (uint32_t)(nl & 0xffff) | ((uint32_t)nspark << 16) | (lmask << RYR_AUX_MASK_SHIFT);
             if (diag) {
                 a.d_xire[g] = xire;
                 a.d_xinaca[g] = xinaca;
