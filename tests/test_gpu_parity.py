@@ -37,7 +37,7 @@ def _assert_same_state(eng, cell, o, label):
 
 
 @pytest.mark.parametrize("scheme", [SCHEME_A, SCHEME_B])
-@pytest.mark.parametrize("dims", [(7, 5, 3), (20, 13, 5), (1, 3, 4)])
+@pytest.mark.parametrize("dims", [(7, 5, 3), (20, 13, 5), (1, 3, 4), (6, 1, 2), (3, 2, 1)])
 def test_fp64_bitexact_through_an_action_potential(scheme, dims):
     p = Params.default(scheme=scheme, stim_offset=2.0)
     d = CellDesc.default(nx=dims[0], ny=dims[1], nz=dims[2], cl=1000.0, seed=11)
@@ -102,6 +102,36 @@ def test_batch_equals_cells_run_alone():
                 assert np.array_equal(one.scalars()[0], bs[c])
                 for f in ("CI", "CS", "CJSR", "NOU", "LCC"):
                     assert np.array_equal(one.field(0, f), batch.field(c, f))
+
+
+def test_full_size_scheme_a_conserves_calcium_up_to_sarcolemmal_flux():
+    """The GPU path at the full 100x20x10 size, checked by a property that
+    holds at any size: under scheme A the change of Total_Ca over a step
+    (cuda_Cell_Detailed.cu:1031-1035) equals dt * (Vs*sum I_NCX - (1+gamma)
+    Vp*sum I_CaL) / N, the only sarcolemmal fluxes (the identity the oracle
+    obeys; test_oracle_closed_forms).  The window holds the stimulus and the
+    upstroke, so I_CaL and I_NCX are both active."""
+    p = Params.default(scheme=SCHEME_A, steps_per_output=1, stim_offset=1.0)
+    d = CellDesc.default(nx=100, ny=20, nz=10, cl=1000.0, seed=5)
+    print("case full-size FP64 calcium balance 100x20x10", flush=True)
+    N = d.ncru
+    cm = 45.0 * N / 16120.0
+    checked = 0
+    with CellEngine(p, d) as eng:
+        eng.step(1)
+        prev = eng.diag()[0][0, 44]
+        for step in range(1, 161):
+            c0 = eng.counters()[0].copy()
+            eng.step(1)
+            row = eng.diag()[0][0]
+            sc = eng.scalars()[0]
+            if np.array_equal(eng.counters()[0], c0):
+                flux = p.dt * (sc[16] * cm / 0.0965 - sc[15] * cm / (0.0965 * 2.0)) / N
+                assert row[44] - prev == pytest.approx(flux, rel=1e-6, abs=1e-12), step
+                checked += 1
+            prev = row[44]
+        assert abs(eng.scalars()[0][15]) > 0.0           # I_CaL flowed inside the window
+    assert checked > 100
 
 
 def test_fp32_production_tracks_fp64():
