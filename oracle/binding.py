@@ -1,7 +1,8 @@
 """ctypes binding of the FP64 CPU oracle (liboracle_ryr.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the
-benchmark's CPU-baseline leg, as the checker.  The product never imports it.
+benchmark's CPU-baseline leg, as the reference results are compared with.  The
+product never imports it.
 `OracleCell` exposes the same reads as ryr_hyperactivity_amd.CellEngine for
 one cell so that parity tests compare like with like.
 """

@@ -3,7 +3,8 @@
  * whole-cell reduction, membrane ODE and output-row diagnostics.
  *
  * TEST INFRASTRUCTURE: only tests/, the smoke check and the benchmark's CPU
- * baseline use this file, as the checker.  Nothing in the product links it.
+ * baseline use this file, as the reference results are compared with.
+ * Nothing in the product links it.
  *
  * Reference files (colinmrees/RyR_hyperactivity):
  *   cuda_Cell_Detailed.cu  - constants :29-145, host ICs :350-366,

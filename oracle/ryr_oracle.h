@@ -1,7 +1,7 @@
 /*
  * ryr_oracle.h - FP64 CPU oracle of the CRU update.  TEST INFRASTRUCTURE
  * ONLY: the test suite, the smoke check and the benchmark's CPU-baseline leg
- * use it as the checker; the product path never links it.
+ * compare results with it; the product path never links it.
  *
  * It restates the reference's algorithm function by function (citations in
  * ryr_oracle.c) in the reference's own data layout: ghost-shell lattice,

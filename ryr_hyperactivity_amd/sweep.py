@@ -322,7 +322,7 @@ def main(argv=None):
                                        predicted=sm.predicted.get(p, (None, None))[0],
                                        plain=hold[p].ead if p in hold else None) for p in grid])
         if args.mode == "compare":
-            # the judge: the plain method on every grid point (same seeds), timed the same way
+            # the plain method on every grid point (same seeds), timed the same way
             res, cost = simulate_points(grid, args.replicas, args.nbeats, lat, seed0=args.seed, progress=log)
             err = np.array([abs(sm.values[p] - res[p].ead) for p in grid])
             sur = np.array([sm.source[p] == "surrogate" for p in grid])
