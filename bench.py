@@ -167,6 +167,19 @@ def cpu_baseline(p, cell, nsteps):
             "wall_s": wall}
 
 
+def ead_map_from_profiles():
+    """The metric's third part, the steered EAD map's GPU time against the
+    plain map's, is a sweep of minutes, not a bench line: it is reported
+    from its committed run (profiles/ead_map.json), marked as such."""
+    path = os.path.join(ROOT, "profiles", "ead_map.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    d["measured_in_this_run"] = False
+    return d
+
+
 def traffic_from_profiles(kernel, workload):
     """HBM bytes per launch of `kernel` on `workload` from the committed PMC
     summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the factor
@@ -243,6 +256,7 @@ def main():
         "roofline": roofline(r, wl),
         "cpu_baseline": cpu,
         "batch": batch,
+        "ead_map": ead_map_from_profiles(),
     }
     print(json.dumps(out), flush=True)
 
