@@ -46,6 +46,7 @@ def lib():
             "orc_read_linescan": (i, [vp, _D, i64]),
             "orc_read_counters": (i, [vp, ctypes.POINTER(i64)]),
             "orc_num_threads": (i, []),
+            "orc_set_num_threads": (None, [i]),
             "orc_set_vclamp": (i, [vp, _D, i64, i]),
             "orc_read_beats": (i, [vp, _D, i, ctypes.POINTER(i)]),
             "orc_ncx": (d, [d, d, d, _D, d]),

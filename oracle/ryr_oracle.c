@@ -1385,7 +1385,7 @@ static void orc_one_step(orc_cell *c)
 #pragma omp parallel
     {
         int64_t loc[RYR_C_COUNT] = {0};
-#pragma omp for schedule(static)
+#pragma omp for collapse(3) schedule(static)
         for (int k = 1; k <= c->Z; ++k)
             for (int j = 1; j <= c->Y; ++j)
                 for (int i = 1; i <= c->X; ++i)
@@ -1457,6 +1457,15 @@ static void orc_one_step(orc_cell *c)
 void orc_step(orc_cell *c, int64_t nsteps)
 {
     for (int64_t s = 0; s < nsteps; ++s) orc_one_step(c);
+}
+
+void orc_set_num_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
 }
 
 int orc_num_threads(void)

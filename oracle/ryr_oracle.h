@@ -34,6 +34,7 @@ int orc_read_diag(const orc_cell *c, double *out, int64_t *step);
 int orc_read_linescan(const orc_cell *c, double *out, int64_t n);
 int orc_read_counters(const orc_cell *c, int64_t *out);
 int orc_num_threads(void);
+void orc_set_num_threads(int n);   /* OpenMP threads of the CRU loop (n > 0) */
 int orc_set_vclamp(orc_cell *c, const double *v, int64_t n, int periodic);
 int orc_read_beats(const orc_cell *c, double *out, int maxbeats, int *nbeats);
 
