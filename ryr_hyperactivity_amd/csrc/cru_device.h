@@ -163,8 +163,7 @@ struct RyrKArgs {
     const long long *line_off;
     double *beats;          /* per cell x RYR_MAX_BEATS x RYR_BEAT_COLS */
     RyrDD *dpartial;        /* output-row chunk sums: per block x 21 */
-    unsigned long long *dcount;   /* output-row integer counts: per cell x 13 */
-    unsigned int *darrive;  /* per cell */
+    unsigned long long *dcount;   /* output-row integer counts: per chunk x 13 */
 };
 
 #define RYR_NCNT 5
@@ -223,6 +222,31 @@ RYR_DEV uint32_t wave_sum_u32(uint32_t x)
     x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x140, 0xF, 0xF, true);
     return (uint32_t)__builtin_amdgcn_readlane((int)x, 0) + (uint32_t)__builtin_amdgcn_readlane((int)x, 16) +
            (uint32_t)__builtin_amdgcn_readlane((int)x, 32) + (uint32_t)__builtin_amdgcn_readlane((int)x, 48);
+}
+
+/* the same for 64-bit values: each DPP step moves both halves and adds
+ * with carry */
+RYR_DEV unsigned long long wave_sum_u64(unsigned long long x)
+{
+#define RYR_DPP_U64(CTRL)                                                                              \
+    do {                                                                                               \
+        const uint32_t lo_ = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)x, CTRL, 0xF, 0xF, true); \
+        const uint32_t hi_ = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(x >> 32), CTRL, 0xF, 0xF, true); \
+        x += ((unsigned long long)hi_ << 32) | lo_;                                                    \
+    } while (0)
+    RYR_DPP_U64(0xB1);
+    RYR_DPP_U64(0x4E);
+    RYR_DPP_U64(0x141);
+    RYR_DPP_U64(0x140);
+#undef RYR_DPP_U64
+    unsigned long long t = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, 16 * r);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), 16 * r);
+        t += ((unsigned long long)hi << 32) | lo;
+    }
+    return t;
 }
 
 /* Pairwise tree over the 64 lanes of a wave in index order: level k joins
@@ -1987,11 +2011,13 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_membrane(RyrKArgs a, int km, doub
 }
 
 /* ------------------------------------------------------- output-row kernel */
-/* One workgroup per 256-CRU chunk (the step kernel's chunks): each forms the
- * pairwise double-double sums of its chunk and the integer counts; the last
- * workgroup of a cell adds the chunk sums in chunk order (the same order as
- * the step reduction and the oracle) and writes the row
- * (cuda_Cell_Detailed.cu:839-962, :1031-1072). */
+/* Output row in two launches, so that no workgroup needs a fence (on this
+ * GPU an agent-scope release writes back the L2):
+ *   k_diag: one workgroup per 256-CRU chunk forms the pairwise double-double
+ *     sums of its chunk, its integer counts and its linescan entries;
+ *   k_diag_cell: one workgroup per cell adds the chunk sums in chunk order
+ *     (the same order as the step reduction and the oracle) and writes the
+ *     row (cuda_Cell_Detailed.cu:839-962, :1031-1072). */
 #define RYR_DIAG_Q 21
 #define RYR_DIAG_I 13
 template <typename TF, typename TS>
@@ -1999,7 +2025,6 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_diag(RyrKArgs a)
 {
     __shared__ RyrDD s_red[RYR_DIAG_Q][RYR_BLOCK / 64];
     __shared__ unsigned long long s_cnt[RYR_DIAG_I];
-    __shared__ int s_last;
     const int b = blockIdx.x;
     const int c = a.block_cell[b];
     const RyrCellInfo C = a.cells[c];
@@ -2108,16 +2133,40 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_diag(RyrKArgs a)
 #pragma unroll
         for (int q = 0; q < RYR_DIAG_Q; ++q)
             pp[q] = dd_add(dd_add(s_red[q][0], s_red[q][1]), dd_add(s_red[q][2], s_red[q][3]));
-        unsigned long long *cc = a.dcount + (long long)c * RYR_DIAG_I;
-        for (int q = 0; q < RYR_DIAG_I; ++q)
-            if (s_cnt[q]) atomicAdd(cc + q, s_cnt[q]);
-        __threadfence();
-        unsigned int prev = atomicAdd(a.darrive + c, 1u);
-        s_last = (prev == (unsigned int)(C.nblocks - 1));
+        unsigned long long *cc = a.dcount + (long long)b * RYR_DIAG_I;
+        for (int q = 0; q < RYR_DIAG_I; ++q) cc[q] = s_cnt[q];
     }
+}
+
+/* one workgroup per cell, after k_diag: chunk sums in chunk order, the row */
+template <typename TF, typename TS>
+__global__ __launch_bounds__(RYR_BLOCK) void k_diag_cell(RyrKArgs a)
+{
+    __shared__ RyrDD s_red[RYR_DIAG_Q][RYR_BLOCK / 64];
+    __shared__ unsigned long long s_cnt[RYR_DIAG_I];
+    const int c = blockIdx.x;
+    const RyrCellInfo C = a.cells[c];
+    RyrMem *M = a.mem + c;
+    if (a.diag_step[c] != M->step - 1) return;   /* not an output step */
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (threadIdx.x < RYR_DIAG_I) s_cnt[threadIdx.x] = 0;
     __syncthreads();
-    if (!s_last) return;
-    __threadfence();
+    /* counts: each lane adds its chunks' (integers, any order) */
+    {
+        unsigned long long ic[RYR_DIAG_I] = {0};
+        for (int bb = (int)threadIdx.x; bb < C.nblocks; bb += RYR_BLOCK) {
+            const unsigned long long *cc = a.dcount + (long long)(C.first_block + bb) * RYR_DIAG_I;
+#pragma unroll
+            for (int q = 0; q < RYR_DIAG_I; ++q) ic[q] += cc[q];
+        }
+        /* a wave sum first: the lanes would otherwise queue on the same 13
+         * LDS words */
+#pragma unroll
+        for (int q = 0; q < RYR_DIAG_I; ++q) {
+            const unsigned long long w = wave_sum_u64(ic[q]);
+            if (lane == 0 && w) atomicAdd(&s_cnt[q], w);
+        }
+    }
     /* chunk sums: pairwise trees over groups of 256 chunks, groups in order */
     RyrDD acc[RYR_DIAG_Q];
 #pragma unroll
@@ -2142,9 +2191,8 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_diag(RyrKArgs a)
     if (threadIdx.x != 0) return;
     {
         RYR_STRICT_FP
-        unsigned long long *cc = a.dcount + (long long)c * RYR_DIAG_I;
         unsigned long long cnt[RYR_DIAG_I];
-        for (int q = 0; q < RYR_DIAG_I; ++q) { cnt[q] = __hip_atomic_load(cc + q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); cc[q] = 0ull; }
+        for (int q = 0; q < RYR_DIAG_I; ++q) cnt[q] = s_cnt[q];
         const double dN = C.dN;
         double *r = a.diag + (long long)c * 45;
         double ncxfwd = acc[0].hi / dN, outAncx = acc[1].hi / dN, cproxit = acc[2].hi / dN;
@@ -2178,8 +2226,6 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_diag(RyrKArgs a)
             else M->b_sparks += (double)cnt[11];
             if (cit > M->b_peak_ci) M->b_peak_ci = cit;
         }
-        a.darrive[c] = 0u;
-        __threadfence();
     }
 }
 

@@ -136,8 +136,11 @@ static hipError_t ryr_go_cru(const RyrKArgs &a, int nchunks, int wg, int kc, boo
     hipError_t ryr_launch_diag_##sfx(const RyrKArgs &a, int nchunks, hipStream_t s,             \
                                      hipEvent_t *ev)                                            \
     {                                                                                            \
-        /* one workgroup per 256-CRU chunk */                                                    \
-        RYR_GO((k_diag<TF, TS>), dim3(nchunks), dim3(RYR_BLOCK), s, ev, a);                      \
+        /* one workgroup per 256-CRU chunk, then one per cell */                                  \
+        if (ev && hipEventRecord(ev[0], s) != hipSuccess) return hipGetLastError();              \
+        hipLaunchKernelGGL((k_diag<TF, TS>), dim3(nchunks), dim3(RYR_BLOCK), 0, s, a);           \
+        hipLaunchKernelGGL((k_diag_cell<TF, TS>), dim3(a.ncells), dim3(RYR_BLOCK), 0, s, a);     \
+        if (ev && hipEventRecord(ev[1], s) != hipSuccess) return hipGetLastError();              \
         return hipGetLastError();                                                                \
     }                                                                                            \
     hipError_t ryr_launch_init_##sfx(const RyrKArgs &a, const RyrInit *init, int nblocks,        \

@@ -321,8 +321,7 @@ extern "C" int ryr_create(int device, const ryr_params *p, const ryr_cell_desc *
     ALLOCT(a.gstep, sizeof(long long));
     ALLOC(a.beats, sizeof(double) * ncells * RYR_MAXB * RYR_BCOLS);
     ALLOCT(a.dpartial, sizeof(RyrDD) * RYR_DIAG_Q * h->nblocks);
-    ALLOCT(a.dcount, sizeof(unsigned long long) * RYR_DIAG_I * ncells);
-    ALLOCT(a.darrive, sizeof(unsigned int) * ncells);
+    ALLOCT(a.dcount, sizeof(unsigned long long) * RYR_DIAG_I * h->nblocks);
 #undef ALLOC
 #undef ALLOCT
 #undef ALLOCX
