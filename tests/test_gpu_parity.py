@@ -60,6 +60,27 @@ def test_fp64_bitexact_through_an_action_potential(scheme, dims):
         assert eng.scalars()[0][S["V"]] > -60.0
 
 
+def test_fp64_bitexact_on_a_cell_of_more_than_256_chunks():
+    """A 160x20x21 cell (67,200 CRUs, 263 chunks): the whole-cell sums and
+    the output-row sums run their loops over groups of 256 chunks more than
+    once, and the sums of the second group join the first in chunk order."""
+    p = Params.default(stim_offset=1.0)
+    d = CellDesc.default(nx=160, ny=20, nz=21, cl=1000.0, seed=13)
+    print("case 160x20x21 FP64, 263 chunks", flush=True)
+    o = OracleCell(p, d)
+    with CellEngine(p, d) as eng:
+        for chunk in (1, 100):                        # an eager step, then a graph-replayed interval
+            eng.step(chunk)
+            o.step(chunk)
+        _assert_same_state(eng, 0, o, "after 101 steps")
+        grow, gst = eng.diag()
+        orow, ost = o.diag()
+        assert gst == ost == 100
+        assert np.array_equal(grow[0], orow), np.nonzero(grow[0] != orow)
+        assert np.array_equal(eng.linescan(0), o.linescan())
+        assert np.array_equal(eng.counters()[0], o.counters())
+
+
 def test_gating_probe_bitexact_including_gaussian_branch():
     rng = np.random.default_rng(3)
     n = 4096
