@@ -1904,11 +1904,17 @@ RYR_DEV void membrane_block(const RyrKArgs &a, int c, long long nstep, double st
     I.s = M->m; I.svikr = C.svikr; I.Cm = C.Cm; I.dN = C.dN; I.dt = dt;
     I.slowncx = (C.variants & RYR_VAR_SLOWNCX) != 0;
 
-    /* this wave's first slice of wave partials is loaded before the current
-     * groups run, so that the loads' latency hides behind them */
+    /* Slice sl of a group of 256 chunks (chunks 64 sl .. 64 sl + 63) is summed
+     * by wave NW - 1 - (sl mod NW): the slices go first to the waves whose
+     * current groups are shortest (I_NaK on wave 3), and a slice past the
+     * cell's last chunk is zero without a tree.  Which wave sums a slice does
+     * not change the sum.  This wave's first slice of wave partials is loaded
+     * before the current groups run, so that the loads' latency hides behind
+     * them. */
+    const int sl0 = NW - 1 - wv;
     RyrDD w0[3][4];
     {
-        const int bb = wv * 64 + lane;
+        const int bb = sl0 * 64 + lane;
 #pragma unroll
         for (int q = 0; q < 3; ++q)
 #pragma unroll
@@ -1932,13 +1938,18 @@ RYR_DEV void membrane_block(const RyrKArgs &a, int c, long long nstep, double st
     /* groups of 256 chunk sums, each a pairwise tree of four 64-chunk wave
      * trees (the same tree whatever the number of waves WG gives) */
     for (int g0 = 0; g0 < C.nblocks; g0 += 256) {
-        for (int sl = wv; sl < 4; sl += NW) {
+        for (int sl = sl0; sl < 4; sl += NW) {
             const int bb = g0 + sl * 64 + lane;
             RyrDD t[3];
+            if (g0 + sl * 64 >= C.nblocks) {
+                if (lane == 0)
+                    for (int q = 0; q < 3; ++q) { s_red[q][sl].hi = 0.0; s_red[q][sl].lo = 0.0; }
+                continue;
+            }
 #pragma unroll
             for (int q = 0; q < 3; ++q) {
                 /* chunk sum = pairwise tree over its four wave partials */
-                if (g0 == 0 && sl == wv) {
+                if (g0 == 0 && sl == sl0) {
                     t[q] = dd_add(dd_add(w0[q][0], w0[q][1]), dd_add(w0[q][2], w0[q][3]));
                 } else if (bb < C.nblocks) {
                     const RyrDD *w = pp + 12LL * bb + q;
