@@ -595,6 +595,7 @@ static void sched_candidates(const ryr_handle *h, std::vector<std::array<int, 3>
     add(h->wg_cyto, 512, RYR_SCHED_SEQ);
     /* the RyR gating beside the sub-voxel sweep and the membrane step */
     add(256, 256, RYR_SCHED_FUSED_GATING);
+    add(256, 128, RYR_SCHED_FUSED_GATING);   /* the gating-free CRU kernel over twice as many CUs */
     add(256, 512, RYR_SCHED_FUSED_GATING);
     add(h->wg_cyto, 256, RYR_SCHED_FUSED_GATING);
     if (h->p.scheme == RYR_SCHEME_A) {
