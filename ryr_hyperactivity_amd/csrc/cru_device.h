@@ -953,9 +953,11 @@ RYR_DEV void mem_sums(const MemIn &I, int lane, double *sh)
     const double qks1 = qks + dt * (qks_inf - qks) / tauqks;
     const double xicatto = lanev(q1, 1) * 0.0965 * (0.00126) * 2.0 * ((0.5) + 1.);
     const double xinacato = lanev(q1, 2) * 0.0965 * (0.025);
-    if (lane == 0) {
-        sh[RYR_MS_CSUBT] = csubt; sh[RYR_MS_QKS] = qks1; sh[RYR_MS_ICAT] = xicatto; sh[RYR_MS_INACA] = xinacato;
-    }
+    /* Every lane holds the same four values and writes them: the lanes of
+     * this wave read these slots next with no barrier in between, so each
+     * lane must read its own write (a write by lane 0 alone would be a race
+     * between divergent branches that the compiler may order either way). */
+    sh[RYR_MS_CSUBT] = csubt; sh[RYR_MS_QKS] = qks1; sh[RYR_MS_ICAT] = xicatto; sh[RYR_MS_INACA] = xinacato;
 }
 
 /* [Na]i and V (:701-738) from the groups' currents; fills slots V and NAI */
@@ -1965,26 +1967,34 @@ RYR_DEV void membrane_block(const RyrKArgs &a, int c, long long nstep, double st
             acc[q] = dd_add(acc[q], dd_add(dd_add(s_red[q][0], s_red[q][1]), dd_add(s_red[q][2], s_red[q][3])));
         __syncthreads();
     }
-    if (wv != 0) return;
-
-    /* wave 0: the parts that need the sums, then [Na]i and V */
-    I.sum_icat = acc[0].hi; I.sum_inaca = acc[1].hi; I.sum_cs = acc[2].hi;
-    mem_sums(I, lane, s_ms);
-
+    /* wave 0: the parts that need the sums, then [Na]i and V.  With more than
+     * one wave, wave 1 then forms the next step's V-only factors while wave 0
+     * stores the membrane state and tracks the beat. */
+    __shared__ double s_vx[2];
     const double stim = stim_of(C, nstep, stim_amp);
-    double mo[13], cu[10];
-    mem_finish(M->m, s_ms, dt, stim, &mo[0], &mo[12]);
+    double vnew = 0.0, xnai_new = 0.0, mo[13], cu[10];
+    if (wv == 0) {
+        I.sum_icat = acc[0].hi; I.sum_inaca = acc[1].hi; I.sum_cs = acc[2].hi;
+        mem_sums(I, lane, s_ms);
+        mem_finish(M->m, s_ms, dt, stim, &mo[0], &mo[12]);
 #pragma unroll
-    for (int q = 1; q < 12; ++q) mo[q] = s_ms[q];
+        for (int q = 1; q < 12; ++q) mo[q] = s_ms[q];
 #pragma unroll
-    for (int q = 0; q < 10; ++q) cu[q] = s_ms[RYR_MS_CSUBT + q];
-    double vnew = mo[0];
-    if (C.vclamp_n > 0) {
-        /* clamp protocols (cuda_Cell_Detailed.cu:733-773) */
-        long long q = C.vclamp_periodic ? nstep % C.vclamp_n : (nstep < C.vclamp_n ? nstep : C.vclamp_n - 1);
-        vnew = C.vclamp[q];
+        for (int q = 0; q < 10; ++q) cu[q] = s_ms[RYR_MS_CSUBT + q];
+        vnew = mo[0];
+        if (C.vclamp_n > 0) {
+            /* clamp protocols (cuda_Cell_Detailed.cu:733-773) */
+            long long q = C.vclamp_periodic ? nstep % C.vclamp_n : (nstep < C.vclamp_n ? nstep : C.vclamp_n - 1);
+            vnew = C.vclamp[q];
+        }
+        xnai_new = mo[12];
+        if (lane == 0) { s_vx[0] = vnew; s_vx[1] = xnai_new; }
     }
-    const double xnai_new = mo[12];
+    if constexpr (NW > 1) {
+        __syncthreads();
+        if (wv == 1) vfactors_wave(M, a.iso2, lane, s_vx[0], s_vx[1]);
+    }
+    if (wv != 0) return;
     if (lane == 0) {
         const double vold = M->m[0];
         for (int q = 0; q < 13; ++q) M->m[q] = mo[q];
@@ -2010,7 +2020,7 @@ RYR_DEV void membrane_block(const RyrKArgs &a, int c, long long nstep, double st
             a.diag_step[c] = nstep;
         }
     }
-    vfactors_wave(M, a.iso2, lane, vnew, xnai_new);
+    if constexpr (NW == 1) vfactors_wave(M, a.iso2, lane, vnew, xnai_new);
     if (lane == 0) M->step = nstep + 1;
 }
 
