@@ -1539,6 +1539,14 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L)
         TF cpv = cp0[g];
         const TS cjsrv = cjsr0[g];
         uint32_t lw = a.lccw[g], rw = a.ryrw[g], ax = a.aux[g];
+        /* the draw blocks of LCC channels 0-3 need only the key, so in the
+         * instances that have the registers they are formed while the state
+         * loads are in flight */
+        ryr_u32x4 lpre[2] = {};
+        if constexpr (LPC == 1 && vf_early) {
+            lpre[0] = ryr_block(C.seed, (uint32_t)n, (uint64_t)nstep, 0u);
+            lpre[1] = ryr_block(C.seed, (uint32_t)n, (uint64_t)nstep, 1u);
+        }
         if (!GSPLIT && reset) ax &= ~0x1ffffu;      /* nl and spark flag; the LCC slots stay (GSPLIT: done by the gating) */
         int nl = (int)(ax & 0xffffu), nspark = (int)((ax >> 16) & 1u);
         uint32_t lmask = (ax >> RYR_AUX_MASK_SHIFT) & 0xffu;
@@ -1573,7 +1581,9 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L)
                     for (int ll = 0; ll < 8; ++ll) {
                         if (ll < npre) {
                             int st = (int)((lw >> (4 * ll)) & 15u);
-                            if (!(ll & 1)) lb = ryr_block(C.seed, (uint32_t)n, (uint64_t)nstep, (uint32_t)(ll >> 1));
+                            if (!(ll & 1))
+                                lb = (vf_early && ll < 4) ? lpre[ll >> 1]
+                                                          : ryr_block(C.seed, (uint32_t)n, (uint64_t)nstep, (uint32_t)(ll >> 1));
                             const double ragg = (ll & 1) ? ryr_slot_uniform(lb.v[2], lb.v[3]) : ryr_slot_uniform(lb.v[0], lb.v[1]);
                             int jj = lcc_step(st, cpx, dt, ragg, vf, betac);
                             nw = (nw & ~(15u << (4 * ll))) | ((uint32_t)jj << (4 * ll));
