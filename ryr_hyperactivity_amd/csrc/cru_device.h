@@ -1577,13 +1577,30 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L)
                 ryr_u32x4 lb = {{0u, 0u, 0u, 0u}};
                 if constexpr (LPC == 1) {
                     const int npre = __builtin_ctz(~lmask);          /* present slots before the first absent */
+                    /* channels 0-3 (every slot of the default four-LCC CRU)
+                     * are stepped with no branch between them, so that their
+                     * divisions interleave; a result past npre is dropped */
+                    int jlo[4];
 #pragma unroll
-                    for (int ll = 0; ll < 8; ++ll) {
+                    for (int ll = 0; ll < 4; ++ll) {
+                        const int st = (int)((lw >> (4 * ll)) & 15u);
+                        if (!(ll & 1))
+                            lb = vf_early ? lpre[ll >> 1]
+                                          : ryr_block(C.seed, (uint32_t)n, (uint64_t)nstep, (uint32_t)(ll >> 1));
+                        const double ragg = (ll & 1) ? ryr_slot_uniform(lb.v[2], lb.v[3]) : ryr_slot_uniform(lb.v[0], lb.v[1]);
+                        jlo[ll] = lcc_step(st, cpx, dt, ragg, vf, betac);
+                    }
+#pragma unroll
+                    for (int ll = 0; ll < 4; ++ll)
+                        if (ll < npre) {
+                            nw = (nw & ~(15u << (4 * ll))) | ((uint32_t)jlo[ll] << (4 * ll));
+                            if (jlo[ll] == 0) ++nlcp;
+                        }
+#pragma unroll
+                    for (int ll = 4; ll < 8; ++ll) {
                         if (ll < npre) {
                             int st = (int)((lw >> (4 * ll)) & 15u);
-                            if (!(ll & 1))
-                                lb = (vf_early && ll < 4) ? lpre[ll >> 1]
-                                                          : ryr_block(C.seed, (uint32_t)n, (uint64_t)nstep, (uint32_t)(ll >> 1));
+                            if (!(ll & 1)) lb = ryr_block(C.seed, (uint32_t)n, (uint64_t)nstep, (uint32_t)(ll >> 1));
                             const double ragg = (ll & 1) ? ryr_slot_uniform(lb.v[2], lb.v[3]) : ryr_slot_uniform(lb.v[0], lb.v[1]);
                             int jj = lcc_step(st, cpx, dt, ragg, vf, betac);
                             nw = (nw & ~(15u << (4 * ll))) | ((uint32_t)jj << (4 * ll));
