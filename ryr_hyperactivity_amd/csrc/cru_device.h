@@ -1547,6 +1547,20 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L)
             lpre[0] = ryr_block(C.seed, (uint32_t)n, (uint64_t)nstep, 0u);
             lpre[1] = ryr_block(C.seed, (uint32_t)n, (uint64_t)nstep, 1u);
         }
+        /* the gating-free instances, which have the registers, also issue the
+         * cleft buffer, JSR buffer, scheme-B and vertical-neighbour loads
+         * here rather than at their use */
+        const bool has_up = n + sz < C.N, has_dn = n >= sz;
+        TF e_sar{}, e_sarh{}, e_sarj{}, e_sarhj{}, e_up{}, e_dn{}, e_sb0{}, e_sb1{}, e_ancx{};
+        TS e_cjb{};
+        if constexpr (GSPLIT) {
+            e_sar = casar[g]; e_sarh = casarh[g]; e_sarj = casarj[g]; e_sarhj = casarhj[g];
+            e_cjb = cjsrbp[g];
+            e_ancx = ancx[g];
+            if (has_up) e_up = cs0[g + sz];
+            if (has_dn) e_dn = cs0[g - sz];
+            if (SCHB) { e_sb0 = ((const TF *)a.sb)[g]; e_sb1 = ((const TF *)a.sb)[P + g]; }
+        }
         if (!GSPLIT && reset) ax &= ~0x1ffffu;      /* nl and spark flag; the LCC slots stay (GSPLIT: done by the gating) */
         int nl = (int)(ax & 0xffffu), nspark = (int)((ax >> 16) & 1u);
         uint32_t lmask = (ax >> RYR_AUX_MASK_SHIFT) & 0xffu;
@@ -1669,11 +1683,11 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L)
         TF ancx_new;
         if (perm) {                                 /* no NCX (:1302) */
             xinaca = 0.0;
-            ancx_new = ancx[g];
+            ancx_new = GSPLIT ? e_ancx : ancx[g];
         } else if constexpr (sizeof(TF) != 8) {
             const RyrVF vf = vf_early ? vf0 : load_vf(M);
             const float csm = (float)cs_d * 1e-3f;
-            const float Ka = (float)ancx[g];
+            const float Ka = (float)(GSPLIT ? e_ancx : ancx[g]);
             const float t2 = 669921.875f * csm * (1.0f + csm * (float)(1.0 / 0.00359));
             const float t3 = (float)vf.ncx_t3a + csm * 2744000.0f;
             const float Inaca = Ka * 21.0f * ((float)vf.ncx_n1 - (float)vf.ncx_n2 * csm) *
@@ -1686,7 +1700,7 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L)
             RYR_STRICT_FP
             const RyrVF vf = vf_early ? vf0 : load_vf(M);
             double cs = cs_d / 1000.0;
-            double Ka = (double)ancx[g];
+            double Ka = (double)(GSPLIT ? e_ancx : ancx[g]);
             double t2 = (87.5) * (87.5) * (87.5) * cs * (1.0 + cs / 0.00359);
             double t3 = vf.ncx_t3a + cs * ((140.0) * (140.0) * (140.0));
             double Inaca = Ka * (21.0 / 1.0) * (vf.ncx_n1 - vf.ncx_n2 * cs)
@@ -1721,22 +1735,23 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L)
                                        / ((600.0 + cj) * (600.0 + cj)));
             double flux = ((-(double)diffjn0 - (double)diffjn1) - xire * (0.00126) * 1.0 / (0.02 * 1));
             dotcjsr = (TS)(beta * flux);
-            cjsrb_new = (TS)((double)cjsrbp[g] + (1 - beta) * flux * dt);
+            cjsrb_new = (TS)((double)(GSPLIT ? e_cjb : cjsrbp[g]) + (1 - beta) * flux * dt);
         }
 
         /* cleft/sub-membrane sub-steps (:1356-1473 / :1475-1597); the
          * cytosol/NSR sub-voxels were advanced by k_cyto */
-        TF v_casar = casar[g], v_casarh = casarh[g], v_casarj = casarj[g], v_casarhj = casarhj[g];
+        TF v_casar = GSPLIT ? e_sar : casar[g], v_casarh = GSPLIT ? e_sarh : casarh[g];
+        TF v_casarj = GSPLIT ? e_sarj : casarj[g], v_casarhj = GSPLIT ? e_sarhj : casarhj[g];
         const TF csb = perm ? (TF)C.ci_bath : csv;             /* perm: bath outside the lattice */
-        const TF cs_up = (n + sz < C.N) ? cs0[g + sz] : csb;   /* k + 1 < Z */
-        const TF cs_dn = (n >= sz) ? cs0[g - sz] : csb;        /* k > 0 */
+        const TF cs_up = has_up ? (GSPLIT ? e_up : cs0[g + sz]) : csb;   /* k + 1 < Z */
+        const TF cs_dn = has_dn ? (GSPLIT ? e_dn : cs0[g - sz]) : csb;   /* k > 0 */
         const TF bcgT = (TF)bcgcur;
         const TF icatT = (TF)icat1, xinacaT = (TF)xinaca, xireT = (TF)xire;
         TF csn = csv;
         const TF *sbp = (const TF *)a.sb;
         if constexpr (sizeof(TF) != 8) {
             float dci[2] = {0.0f, 0.0f};
-            if (SCHB) { dci[0] = (float)sbp[g]; dci[1] = (float)sbp[P + g]; }
+            if (SCHB) { dci[0] = (float)(GSPLIT ? e_sb0 : sbp[g]); dci[1] = (float)(GSPLIT ? e_sb1 : sbp[P + g]); }
             float cpf = (float)cpv, csf = (float)csv;
             float sar = (float)v_casar, sarh = (float)v_casarh, sarj = (float)v_casarj, sarhj = (float)v_casarhj;
             fast_cleft<!SCHB>((float)dt, (float)cil[0], (float)cil[1], (float)csv, (float)cs_up,
@@ -1750,7 +1765,7 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L)
             v_casar = (TF)sar; v_casarh = (TF)sarh; v_casarj = (TF)sarj; v_casarhj = (TF)sarhj;
         } else {
         TF dotci01[2] = {TF(0), TF(0)};
-        if (SCHB) { dotci01[0] = sbp[g]; dotci01[1] = sbp[P + g]; }
+        if (SCHB) { dotci01[0] = GSPLIT ? e_sb0 : sbp[g]; dotci01[1] = GSPLIT ? e_sb1 : sbp[P + g]; }
         const TF dth = dtT / TF(5);
         if (!SCHB) {
             /* scheme A (:1396-1464) */
