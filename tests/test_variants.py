@@ -114,14 +114,21 @@ VARIANT_CASES = [
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("shape", [None, (256, 256, 3), (128, 64, 1)], ids=["tuned", "gating-fused", "fused"])
 @pytest.mark.parametrize("name,pk,dk", VARIANT_CASES, ids=[c[0] for c in VARIANT_CASES])
-def test_variant_fp64_bitexact_against_the_oracle(name, pk, dk):
+def test_variant_fp64_bitexact_against_the_oracle(name, pk, dk, shape):
+    """Every variant, FP64 against the oracle, at the tuned shape and forced
+    onto both CRU kernel families: the gating-free one of the gating-fused
+    schedule (state, buffer and neighbour loads issued together) and the
+    128-lane one that also gates (loads at their use)."""
     from ryr_hyperactivity_amd import CellEngine
-    print(f"case variant {name}", flush=True)
+    print(f"case variant {name} shape {shape}", flush=True)
     p = Params.default(stim_offset=1.0, **pk)
     d = CellDesc.default(nx=9, ny=5, nz=3, cl=10.0, seed=6, **dk)   # 400 steps per cycle: iso events by step 2200
     o = OracleCell(p, d)
     with CellEngine(p, d) as eng:
+        if shape is not None:
+            eng.set_launch_shape(*shape)
         for chunk in (1, 99, 700, 1500, 37):
             eng.step(chunk)
             o.step(chunk)
