@@ -60,6 +60,35 @@ def test_fp64_bitexact_through_an_action_potential(scheme, dims):
         assert eng.scalars()[0][S["V"]] > -60.0
 
 
+def test_fp64_bitexact_at_full_size_through_upstroke_and_release():
+    """The benchmark's own cell (100x20x10 CRUs, Ku2 2.5, I_Kr off) in FP64,
+    stimulated at 2 ms: bit-identical to the oracle through 400 ms, i.e. the
+    upstroke, the L-type trigger, the release and the plateau of the long
+    action potential, with the launch-shape tuning and graph-replayed
+    intervals on the way."""
+    p = Params.default(stim_offset=2.0)
+    d = CellDesc.default(nx=100, ny=20, nz=10, cl=2000.0, Ku2=2.5, ikr_scale=0.0, seed=1)
+    print("case 100x20x10 FP64 through 16000 steps", flush=True)
+    o = OracleCell(p, d)
+    with CellEngine(p, d) as eng:
+        done = 0
+        for chunk in (1, 99, 1900, 2000, 12000):
+            eng.step(chunk)
+            o.step(chunk)
+            done += chunk
+            _assert_same_state(eng, 0, o, f"full size after {done} steps")
+            print(f"  {done} steps equal", flush=True)
+            if done == 4000:
+                assert eng.scalars()[0][S["V"]] > -40.0           # on the plateau at 100 ms
+        grow, gst = eng.diag()
+        orow, ost = o.diag()
+        assert gst == ost and gst > 0
+        assert np.array_equal(grow[0], orow), np.nonzero(grow[0] != orow)
+        assert np.array_equal(eng.linescan(0), o.linescan())
+        assert np.array_equal(eng.counters()[0], o.counters())
+        assert np.array_equal(eng.beats(0), o.beats())
+
+
 def test_fp64_bitexact_on_a_cell_of_more_than_256_chunks():
     """A 160x20x21 cell (67,200 CRUs, 263 chunks): the whole-cell sums and
     the output-row sums run their loops over groups of 256 chunks more than
