@@ -89,6 +89,32 @@ def test_fp64_bitexact_at_full_size_through_upstroke_and_release():
         assert np.array_equal(eng.beats(0), o.beats())
 
 
+def test_fp64_full_batch_cells_match_the_oracle():
+    """The benchmark's 256-cell batch (3.07 M CRUs in three lattice sizes,
+    per-cell Ku2, I_Kr and cycle length) in FP64, at the launch shape the
+    tuner keeps for it: the first cell of each size is bit-identical to the
+    oracle's run of that cell alone after 2,030 steps through the stimulus
+    and the upstroke."""
+    import bench
+    _, cells = bench.make_batch(0)
+    p = Params.default(stim_offset=2.0)
+    pick, seen = [], set()
+    for c, d in enumerate(cells):
+        if (d.nx, d.ny, d.nz) not in seen:
+            seen.add((d.nx, d.ny, d.nz))
+            pick.append(c)
+    assert len(pick) == 3
+    print(f"case 256-cell batch FP64, cells {pick}", flush=True)
+    with CellEngine(p, cells) as eng:
+        for chunk in (1, 99, 130, 1800):
+            eng.step(chunk)
+        for c in pick:
+            o = OracleCell(p, cells[c])
+            o.step(2030)
+            _assert_same_state(eng, c, o, f"batch cell {c}")
+            print(f"  cell {c} equal", flush=True)
+
+
 def test_fp64_bitexact_on_a_cell_of_more_than_256_chunks():
     """A 160x20x21 cell (67,200 CRUs, 263 chunks): the whole-cell sums and
     the output-row sums run their loops over groups of 256 chunks more than
