@@ -16,6 +16,8 @@
  *                      holds two planes); the 3 partners in the same slot and
  *                      the x partner inside the span come from LDS after one
  *                      barrier, the y and z partners from memory;
+ *   two slots a lane   streams+partners with each lane taking two adjacent
+ *                      slots (8- and 16-byte loads and stores);
  *   split f64 pools    streams+partners with each f64 pool held as an f32
  *                      value plus a 16-bit residual in units of 2^-15 of the
  *                      value's ulp (two arrays, 6 bytes instead of 8): 68
@@ -78,6 +80,51 @@ __global__ __launch_bounds__(128) void k_sweep(SweepBufs B)
              B.cn0[px + g] + B.cn0[px + clampi(g + ox, hi)];
     }
     finish(B, e, cc, cn, s, sn);
+}
+
+/* streams+partners, two adjacent slots per lane; grid: chunks * 8 workgroups
+ * of 128 (a workgroup is one plane of one chunk) */
+__global__ __launch_bounds__(128) void k_sweep2(SweepBufs B)
+{
+    const int L = blockIdx.x;
+    const int b = L >> 3, ii = L & 7;
+    const long long P = B.P, hi = P - 2;
+    const long long g = (long long)b * 256 + 2 * threadIdx.x;
+    const long long e = ii * P + g;
+    const float2 cc = *(const float2 *)(B.ci0 + e);
+    const double2 cn = *(const double2 *)(B.cn0 + e);
+    const long long oz = (ii & 1) ? SZ : -SZ, oy = (ii & 2) ? SY : -SY, ox = (ii & 4) ? 1 : -1;
+    const long long pz = (ii ^ 1) * P, py = (ii ^ 2) * P, px = (ii ^ 4) * P;
+    const long long gz = clampi(g + oz, hi), gy = clampi(g + oy, hi);
+    const float2 a0 = *(const float2 *)(B.ci0 + pz + g), a1 = *(const float2 *)(B.ci0 + pz + gz);
+    const float2 a2 = *(const float2 *)(B.ci0 + py + g), a3 = *(const float2 *)(B.ci0 + py + gy);
+    const float2 a4 = *(const float2 *)(B.ci0 + px + g);
+    const double2 n0 = *(const double2 *)(B.cn0 + pz + g), n1 = *(const double2 *)(B.cn0 + pz + gz);
+    const double2 n2 = *(const double2 *)(B.cn0 + py + g), n3 = *(const double2 *)(B.cn0 + py + gy);
+    const double2 n4 = *(const double2 *)(B.cn0 + px + g);
+    /* x partners: g - 1 / g + 2 outside the pair, the pair's other slot inside */
+    const long long xo = clampi(ox < 0 ? g - 1 : g + 2, P - 1);
+    const float ax = B.ci0[px + xo];
+    const double nx = B.cn0[px + xo];
+    float s[2], sn_f[2];
+    double sn[2];
+    s[0] = a0.x + a1.x + a2.x + a3.x + a4.x + (ox < 0 ? ax : a4.y);
+    s[1] = a0.y + a1.y + a2.y + a3.y + a4.y + (ox < 0 ? a4.x : ax);
+    sn[0] = n0.x + n1.x + n2.x + n3.x + n4.x + (ox < 0 ? nx : n4.y);
+    sn[1] = n0.y + n1.y + n2.y + n3.y + n4.y + (ox < 0 ? n4.x : nx);
+    (void)sn_f;
+    const float2 v0 = *(const float2 *)(B.b0 + e), v1 = *(const float2 *)(B.b1 + e), v2 = *(const float2 *)(B.b2 + e);
+    const float2 v3 = *(const float2 *)(B.b3 + e), v4 = *(const float2 *)(B.b4 + e);
+    const double2 v5 = *(const double2 *)(B.b5 + e);
+    const float d0 = 1e-3f * (s[0] - 6.f * cc.x), d1 = 1e-3f * (s[1] - 6.f * cc.y);
+    *(float2 *)(B.ci1 + e) = make_float2(cc.x + d0, cc.y + d1);
+    *(double2 *)(B.cn1 + e) = make_double2(cn.x + 1e-3 * (sn[0] - 6.0 * cn.x), cn.y + 1e-3 * (sn[1] - 6.0 * cn.y));
+    *(float2 *)(B.b0 + e) = make_float2(v0.x + d0 * v1.x, v0.y + d1 * v1.y);
+    *(float2 *)(B.b1 + e) = make_float2(v1.x + d0 * v2.x, v1.y + d1 * v2.y);
+    *(float2 *)(B.b2 + e) = make_float2(v2.x + d0 * v3.x, v2.y + d1 * v3.y);
+    *(float2 *)(B.b3 + e) = make_float2(v3.x + d0 * v4.x, v3.y + d1 * v4.y);
+    *(float2 *)(B.b4 + e) = make_float2(v4.x + d0 * v0.x, v4.y + d1 * v0.y);
+    *(double2 *)(B.b5 + e) = make_double2(v5.x + (double)d0, v5.y + (double)d1);
 }
 
 /* An f64 pool as f32 + 16-bit residual: x = h + q * ulp(h) * 2^-15, with
@@ -227,6 +274,7 @@ int main(int argc, char **argv)
     int rc = timeit("streams", bytes, [&] { hipLaunchKernelGGL(k_sweep<false>, dim3(g128), dim3(128), 0, 0, B); });
     if (!rc) rc = timeit("streams+partners", bytes, [&] { hipLaunchKernelGGL(k_sweep<true>, dim3(g128), dim3(128), 0, 0, B); });
     if (!rc) rc = timeit("partners via LDS", bytes, [&] { hipLaunchKernelGGL(k_sweep_lds, dim3(g256), dim3(256), 0, 0, B); });
+    if (!rc) rc = timeit("two slots a lane", bytes, [&] { hipLaunchKernelGGL(k_sweep2, dim3(g256), dim3(128), 0, 0, B); });
     /* the split pools live in the first 6 bytes per element of the f64 allocations */
     SplitBufs S;
     S.ci0 = B.ci0; S.ci1 = B.ci1;
