@@ -2230,25 +2230,32 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_diag_cell(RyrKArgs a)
             if (lane == 0 && w) atomicAdd(&s_cnt[q], w);
         }
     }
-    /* chunk sums: pairwise trees over groups of 256 chunks, groups in order */
+    /* chunk sums: pairwise trees over groups of 256 chunks, groups in order.
+     * A group is four 64-chunk slices; wave w forms the slice trees of the
+     * quantities q = w mod 4, and a slice past the cell's last chunk is zero
+     * without a tree.  Which wave forms a tree does not change it. */
     RyrDD acc[RYR_DIAG_Q];
 #pragma unroll
     for (int q = 0; q < RYR_DIAG_Q; ++q) { acc[q].hi = 0.0; acc[q].lo = 0.0; }
     for (int g0 = 0; g0 < C.nblocks; g0 += RYR_BLOCK) {
-        const int bb = g0 + (int)threadIdx.x;
-        const RyrDD *pp = a.dpartial + (long long)(C.first_block + bb) * RYR_DIAG_Q;
-#pragma unroll
-        for (int q = 0; q < RYR_DIAG_Q; ++q) {
-            RyrDD t;
-            if (bb < C.nblocks) t = pp[q];
-            else { t.hi = 0.0; t.lo = 0.0; }
-            t = wave_tree(t);
-            if (lane == 0) s_red[q][wv] = t;
+        for (int sl = 0; sl < RYR_BLOCK / 64; ++sl) {
+            const int bb = g0 + sl * 64 + lane;
+            const bool empty = g0 + sl * 64 >= C.nblocks;
+            const RyrDD *pp = a.dpartial + (long long)(C.first_block + bb) * RYR_DIAG_Q;
+            for (int q = wv; q < RYR_DIAG_Q; q += RYR_BLOCK / 64) {
+                RyrDD t;
+                t.hi = 0.0; t.lo = 0.0;
+                if (!empty) {
+                    if (bb < C.nblocks) t = pp[q];
+                    t = wave_tree(t);
+                }
+                if (lane == 0) s_red[q][sl] = t;
+            }
         }
         __syncthreads();
-#pragma unroll
-        for (int q = 0; q < RYR_DIAG_Q; ++q)
-            acc[q] = dd_add(acc[q], dd_add(dd_add(s_red[q][0], s_red[q][1]), dd_add(s_red[q][2], s_red[q][3])));
+        if (threadIdx.x == 0)
+            for (int q = 0; q < RYR_DIAG_Q; ++q)
+                acc[q] = dd_add(acc[q], dd_add(dd_add(s_red[q][0], s_red[q][1]), dd_add(s_red[q][2], s_red[q][3])));
         __syncthreads();
     }
     if (threadIdx.x != 0) return;
