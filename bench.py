@@ -2,9 +2,11 @@
 """Benchmark: simulated cell-ms per wall-second of the CRU update on one MI355X.
 
 A "step" is one time step (dt = 0.025 ms) of the hot path over the whole
-batch: the CRU kernel over every CRU, the in-kernel whole-cell reduction and
+batch: the CRU kernels over every CRU, the on-device whole-cell reduction and
 membrane ODE, and (every 100 steps) the output-row reduction.  Inputs live in
 HBM before the timed region starts; nothing is copied to the host inside it.
+The timed steps are the ones production runs: replays of captured step
+graphs (ryr_prepare captures them before the timed region).
 
 Workloads (synthetic, model ICs; no datasets involved):
   cell  : one 100x20x10-CRU cell (20,000 CRUs), RyR multiplier Ku2 = 2.5,
@@ -12,10 +14,12 @@ Workloads (synthetic, model ICs; no datasets involved):
   batch : 256 independent cells, lattices drawn from {50x10x10, 75x15x10,
           100x20x10}, Ku2 ~ U[1,3], I_Kr scale ~ U[0,1] (lqt off, so the scale
           applies), CL in {1000,2000,3000}, seed = cell index, same precision
-The JSON line reports the `cell` workload as `value` and the batch beside it.
+The JSON line reports the `cell` workload as `value`, the batch beside it, and
+both workloads again in FP64 (the reference's arithmetic) under `f64`.
 """
 import argparse
 import json
+import math
 import os
 import platform
 import sys
@@ -29,42 +33,49 @@ import numpy as np  # noqa: E402
 METRIC = "simulated cell-ms/wall-s (1 cell, 256-cell batch); EAD-map GPU-time vs plain"
 HBM_PEAK_GBS = 8000.0
 
+# Algorithmic HBM bytes of one interior CRU-step, the unit of work of the
+# roofline: every live state word read once and written once, halo re-reads,
+# output-step extras and per-step scalars excluded.  FP32 production: 73 live
+# FP32 words read and written (8 ci, 8 cnsr, 6x8 cytosolic buffers, 4
+# sarcolemmal buffers, cs, cp, cjsr, cjsrb, Ancx) = 584 B, plus one 8-byte
+# packed gating word read and written = 16 B.  FP64: 73 x 8 x 2 + 16.
+UNIT_BYTES = {1: 600, 0: 1184}
 
-def alg_bytes_per_cru_step(precision):
-    """Algorithmic HBM bytes of one interior CRU-step of the whole step: every
-    live state word read once and written once (neighbour re-reads, ghost
-    cells and the output-step extras excluded).  FP32 mode: 55 float words
-    (8 ci, 5x8 fast buffers, cs, cp, Ancx, 4 sarcolemmal buffers) + 18 double
-    words (8 cnsr, 8 slow troponin, cjsr, CSQN-bound) + 3 packed 32-bit
-    gating/flag words.  FP64 mode: 73 double words + the 3 packed words."""
+
+def stored_bytes_per_cru_step(precision):
+    """HBM bytes of one interior CRU-step as this build stores the state:
+    FP32 mode keeps 55 words in float, 18 in double (8 cnsr, 8 slow troponin,
+    cjsr, CSQN-bound) and 3 packed 32-bit gating/flag words; FP64 mode 73
+    doubles and the 3 packed words.  Each read once and written once."""
     if precision == 1:
         return 2 * (55 * 4 + 18 * 8 + 3 * 4)
     return 2 * (73 * 8 + 3 * 4)
 
 
-def alg_bytes_per_kernel(precision, sched=0):
-    """Algorithmic HBM bytes per interior CRU of each step kernel (scheme A),
-    every word the kernel must touch counted once.
+def kernel_bytes(precision, sched=0):
+    """Bytes per interior CRU that each step kernel must touch (scheme A),
+    every word counted once per kernel.
     k_cyto (sub-voxel kernel): reads 8 ci, 8 cnsr, 5x8 fast buffers, 8 slow
     troponin, cjsr, cs, cp; writes 8 ci, 8 cnsr, 5x8 fast buffers, 8 slow
     troponin.  FP32: 336 + 320 = 656 B; FP64: 536 + 512 = 1048 B.
     k_cru_step (CRU kernel): reads ci/cnsr of sub-voxels 0-1, cs, cp, cjsr,
     cjsrb, Ancx, 4 sarcolemmal buffers, 3 packed words; writes Ancx, the 4
     buffers, cp, cs, cjsr, cjsrb, the packed words.  FP32: 80 + 56 = 136 B;
-    FP64: 116 + 84 = 200 B.  The two add up to the whole-step figure plus the
-    40 (FP64: 56) bytes both kernels read.
-    k_membrane: per cell, not per CRU (a few hundred bytes): 0 here.
+    FP64: 116 + 84 = 200 B.
     Schedule 3 (RyR gating in the sub-voxel launch): the sub-voxel launch also
-    reads and writes the RyR-count and flag words (cp and cjsr it reads
-    anyway), +16 B; the CRU kernel no longer writes the RyR-count word, -4 B."""
+    reads and writes the RyR-count and flag words, +16 B; the CRU kernel no
+    longer writes the RyR-count word, -4 B."""
     if precision == 1:
-        b = {"k_cyto": 656, "k_cru_step": 136, "k_membrane": 0}
+        b = {"k_cyto": 656, "k_cru_step": 136}
     else:
-        b = {"k_cyto": 1048, "k_cru_step": 200, "k_membrane": 0}
+        b = {"k_cyto": 1048, "k_cru_step": 200}
     if sched == 3:
         b["k_cyto"] += 16
         b["k_cru_step"] -= 4
     return b
+
+
+SCHED_NAMES = ["sequential", "membrane-fused", "concurrent", "membrane-and-gating-fused"]
 
 
 def make_cell(prec):
@@ -87,19 +98,37 @@ def make_batch(prec, ncells=256, seed=2024):
     return Params.default(precision=prec, lqt=0), cells    # lqt off: the I_Kr scale applies
 
 
-def run_gpu(p, cells, steps, warmup, prof_steps, shape=None):
-    """Timed region: `steps` graph-replayed steps, bracketed by a device
+def window_start(steps, spo):
+    """Phase (step mod spo) at which the timed window starts, so that it
+    holds round(steps/spo) output steps, the share production carries: a
+    window starting on an output step holds ceil(steps/spo), one starting
+    just after it floor(steps/spo)."""
+    return 0 if round(steps / spo) == math.ceil(steps / spo) else 1
+
+
+def run_gpu(p, cells, steps, warmup, prof_steps, shape=None, floor=False):
+    """Timed region: exactly `steps` steps, all of them graph replays (the
+    graphs are captured by ryr_prepare before it), bracketed by a device
     synchronisation on both sides (wall clock) and by HIP events on the
-    engine's stream.  Then `prof_steps` more steps with every kernel stamped
-    by its own start/end events (the per-kernel durations of the roofline)."""
+    engine's stream.  Then `prof_steps` more steps in the same schedule with
+    every launch stamped by its own start/end events (the per-kernel
+    durations of the roofline), and the launch floor of the step graph."""
     from ryr_hyperactivity_amd import CellEngine
+    from ryr_hyperactivity_amd._abi import S
     ncru = sum(c.ncru for c in cells)
+    spo = p.steps_per_output
     with CellEngine(p, cells, device=0) as eng:
         if shape:
             eng.set_launch_shape(*shape)
         eng.step(warmup)
+        extra = 0
         while eng.launch_shape()[2] < 0:       # the shape tuner runs inside the warm-up, never in the timed region
             eng.step(50)
+            extra += 50
+        done = int(eng.scalars()[0][S["STEP"]])
+        align = (window_start(steps, spo) - done) % spo
+        eng.step(align)
+        eng.prepare(steps)
         eng.sync()
         eng.kernel_time()                      # reset the event window
         t0 = time.perf_counter()
@@ -111,64 +140,74 @@ def run_gpu(p, cells, steps, warmup, prof_steps, shape=None):
         assert np.all(np.isfinite(v)), "non-finite membrane potential"
         shape = eng.launch_shape()
         prof = eng.profile_kernels(prof_steps)
+        floor_us = eng.latency_floor(spo) if floor else None
     sim_ms = len(cells) * steps * p.dt
     step_s = ev_ms * 1e-3 / max(launches, 1)
-    bpc = alg_bytes_per_kernel(p.precision, shape[2])
+    kb = kernel_bytes(p.precision, shape[2])
     kernels = {}
     for name, (us, cnt) in prof.items():
         d = {"avg_us": us, "launches": cnt}
-        if name in bpc and bpc[name] and us > 0:
-            d["bytes_per_launch"] = ncru * bpc[name]
-            d["achieved_GBs"] = ncru * bpc[name] / (us * 1e-6) / 1e9
+        if name in kb and us > 0:
+            d["kernel_bytes_per_launch"] = ncru * kb[name]
+            d["kernel_GBs"] = ncru * kb[name] / (us * 1e-6) / 1e9
         kernels[name] = d
-    # dominant kernel for the HBM roofline: of the kernels that take at least a quarter of the summed kernel
-    # time, the one that moves the most bytes.  k_membrane moves no per-CRU bytes (one workgroup per cell,
-    # latency-bound; in the fused schedules not a launch of its own) and k_diag runs once per output row.  On
-    # the single cell the sub-voxel and CRU kernels take about the same time, and a rule by time alone would
-    # flip between them from run to run.
-    hbm = {k: v for k, v in kernels.items() if v.get("bytes_per_launch")}
-    tot = sum(v["avg_us"] for v in hbm.values())
-    big = {k: v for k, v in hbm.items() if v["avg_us"] >= 0.25 * tot}
-    dom = max(big, key=lambda k: big[k]["bytes_per_launch"])
+    # The dominant kernel is the per-CRU kernel with the longest duration as production runs it (under the
+    # concurrent schedule the two overlap; under the fused ones the sub-voxel launch carries the membrane step
+    # and the gating).  The unit of work's algorithmic bytes (UNIT_BYTES per CRU-step, the whole step's state)
+    # over that kernel's duration is the roofline's achieved rate.
+    dom = max(("k_cyto", "k_cru_step"), key=lambda k: kernels[k]["avg_us"])
     return {
         "cell_ms_per_s": sim_ms / wall,
         "wall_s": wall,
         "ms_per_step": 1e3 * wall / steps,
         "event_ms_per_step": 1e3 * step_s,
+        "graph_replayed_steps": steps,
+        "window_start_phase": window_start(steps, spo),
+        "warmup_total": warmup + extra + align,
         "ncru": ncru,
-        "step_bytes": ncru * alg_bytes_per_cru_step(p.precision),
-        "step_GBs": ncru * alg_bytes_per_cru_step(p.precision) / step_s / 1e9,
-        "launch_shape": {"k_cyto": shape[0], "k_cru_step": shape[1], "schedule": ["sequential", "membrane-fused", "concurrent", "membrane-and-gating-fused"][shape[2]]},
+        "unit_bytes": UNIT_BYTES[p.precision],
+        "stored_bytes_per_cru_step": stored_bytes_per_cru_step(p.precision),
+        "step_GBs": ncru * UNIT_BYTES[p.precision] / step_s / 1e9,
+        "launch_shape": {"k_cyto": shape[0], "k_cru_step": shape[1], "schedule": SCHED_NAMES[shape[2]]},
         "kernels": kernels,
         "dominant": dom,
+        "latency_floor_us": floor_us,
     }
 
 
 def roofline(r, workload):
     k = r["dominant"]
-    kd = r["kernels"][k]
-    return {"bound": "hbm", "achieved": kd.get("achieved_GBs"), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": (kd.get("achieved_GBs") or 0.0) / HBM_PEAK_GBS,
+    us = r["kernels"][k]["avg_us"]
+    alg = r["ncru"] * r["unit_bytes"]
+    ach = alg / (us * 1e-6) / 1e9
+    return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
             "traffic": traffic_from_profiles(k, workload),
-            "kernel": k, "kernel_us": kd["avg_us"], "bytes_per_launch": kd.get("bytes_per_launch"),
+            "kernel": k, "kernel_us": us, "bytes_per_launch": alg, "unit_bytes": r["unit_bytes"],
             "step_GBs": r["step_GBs"], "step_frac": r["step_GBs"] / HBM_PEAK_GBS,
-            "kernels": {n: {"us": v["avg_us"], "GBs": v.get("achieved_GBs")} for n, v in r["kernels"].items()},
+            "stored_bytes_per_cru_step": r["stored_bytes_per_cru_step"],
+            "latency_floor_us_per_step": r["latency_floor_us"],
+            "kernels": {n: {"us": v["avg_us"], "launches": v["launches"], "kernel_GBs": v.get("kernel_GBs")}
+                        for n, v in r["kernels"].items()},
             "launch_shape": r["launch_shape"]}
 
 
 def cpu_baseline(p, cell, nsteps, nsteps_1core=None):
     """The FP64 oracle (OpenMP over CRUs) on this host, on a bounded sample:
-    all threads, then (a smaller sample) one thread."""
+    all threads over `nsteps` steps from rest through a stimulus at 1 ms (the
+    upstroke, the release and the start of the plateau), then one thread over
+    the next steps, on the plateau."""
     from oracle.binding import OracleCell, lib
     nthr = lib().orc_num_threads()
+    p.stim_offset = 1.0
     o = OracleCell(p, cell)
-    o.step(10)
     t0 = time.perf_counter()
     o.step(nsteps)
     wall = time.perf_counter() - t0
+    v_end = float(o.scalars()[1])
     out = {"value": nsteps * p.dt / wall, "unit": "simulated cell-ms/wall-s",
            "cores": nthr, "kind": "port",
-           "sample": f"{nsteps} steps ({nsteps * p.dt:g} ms) of one {cell.nx}x{cell.ny}x{cell.nz} cell, FP64 "
+           "sample": f"{nsteps} steps ({nsteps * p.dt:g} ms) of one {cell.nx}x{cell.ny}x{cell.nz} cell from rest, "
+                     f"stimulated at 1 ms (upstroke, release, plateau; V = {v_end:.1f} mV at the end), FP64 "
                      f"oracle, {nthr} OpenMP threads, {platform.processor() or 'x86_64'}",
            "wall_s": wall}
     n1 = nsteps_1core if nsteps_1core is not None else max(1, nsteps // 16)
@@ -179,8 +218,8 @@ def cpu_baseline(p, cell, nsteps, nsteps_1core=None):
         w1 = time.perf_counter() - t0
     finally:
         lib().orc_set_num_threads(nthr)
-    out["single_core"] = {"value": n1 * p.dt / w1, "cores": 1, "sample": f"the next {n1} steps, one thread",
-                          "wall_s": w1}
+    out["single_core"] = {"value": n1 * p.dt / w1, "cores": 1,
+                          "sample": f"the next {n1} steps (on the plateau), one thread", "wall_s": w1}
     return out
 
 
@@ -209,6 +248,12 @@ def traffic_from_profiles(kernel, workload):
     return t.get(f"{kernel}/{workload}", {}).get("hbm_bytes_per_launch")
 
 
+def sub_record(r, workload, steps, warmup):
+    return {"value": r["cell_ms_per_s"], "unit": "simulated cell-ms/wall-s", "ncru": r["ncru"], "steps": steps,
+            "warmup": warmup, "ms_per_step": r["ms_per_step"], "event_ms_per_step": r["event_ms_per_step"],
+            "roofline": roofline(r, workload)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -221,6 +266,7 @@ def main():
                     help="steps with per-kernel event stamps after the timed region (default 400 cell / 100 batch)")
     ap.add_argument("--cpu-steps", type=int, default=4000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-f64", action="store_true", help="skip the FP64 sub-records")
     ap.add_argument("--shape", default=None,
                     help="fix the launch shape WG_CYTO,WG_CRU,SCHED instead of tuning it (e.g. 256,64,2)")
     args = ap.parse_args()
@@ -236,19 +282,32 @@ def main():
     p, cells = make_cell(prec)
     r = None
     if args.workload in ("cell", "both"):
-        r = run_gpu(p, cells, steps, warmup, args.prof_steps or 400, shape)
+        r = run_gpu(p, cells, steps, warmup, args.prof_steps or 400, shape, floor=True)
         print(f"# cell: {json.dumps(r)}", file=sys.stderr, flush=True)
     batch = None
+    bw = args.warmup if args.warmup is not None else 100
     if args.workload in ("batch", "both"):
         pb, bcells = make_batch(prec)
-        bw = args.warmup if args.warmup is not None else 100
         rb = run_gpu(pb, bcells, args.batch_steps, bw, args.prof_steps or 100, shape)
         print(f"# batch: {json.dumps(rb)}", file=sys.stderr, flush=True)
-        batch = {"value": rb["cell_ms_per_s"], "unit": "simulated cell-ms/wall-s", "ncells": len(bcells),
-                 "ncru": rb["ncru"], "steps": args.batch_steps, "warmup": bw, "ms_per_step": rb["ms_per_step"],
-                 "roofline": roofline(rb, "batch")}
+        batch = dict(sub_record(rb, "batch", args.batch_steps, bw), ncells=len(bcells))
         if r is None:
             r, steps, warmup = rb, args.batch_steps, bw
+    f64 = None
+    if prec == 1 and not args.no_f64:
+        f64 = {}
+        if args.workload in ("cell", "both"):
+            p64, c64 = make_cell(0)
+            n64 = min(steps, 2000)
+            r64 = run_gpu(p64, c64, n64, warmup, 200, None)
+            print(f"# cell f64: {json.dumps(r64)}", file=sys.stderr, flush=True)
+            f64["cell"] = sub_record(r64, "cell_f64", n64, warmup)
+        if args.workload in ("batch", "both"):
+            pb64, bc64 = make_batch(0)
+            nb64 = min(args.batch_steps, 300)
+            rb64 = run_gpu(pb64, bc64, nb64, bw, 50, None)
+            print(f"# batch f64: {json.dumps(rb64)}", file=sys.stderr, flush=True)
+            f64["batch"] = dict(sub_record(rb64, "batch_f64", nb64, bw), ncells=len(bc64))
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(make_cell(0)[0], cells[0], args.cpu_steps)
@@ -269,10 +328,13 @@ def main():
         "data": "synthetic (model initial conditions, Philox seeds)",
         "config": {"workload": "1 cell 100x20x10 CRUs, Ku2=2.5, I_Kr=0, CL 2000 ms" if wl == "cell"
                    else "256-cell uneven batch", "dt_ms": p.dt, "precision": args.precision,
-                   "ncru": r["ncru"]},
+                   "ncru": r["ncru"], "graph_replayed_steps": r["graph_replayed_steps"],
+                   "window_start_phase": r["window_start_phase"], "warmup_total": r["warmup_total"],
+                   "event_ms_per_step": r["event_ms_per_step"]},
         "roofline": roofline(r, wl),
         "cpu_baseline": cpu,
         "batch": batch,
+        "f64": f64,
         "ead_map": ead_map_from_profiles(),
     }
     print(json.dumps(out), flush=True)

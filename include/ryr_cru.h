@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define RYR_ABI_VERSION 3
+#define RYR_ABI_VERSION 4
 
 #define RYR_OK            0
 #define RYR_E_ARG        -1   /* invalid argument */
@@ -202,6 +202,17 @@ int ryr_create(int device, const ryr_params *p, const ryr_cell_desc *cells, int 
  * kept; fetch them with ryr_read_diag before the next output step. */
 int ryr_step(ryr_handle *h, int64_t nsteps);
 
+/* ryr_step splits its steps into segments that end at output steps and
+ * replays each segment from a captured HIP graph: a whole output interval is
+ * captured at its first use, a shorter segment once the same length has been
+ * asked for before.  ryr_prepare captures now, without advancing anything,
+ * every graph that ryr_step(h, nsteps) called next would replay, so that
+ * call launches graphs only (a caller timing a few steps, or a loop of equal
+ * chunks, prepares once).  At most 8 graphs are kept; RYR_E_STATE while
+ * the launch shape is still being tuned.  No reference counterpart (the
+ * reference launches Compute once per step, cuda_Cell_Detailed.cu:420). */
+int ryr_prepare(ryr_handle *h, int64_t nsteps);
+
 /* Block until queued work on the handle's stream is done. */
 int ryr_sync(ryr_handle *h);
 
@@ -282,18 +293,33 @@ int ryr_kernel_time(ryr_handle *h, double *ms, int64_t *launches);
 int ryr_set_launch_shape(ryr_handle *h, int wg_cyto, int wg_cru, int sched);
 int ryr_get_launch_shape(const ryr_handle *h, int *wg_cyto, int *wg_cru, int *sched);
 
-/* Advances nsteps steps with every kernel launched on its own (no graph) and
- * stamped by HIP events at its start and end; avg_us[k] / count[k] receive the
- * mean duration (microseconds) and launch count of kernel k: 0 sub-voxel,
- * 1 CRU, 2 membrane, 3 output row.  The steps are ordinary simulation steps.
- * Used by the benchmark for per-kernel roofline figures. */
+/* Advances nsteps steps in the tuned schedule with every launch eager (no
+ * graph) and stamped by HIP events at its start and end; avg_us[k] / count[k]
+ * receive the mean duration (microseconds) and launch count of kernel k:
+ * 0 sub-voxel, 1 CRU, 2 membrane, 3 output row.  Each launch is the one
+ * production makes: under the concurrent schedule the CRU kernel runs beside
+ * the sub-voxel kernel, and under the fused schedules the sub-voxel launch
+ * carries the previous step's membrane step (and the RyR gating), so k = 2
+ * counts only the membrane launches of output steps and of the last step.
+ * The steps are ordinary simulation steps.  RYR_E_STATE while the shape is
+ * being tuned.  Used by the benchmark for per-kernel roofline figures. */
 #define RYR_NPROF 4
 int ryr_profile_kernels(ryr_handle *h, int64_t nsteps, double *avg_us, int64_t *count);
+
+/* Launch floor of the production step: the graph ryr_step would replay for
+ * min(nsteps, steps_per_output) steps under the current launch shape, with
+ * every kernel replaced by an empty kernel of the same grid and workgroup,
+ * replayed and timed by events; *us_per_step gets the best replay's time per
+ * step (microseconds).  No simulation step is taken.  RYR_E_STATE while the
+ * shape is being tuned.  Measurement only; no reference counterpart. */
+int ryr_latency_floor(ryr_handle *h, int64_t nsteps, double *us_per_step);
 
 /* Snapshot/restore of the whole device state (all cells, all fields,
  * membrane scalars, step counter).  The reference has none; with a
  * counter-based generator the random stream position is the step index, so a
- * restored handle continues bit-identically. */
+ * restored handle continues bit-identically.  ryr_load returns RYR_E_STATE,
+ * before it writes any state, unless the snapshot was saved by a handle of
+ * the same precision, cells, lattice shapes, parameters, variants and seeds. */
 int ryr_save(ryr_handle *h, const char *path);
 int ryr_load(ryr_handle *h, const char *path);
 

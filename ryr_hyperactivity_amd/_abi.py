@@ -14,6 +14,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libryr_cru.so")
 
+ABI_VERSION = 4
 RYR_OK = 0
 RYR_E_ARG = -1
 RYR_E_HIP = -2
@@ -123,6 +124,8 @@ def load_library(path: str | None = None):
         "ryr_cell_desc_default": (None, [P(CellDesc)]),
         "ryr_create": (c_int, [c_int, P(Params), P(CellDesc), c_int, P(vp)]),
         "ryr_step": (c_int, [vp, c_i64]),
+        "ryr_prepare": (c_int, [vp, c_i64]),
+        "ryr_latency_floor": (c_int, [vp, c_i64, P(c_double)]),
         "ryr_sync": (c_int, [vp]),
         "ryr_read_scalars": (c_int, [vp, P(c_double)]),
         "ryr_read_field": (c_int, [vp, c_int, c_int, P(c_double), c_i64]),
@@ -163,7 +166,7 @@ def load_library(path: str | None = None):
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    if lib.ryr_abi_version() != 3:
+    if lib.ryr_abi_version() != ABI_VERSION:
         raise LibraryMissing(f"{p}: ABI version mismatch")
     if path is None:
         _lib = lib
@@ -172,6 +175,7 @@ def load_library(path: str | None = None):
 
 # exported symbols every build must carry (checked by the CPU test suite)
 EXPORTED = ["ryr_abi_version", "ryr_params_default", "ryr_cell_desc_default", "ryr_create", "ryr_step",
+            "ryr_prepare", "ryr_latency_floor",
             "ryr_sync", "ryr_read_scalars", "ryr_read_field", "ryr_write_field", "ryr_write_scalar",
             "ryr_read_diag", "ryr_read_linescan", "ryr_read_counters", "ryr_cru_steps",
             "ryr_kernel_time", "ryr_save", "ryr_load", "ryr_destroy", "ryr_last_error",

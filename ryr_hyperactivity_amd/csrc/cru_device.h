@@ -2465,11 +2465,31 @@ __global__ void k_export(RyrKArgs a, int cell, int field, int planes, double *ou
     out[e] = absent ? 16.0 : field_get<TF, TS>(a, field, cur, (long long)p * a.P + g, g, p);
 }
 
+/* planes*N elements, element e = p*N + n (one thread each).  The LCC field
+ * packs the 8 slots of a CRU into the words lccw[g] and aux[g], so it is
+ * written by one thread per CRU that assembles both words from its 8 inputs
+ * (threads of different slots would otherwise read-modify-write the same
+ * words). */
 template <typename TF, typename TS>
 __global__ void k_import(RyrKArgs a, int cell, int field, int planes, const double *in)
 {
     const RyrCellInfo C = a.cells[cell];
     const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (field == 21) {
+        if (e >= C.N) return;
+        const long long g = C.off + e;
+        uint32_t w = a.lccw[g], mask = 0u;
+        for (int p = 0; p < 8; ++p) {
+            const double v = in[(long long)p * C.N + e];
+            if (v < 16.0) {
+                mask |= 1u << p;
+                w = (w & ~(15u << (4 * p))) | (((uint32_t)v & 15u) << (4 * p));
+            }
+        }
+        a.lccw[g] = w;
+        a.aux[g] = (a.aux[g] & ~(255u << RYR_AUX_MASK_SHIFT)) | (mask << RYR_AUX_MASK_SHIFT);
+        return;
+    }
     if (e >= (long long)planes * C.N) return;
     const int p = (int)(e / C.N);
     const long long n = e % C.N;

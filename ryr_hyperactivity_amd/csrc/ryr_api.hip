@@ -36,7 +36,11 @@ struct ryr_handle {
     long long gbase = 0;            /* value of *a.gstep as the enqueued work will see it; -1 unknown */
     double *stage = nullptr;
     size_t stage_n = 0;
-    hipGraphExec_t graph = nullptr;
+    /* captured step segments: `n` steps that start on an output step (out0)
+     * or hold none, replayed by ryr_step; `seen` counts the uses of a
+     * segment that has no graph yet */
+    struct Seg { bool out0; long long n; hipGraphExec_t ge; int seen; };
+    std::vector<Seg> segs;
     bool graph_failed = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipEvent_t kev[8] = {};         /* per-kernel start/end stamps: cyto, cru, membrane, diag */
@@ -53,6 +57,7 @@ struct ryr_handle {
     long long launches = 0;
     std::string err;
     std::vector<double *> vclamp;   /* per cell clamp waveform (device) */
+    bool null_launch = false;       /* enqueue empty kernels of the same shapes (ryr_latency_floor) */
 };
 
 static std::string g_create_err;
@@ -421,25 +426,49 @@ static hipError_t ensure_base(ryr_handle *h, long long nahead)
     return hipSuccess;
 }
 
+/* Empty kernel with the grid and workgroup of a step kernel: the launch
+ * floor of a step's graph (ryr_latency_floor) */
+__global__ void k_null(int) {}
+
+static hipError_t go_null(ryr_handle *h, unsigned grid, unsigned wg, hipStream_t st = nullptr)
+{
+    hipLaunchKernelGGL(k_null, dim3(grid), dim3(wg), 0, st ? st : h->stream, 0);
+    return hipGetLastError();
+}
+
 #define RYR_F32(h) ((h)->p.precision == RYR_PREC_F32)
 static hipError_t go_cyto(ryr_handle *h, int wg, int kc, int km, hipEvent_t *ev, bool gate = false)
 {
+    if (h->null_launch) {
+        const unsigned per = (unsigned)(RYR_BLOCK / wg) * (unsigned)h->nblocks;
+        return go_null(h, (km >= 0 ? (unsigned)h->ncells : 0u) + per * (gate ? 8u + RYR_GATE_LPC : 8u), wg);
+    }
     return RYR_F32(h) ? ryr_launch_cyto_f32(h->a, h->nblocks, wg, kc, km, gate, h->p.stim_amp, h->stream, ev)
                       : ryr_launch_cyto_f64(h->a, h->nblocks, wg, kc, km, gate, h->p.stim_amp, h->stream, ev);
 }
 static hipError_t go_cru(ryr_handle *h, int wg, int kc, hipEvent_t *ev, hipStream_t st = nullptr, bool split = false)
 {
     if (!st) st = h->stream;
+    if (h->null_launch) {
+        const unsigned nb = (unsigned)h->nblocks;
+        return wg == 64 ? go_null(h, 4 * nb, 64, st) : wg == 128 ? go_null(h, 2 * nb, 128, st)
+             : wg == 256 ? go_null(h, nb, 256, st) : go_null(h, 2 * nb, 256, st);
+    }
     return RYR_F32(h) ? ryr_launch_cru_f32(h->a, h->nblocks, wg, kc, split, st, ev)
                       : ryr_launch_cru_f64(h->a, h->nblocks, wg, kc, split, st, ev);
 }
 static hipError_t go_mem(ryr_handle *h, int km, hipEvent_t *ev)
 {
+    if (h->null_launch) return go_null(h, (unsigned)h->ncells, RYR_BLOCK);
     return RYR_F32(h) ? ryr_launch_mem_f32(h->a, h->ncells, km, h->p.stim_amp, h->stream, ev)
                       : ryr_launch_mem_f64(h->a, h->ncells, km, h->p.stim_amp, h->stream, ev);
 }
 static hipError_t go_diag(ryr_handle *h, hipEvent_t *ev)
 {
+    if (h->null_launch) {
+        hipError_t e = go_null(h, (unsigned)h->nblocks, RYR_BLOCK);
+        return e == hipSuccess ? go_null(h, (unsigned)h->ncells, RYR_BLOCK) : e;
+    }
     return RYR_F32(h) ? ryr_launch_diag_f32(h->a, h->nblocks, h->stream, ev)
                       : ryr_launch_diag_f64(h->a, h->nblocks, h->stream, ev);
 }
@@ -511,10 +540,59 @@ static hipError_t enqueue_timed_step(ryr_handle *h, int wg_cyto, int wg_cru, boo
     return e;
 }
 
+/* One step of the production schedule with every launch stamped by its own
+ * events, for ryr_profile_kernels: the concurrent schedule runs the CRU
+ * kernel beside the sub-voxel kernel on the second stream; the fused
+ * schedules carry the previous step's membrane step (*pending) in the
+ * sub-voxel launch and run it on its own only at an output step and at the
+ * last step (`last`), so each kernel's stamp is its duration as production
+ * runs it.  *ran[k] tells which of the four stamps were recorded; *tail is
+ * the event that ends the step on the main stream. */
+static hipError_t enqueue_profiled_step(ryr_handle *h, bool last, bool *pending, bool ran[4], hipEvent_t *tail)
+{
+    const int k = (int)(h->step - h->gbase);
+    const bool out = h->step % h->p.steps_per_output == 0;
+    const int sc = h->sched;
+    for (int q = 0; q < 4; ++q) ran[q] = false;
+    hipError_t e = hipSuccess;
+    if (sc == RYR_SCHED_CONCURRENT) {
+        e = hipEventRecord(h->fork, h->stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(h->stream2, h->fork, 0);
+        if (e == hipSuccess) e = go_cru(h, h->wg_cru, k, h->kev + 2, h->stream2);
+        if (e == hipSuccess) e = hipEventRecord(h->join, h->stream2);
+        if (e == hipSuccess) e = go_cyto(h, h->wg_cyto, k, -1, h->kev);
+        if (e == hipSuccess) e = hipStreamWaitEvent(h->stream, h->join, 0);
+        if (e == hipSuccess) e = go_mem(h, k, h->kev + 4);
+        ran[0] = ran[1] = ran[2] = true;
+        *tail = h->kev[5];
+    } else {
+        const bool gate = sc == RYR_SCHED_FUSED_GATING;
+        const bool fuse = sc == RYR_SCHED_FUSED || gate;
+        e = go_cyto(h, h->wg_cyto, k, *pending ? k - 1 : -1, h->kev, gate);
+        if (e == hipSuccess) e = go_cru(h, h->wg_cru, k, h->kev + 2, nullptr, gate);
+        ran[0] = ran[1] = true;
+        *tail = h->kev[3];
+        if (e == hipSuccess && (out || last || !fuse)) {
+            e = go_mem(h, k, h->kev + 4);
+            ran[2] = true;
+            *tail = h->kev[5];
+            *pending = false;
+        } else {
+            *pending = true;
+        }
+    }
+    if (e == hipSuccess && out) {
+        e = go_diag(h, h->kev + 6);
+        ran[3] = true;
+        *tail = h->kev[7];
+    }
+    return e;
+}
+
 static void drop_graph(ryr_handle *h)
 {
-    if (h->graph) hipGraphExecDestroy(h->graph);
-    h->graph = nullptr;
+    for (auto &q : h->segs) if (q.ge) hipGraphExecDestroy(q.ge);
+    h->segs.clear();
     h->graph_failed = false;
 }
 
@@ -651,32 +729,72 @@ static int tune_candidate(ryr_handle *h, const std::array<int, 3> &c, float *bes
     return RYR_OK;
 }
 
-/* one output interval (spo steps starting at an output step, offsets 0..spo-1
- * from the base) and the advance of the base, as a graph */
-static bool build_graph(ryr_handle *h)
+/* A step segment as a graph: n steps that start on an output step (out0) or
+ * hold none (so the segment never crosses an output step after its first),
+ * at offsets 0..n-1 from the base, then a node that advances the base by n.
+ * The kernels read the absolute step from the base, so one graph serves
+ * every segment of that kind wherever it falls. */
+#define RYR_MAX_SEGS 8
+static ryr_handle::Seg *find_seg(ryr_handle *h, bool out0, long long n)
 {
-    if (h->graph) return true;
+    for (auto &q : h->segs) if (q.out0 == out0 && q.n == n) return &q;
+    return nullptr;
+}
+
+static bool capture_seg(ryr_handle *h, ryr_handle::Seg &q)
+{
     if (h->graph_failed) return false;
-    hipGraph_t g = nullptr;
     const long long spo = h->p.steps_per_output;
-    bool ok = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    /* offsets are formed from h->step - h->gbase: capture as if the segment
+     * started at a base of its own (nothing runs during a capture) */
+    const long long sv_step = h->step, sv_base = h->gbase;
+    h->step = q.out0 ? 0 : 1;
+    h->gbase = h->step;
+    hipGraph_t g = nullptr;
+    bool ok = q.n >= 1 && (q.out0 ? q.n <= spo : q.n < spo) &&
+              hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
     if (ok) {
-        ok = enqueue_run(h, spo, h->sched < 0 ? RYR_SCHED_SEQ : h->sched) == hipSuccess;
+        ok = enqueue_run(h, q.n, h->sched < 0 ? RYR_SCHED_SEQ : h->sched) == hipSuccess;
         if (ok) {
-            hipLaunchKernelGGL(k_stepbase, dim3(1), dim3(64), 0, h->stream, h->a.gstep, spo, 1);
+            hipLaunchKernelGGL(k_stepbase, dim3(1), dim3(64), 0, h->stream, h->a.gstep, q.n, 1);
             ok = hipGetLastError() == hipSuccess;
         }
         hipError_t e = hipStreamEndCapture(h->stream, &g);
         ok = ok && e == hipSuccess && g;
     }
-    if (ok) ok = hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0) == hipSuccess;
+    h->step = sv_step;
+    h->gbase = sv_base;
+    if (ok) ok = hipGraphInstantiate(&q.ge, g, nullptr, nullptr, 0) == hipSuccess;
     if (g) hipGraphDestroy(g);
     if (!ok) {
-        h->graph = nullptr;
+        q.ge = nullptr;
         h->graph_failed = true;
         (void)hipGetLastError();
     }
     return ok;
+}
+
+/* The graph for the next n steps (n within one output interval), or null
+ * to run them eagerly.  A whole output interval is captured at its first
+ * use; any other segment once it has been asked for before (or prepared by
+ * ryr_prepare), so that one-off remainders are not captured. */
+static hipGraphExec_t seg_graph(ryr_handle *h, bool out0, long long n, bool capture_now)
+{
+    if (n < 2 || h->sched < 0) return nullptr;
+    ryr_handle::Seg *q = find_seg(h, out0, n);
+    if (!q) {
+        if (h->segs.size() >= 4 * RYR_MAX_SEGS) return nullptr;
+        h->segs.push_back({out0, n, nullptr, 0});
+        q = &h->segs.back();
+    }
+    if (q->ge) return q->ge;
+    q->seen += 1;
+    const bool whole = out0 && n == h->p.steps_per_output;
+    if (!(capture_now || whole || q->seen >= 2)) return nullptr;
+    int ncap = 0;
+    for (auto &x : h->segs) ncap += x.ge ? 1 : 0;
+    if (ncap >= RYR_MAX_SEGS) return nullptr;
+    return capture_seg(h, *q) ? q->ge : nullptr;
 }
 
 extern "C" int ryr_step(ryr_handle *h, int64_t nsteps)
@@ -721,8 +839,11 @@ extern "C" int ryr_step(ryr_handle *h, int64_t nsteps)
         bool ok = false;
         int rc = tune_candidate(h, h->cand[h->cand_i], &ms, &ok);
         if (rc) return rc;
-        if (ok) h->cand_ms[h->cand_i] = ms;
-        nsteps -= span;
+        /* a candidate that could not be captured advanced no step */
+        if (ok) {
+            h->cand_ms[h->cand_i] = ms;
+            nsteps -= span;
+        }
         if (++h->cand_i == h->cand.size()) {
             size_t best = 0;
             for (size_t q = 1; q < h->cand.size(); ++q)
@@ -735,20 +856,18 @@ extern "C" int ryr_step(ryr_handle *h, int64_t nsteps)
         }
     }
     while (nsteps > 0) {
-        if (h->step % spo == 0 && nsteps >= spo && spo >= 4) {
-            if (h->gbase != h->step) RYR_HIPCHECK(h, set_base(h, h->step));
-            if (build_graph(h)) {
-                RYR_HIPCHECK(h, hipGraphLaunch(h->graph, h->stream));
-                h->step += spo;
-                h->gbase += spo;
-                h->launches += spo;
-                nsteps -= spo;
-                continue;
-            }
-        }
-        /* eager run up to the next output step (or the end) */
+        /* segments end at the next output step (or the end of the call) */
         long long n = spo - h->step % spo;
         if (n > nsteps) n = nsteps;
+        if (hipGraphExec_t ge = seg_graph(h, h->step % spo == 0, n, false)) {
+            if (h->gbase != h->step) RYR_HIPCHECK(h, set_base(h, h->step));
+            RYR_HIPCHECK(h, hipGraphLaunch(ge, h->stream));
+            h->step += n;
+            h->gbase += n;
+            h->launches += n;
+            nsteps -= n;
+            continue;
+        }
         RYR_HIPCHECK(h, ensure_base(h, n));
         RYR_HIPCHECK(h, enqueue_run(h, n, h->sched < 0 ? RYR_SCHED_SEQ : h->sched));
         h->step += n;
@@ -756,6 +875,73 @@ extern "C" int ryr_step(ryr_handle *h, int64_t nsteps)
         nsteps -= n;
     }
     RYR_HIPCHECK(h, hipEventRecord(h->ev1, h->stream));
+    return RYR_OK;
+}
+
+extern "C" int ryr_prepare(ryr_handle *h, int64_t nsteps)
+{
+    if (!h || nsteps < 0) return RYR_E_ARG;
+    if (h->sched < 0) { h->err = "ryr_prepare: the launch shape is still being tuned"; return RYR_E_STATE; }
+    RYR_HIPCHECK(h, hipSetDevice(h->device));
+    const long long spo = h->p.steps_per_output;
+    long long s = h->step;
+    while (nsteps > 0) {
+        long long n = spo - s % spo;
+        if (n > nsteps) n = nsteps;
+        (void)seg_graph(h, s % spo == 0, n, true);
+        s += n;
+        nsteps -= n;
+    }
+    if (h->graph_failed) { h->err = "ryr_prepare: graph capture failed"; return RYR_E_HIP; }
+    return RYR_OK;
+}
+
+extern "C" int ryr_latency_floor(ryr_handle *h, int64_t nsteps, double *us_per_step)
+{
+    if (!h || nsteps < 2 || !us_per_step) return RYR_E_ARG;
+    if (h->sched < 0) { h->err = "ryr_latency_floor: the launch shape is still being tuned"; return RYR_E_STATE; }
+    RYR_HIPCHECK(h, hipSetDevice(h->device));
+    RYR_HIPCHECK(h, hipStreamSynchronize(h->stream));
+    const long long spo = h->p.steps_per_output;
+    const long long n = nsteps < spo ? nsteps : spo;
+    /* the same segment the production path replays (whole intervals start on
+     * an output step), with every kernel an empty one of the same shape */
+    const long long sv_step = h->step, sv_base = h->gbase;
+    h->step = n == spo ? 0 : 1;
+    h->gbase = h->step;
+    h->null_launch = true;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    bool ok = hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal) == hipSuccess;
+    if (ok) {
+        ok = enqueue_run(h, n, h->sched) == hipSuccess && go_null(h, 1, 64) == hipSuccess;   /* base advance */
+        ok = hipStreamEndCapture(h->stream, &g) == hipSuccess && g && ok;
+    }
+    h->null_launch = false;
+    h->step = sv_step;
+    h->gbase = sv_base;
+    if (ok) ok = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) == hipSuccess;
+    if (g) hipGraphDestroy(g);
+    if (!ok) {
+        (void)hipGetLastError();
+        h->err = "ryr_latency_floor: graph capture failed";
+        return RYR_E_HIP;
+    }
+    float best = 1e30f;
+    int rc = RYR_OK;
+    for (int r = 0; r < 6 && rc == RYR_OK; ++r) {
+        float ms = 0.0f;
+        if (hipEventRecord(h->kev[0], h->stream) != hipSuccess || hipGraphLaunch(ge, h->stream) != hipSuccess ||
+            hipEventRecord(h->kev[1], h->stream) != hipSuccess || hipEventSynchronize(h->kev[1]) != hipSuccess ||
+            hipEventElapsedTime(&ms, h->kev[0], h->kev[1]) != hipSuccess) {
+            h->err = "ryr_latency_floor: replay failed";
+            rc = RYR_E_HIP;
+        }
+        if (r > 0 && ms < best) best = ms;
+    }
+    hipGraphExecDestroy(ge);
+    if (rc) return rc;
+    *us_per_step = 1e3 * (double)best / (double)n;
     return RYR_OK;
 }
 
@@ -822,17 +1008,21 @@ extern "C" int ryr_profile_kernels(ryr_handle *h, int64_t nsteps, double *avg_us
     if (!h || nsteps < 0 || !avg_us || !count) return RYR_E_ARG;
     RYR_HIPCHECK(h, hipSetDevice(h->device));
     for (int k = 0; k < RYR_NPROF; ++k) { avg_us[k] = 0.0; count[k] = 0; }
-    if (h->wg_cyto == 0) {
-        int rc = ryr_step(h, 3 * RYR_TUNE_ROUNDS - h->tune_i);
-        if (rc) return rc;
+    if (h->sched < 0) {
+        h->err = "ryr_profile_kernels: the step schedule is still being tuned; step past the tuning first";
+        return RYR_E_STATE;
     }
     double tot[RYR_NPROF] = {0, 0, 0, 0};
+    bool pending = false;
     for (int64_t i = 0; i < nsteps; ++i) {
-        bool d = false;
+        bool ran[4];
+        hipEvent_t tail = nullptr;
         RYR_HIPCHECK(h, ensure_base(h, 1));
-        RYR_HIPCHECK(h, enqueue_timed_step(h, h->wg_cyto, h->wg_cru, &d));
-        RYR_HIPCHECK(h, hipEventSynchronize(d ? h->kev[7] : h->kev[5]));
-        for (int k = 0; k < (d ? 4 : 3); ++k) {
+        RYR_HIPCHECK(h, enqueue_profiled_step(h, i == nsteps - 1, &pending, ran, &tail));
+        RYR_HIPCHECK(h, hipEventSynchronize(tail));
+        if (ran[1]) RYR_HIPCHECK(h, hipEventSynchronize(h->kev[3]));
+        for (int k = 0; k < 4; ++k) {
+            if (!ran[k]) continue;
             float ms = 0.0f;
             RYR_HIPCHECK(h, hipEventElapsedTime(&ms, h->kev[2 * k], h->kev[2 * k + 1]));
             tot[k] += 1e3 * (double)ms;
@@ -1024,12 +1214,41 @@ extern "C" int ryr_cru_steps(ryr_handle *h, int64_t *out)
     return RYR_OK;
 }
 
-/* snapshot: header + every device allocation in allocation order */
+/* snapshot: header + every device allocation in allocation order.  `cfg`
+ * is a hash of every run parameter and every cell description the handle was
+ * created with (field by field, so struct padding plays no part): a
+ * snapshot loads only into a handle of the same configuration, since the
+ * state arrays alone do not record the lattice shape, the scheme, dt, the
+ * variants, the seeds or the channel parameters they were advanced with. */
 struct RyrSnapHeader {
     char magic[8];
     int version, precision, ncells, pad_;
     long long P, step, nallocs;
+    unsigned long long cfg;
 };
+
+static void fnv_bytes(unsigned long long &hsh, const void *p, size_t n)
+{
+    const unsigned char *b = (const unsigned char *)p;
+    for (size_t i = 0; i < n; ++i) { hsh ^= b[i]; hsh *= 0x100000001b3ull; }
+}
+
+static unsigned long long config_hash(const ryr_handle *h)
+{
+    unsigned long long s = 0xcbf29ce484222325ull;
+    auto d = [&](double x) { fnv_bytes(s, &x, sizeof x); };
+    auto i = [&](long long x) { fnv_bytes(s, &x, sizeof x); };
+    const ryr_params &p = h->p;
+    d(p.dt); i(p.scheme); i(p.precision); i(p.iso2); i(p.tapsi); i(p.lqt); i(p.nlcc);
+    i(p.steps_per_output); d(p.stim_amp); d(p.stim_dur); d(p.stim_offset); d(p.ci0); d(p.v0);
+    i(p.cm_mode); d(p.cm); i(p.variants);
+    i(h->ncells);
+    for (const ryr_cell_desc &c : h->desc) {
+        i(c.nx); i(c.ny); i(c.nz); i((long long)c.seed); d(c.cl); d(c.cjsr0); d(c.ncx_scale);
+        d(c.Ku); d(c.Kb); d(c.ica_scale); d(c.Ku2); d(c.ikr_scale);
+    }
+    return s;
+}
 
 extern "C" int ryr_save(ryr_handle *h, const char *path)
 {
@@ -1040,9 +1259,10 @@ extern "C" int ryr_save(ryr_handle *h, const char *path)
     if (!f) { h->err = std::string("cannot open ") + path; return RYR_E_ARG; }
     RyrSnapHeader hd;
     std::memset(&hd, 0, sizeof hd);
-    std::memcpy(hd.magic, "RYRSNAP", 8);
+    std::memcpy(hd.magic, "RYRSNP2", 8);
     hd.version = RYR_ABI_VERSION; hd.precision = h->p.precision; hd.ncells = h->ncells;
     hd.P = h->P; hd.step = h->step; hd.nallocs = (long long)h->allocs.size();
+    hd.cfg = config_hash(h);
     bool ok = std::fwrite(&hd, sizeof hd, 1, f) == 1;
     std::vector<char> buf;
     for (auto &al : h->allocs) {
@@ -1065,9 +1285,9 @@ extern "C" int ryr_load(ryr_handle *h, const char *path)
     FILE *f = std::fopen(path, "rb");
     if (!f) { h->err = std::string("cannot open ") + path; return RYR_E_ARG; }
     RyrSnapHeader hd;
-    bool ok = std::fread(&hd, sizeof hd, 1, f) == 1 && std::memcmp(hd.magic, "RYRSNAP", 8) == 0 &&
+    bool ok = std::fread(&hd, sizeof hd, 1, f) == 1 && std::memcmp(hd.magic, "RYRSNP2", 8) == 0 &&
               hd.version == RYR_ABI_VERSION && hd.precision == h->p.precision && hd.ncells == h->ncells &&
-              hd.P == h->P && hd.nallocs == (long long)h->allocs.size();
+              hd.P == h->P && hd.nallocs == (long long)h->allocs.size() && hd.cfg == config_hash(h);
     std::vector<char> buf;
     for (auto &al : h->allocs) {
         if (!ok) break;
@@ -1090,7 +1310,7 @@ extern "C" void ryr_destroy(ryr_handle *h)
     if (!h) return;
     hipSetDevice(h->device);
     if (h->stream) hipStreamSynchronize(h->stream);
-    if (h->graph) hipGraphExecDestroy(h->graph);
+    drop_graph(h);
     for (auto &al : h->allocs) hipFree(al.first);
     for (void *t : h->tables) hipFree(t);
     if (h->stage) hipFree(h->stage);

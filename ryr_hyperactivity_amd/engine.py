@@ -70,6 +70,17 @@ class CellEngine:
     def step(self, nsteps: int = 1):
         self._check(self.lib.ryr_step(self.h, int(nsteps)), "ryr_step")
 
+    def prepare(self, nsteps: int):
+        """Capture now the graphs that step(nsteps) called next would replay."""
+        self._check(self.lib.ryr_prepare(self.h, int(nsteps)), "ryr_prepare")
+
+    def latency_floor(self, nsteps: int) -> float:
+        """Microseconds per step of the production step graph with every
+        kernel an empty one of the same shape (the launch floor)."""
+        us = ctypes.c_double()
+        self._check(self.lib.ryr_latency_floor(self.h, int(nsteps), ctypes.byref(us)), "ryr_latency_floor")
+        return us.value
+
     def sync(self):
         self._check(self.lib.ryr_sync(self.h), "ryr_sync")
 

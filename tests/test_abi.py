@@ -47,7 +47,7 @@ def test_struct_layout_and_defaults_match_the_header():
     for f, _ in CellDesc._fields_:
         assert getattr(d, f) == getattr(e, f), f
     assert ctypes.sizeof(Params) == 104 and ctypes.sizeof(CellDesc) == 88
-    assert lib.ryr_abi_version() == 3
+    assert lib.ryr_abi_version() == _abi.ABI_VERSION == 4
 
 
 def _create(p, cells):

@@ -1,0 +1,126 @@
+"""GPU tests of the C ABI's host-side contracts: exact step counts through
+the shape tuning, graph capture of arbitrary step segments (ryr_prepare),
+field writes (LCC slots packed per CRU), snapshot refusal on a different
+configuration, and the launch floor measurement.  Values are checked against
+the FP64 oracle, which the FP64 GPU path equals bit for bit."""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from ryr_hyperactivity_amd import CellDesc, CellEngine, Params, FIELDS
+from ryr_hyperactivity_amd.engine import EngineError
+from ryr_hyperactivity_amd._abi import S
+from oracle.binding import OracleCell
+
+pytestmark = pytest.mark.gpu
+
+
+def _same(eng, cell, o, label):
+    for f in FIELDS:
+        g, r = eng.field(cell, f), o.field(f)
+        assert np.array_equal(g, r), f"{label}: field {f}"
+    assert np.array_equal(eng.scalars()[cell], o.scalars()), f"{label}: scalars"
+
+
+def test_step_counts_are_exact_through_the_tuning():
+    """ryr_step(n) advances exactly n steps whatever part of the size and
+    schedule tuning a call falls into, and the result equals the oracle's."""
+    p = Params.default(stim_offset=1.0)
+    d = CellDesc.default(nx=12, ny=7, nz=3, cl=1000.0, seed=21)
+    o = OracleCell(p, d)
+    done = 0
+    print("case tuning step counts 12x7x3", flush=True)
+    with CellEngine(p, d) as eng:
+        for n in (1, 4, 3, 2, 7, 11, 13, 25, 5, 99, 101, 3, 60, 17):
+            eng.step(n)
+            o.step(n)
+            done += n
+            assert eng.scalars()[0][S["STEP"]] == done, n
+            assert eng.cru_steps() == done * d.ncru
+        assert eng.launch_shape()[2] >= 0          # tuning finished inside these calls
+        _same(eng, 0, o, f"after {done} steps")
+
+
+@pytest.mark.parametrize("sched", [0, 1, 2, 3])
+def test_prepared_segments_replay_bit_exactly(sched):
+    """Segments of any length and phase, captured by ryr_prepare or at their
+    second use, replay the same values as eager steps (FP64 == oracle)."""
+    p = Params.default(stim_offset=1.0)
+    d = CellDesc.default(nx=14, ny=6, nz=4, cl=1000.0, seed=22)
+    o = OracleCell(p, d)
+    print(f"case prepared segments sched={sched}", flush=True)
+    with CellEngine(p, d) as eng:
+        eng.set_launch_shape(256, 256, sched)
+        for n, prep in ((37, True), (37, False), (20, True), (63, True), (20, False), (20, False), (143, True),
+                        (250, True), (1, True), (99, False)):
+            if prep:
+                eng.prepare(n)
+            eng.step(n)
+            o.step(n)
+        _same(eng, 0, o, "after prepared segments")
+        grow, gst = eng.diag()
+        orow, ost = o.diag()
+        assert gst == ost and np.array_equal(grow[0], orow)
+
+
+def test_write_field_lcc_and_counts_round_trip_and_step_like_the_oracle():
+    """ryr_write_field of the LCC slots (eight 4-bit states packed per CRU),
+    of the RyR counts, of cs and of ci reads back exactly, and the next steps
+    equal the oracle's after the same writes."""
+    p = Params.default(stim_offset=1.0)
+    d = CellDesc.default(nx=9, ny=7, nz=5, cl=1000.0, seed=23)
+    n = d.ncru
+    rng = np.random.default_rng(5)
+    lcc = rng.integers(0, 16, (8, n)).astype(float)
+    lcc[rng.random((8, n)) < 0.3] = 16.0                # absent slots, some below present ones
+    nou = rng.integers(0, 10, n)
+    nob = rng.integers(0, 10, n)
+    ncu = rng.integers(0, 50, n)
+    ncb = 100 - nou - nob - ncu
+    o = OracleCell(p, d)
+    print("case write_field round trip 9x7x5", flush=True)
+    with CellEngine(p, d) as eng:
+        eng.step(30)
+        o.step(30)
+        cs = eng.field(0, "CS") * rng.uniform(0.5, 2.0, n)
+        ci = eng.field(0, "CI") * rng.uniform(0.8, 1.2, (8, n))
+        writes = {"LCC": lcc, "NOU": nou, "NOB": nob, "NCU": ncu, "NCB": ncb, "CS": cs, "CI": ci}
+        for f, v in writes.items():
+            eng.set_field(0, f, v)
+            o.set_field(f, v)
+        for f, v in writes.items():
+            assert np.array_equal(eng.field(0, f).reshape(np.shape(v)), np.asarray(v, dtype=float)), f
+        _same(eng, 0, o, "after the writes")
+        eng.step(70)
+        o.step(70)
+        _same(eng, 0, o, "70 steps after the writes")
+
+
+def test_snapshot_loads_only_into_the_same_configuration():
+    p = Params.default()
+    d = CellDesc.default(nx=10, ny=6, nz=3, seed=4)
+    print("case snapshot configuration check", flush=True)
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "snap.bin")
+        with CellEngine(p, d) as eng:
+            eng.step(20)
+            eng.save(path)
+            eng.load(path)                                   # same handle: accepted
+            assert eng.scalars()[0][S["STEP"]] == 20
+        # same padded size (one 256-CRU chunk), transposed lattice: refused
+        with CellEngine(p, CellDesc.default(nx=6, ny=10, nz=3, seed=4)) as other:
+            with pytest.raises(EngineError):
+                other.load(path)
+            assert other.scalars()[0][S["STEP"]] == 0
+        for kw in ({"seed": 5}, {"Ku2": 2.0}):
+            with CellEngine(p, CellDesc.default(nx=10, ny=6, nz=3, **dict({"seed": 4}, **kw))) as other:
+                with pytest.raises(EngineError):
+                    other.load(path)
+        with CellEngine(Params.default(scheme=1), d) as other:
+            with pytest.raises(EngineError):
+                other.load(path)
+        with CellEngine(p, d) as same:                       # a new handle of the same configuration
+            same.load(path)
+            assert same.scalars()[0][S["STEP"]] == 20

@@ -96,8 +96,9 @@ def test_fp64_full_batch_cells_match_the_oracle():
     oracle's run of that cell alone after 2,030 steps through the stimulus
     and the upstroke."""
     import bench
-    _, cells = bench.make_batch(0)
-    p = Params.default(stim_offset=2.0)
+    pb, cells = bench.make_batch(0)
+    assert pb.lqt == 0                               # each cell's I_Kr scale applies, as in the benchmark
+    p = Params.default(stim_offset=2.0, lqt=0)
     pick, seen = [], set()
     for c, d in enumerate(cells):
         if (d.nx, d.ny, d.nz) not in seen:
@@ -113,6 +114,39 @@ def test_fp64_full_batch_cells_match_the_oracle():
             o.step(2030)
             _assert_same_state(eng, c, o, f"batch cell {c}")
             print(f"  cell {c} equal", flush=True)
+
+
+def test_config0_three_beats_from_baseline_ics_match_the_oracle():
+    """The CPU-runnable configuration: 32x8x4 = 1,024 CRUs, baseline ICs
+    (ci 0.1 uM, cjsr = cnsr 750 uM, V -86 mV, every channel closed), CL
+    1000 ms with a 2 ms stimulus, 3 beats = 120,000 steps, FP64, seed 1.
+    Every output row (1,200 of them), the final state, the counters and the
+    three beats' summaries equal the oracle's."""
+    p = Params.default(ci0=0.1, v0=-86.0, stim_dur=2.0)
+    d = CellDesc.default(nx=32, ny=8, nz=4, cl=1000.0, cjsr0=750.0, seed=1)
+    print("case config 0: 32x8x4, 120000 steps", flush=True)
+    o = OracleCell(p, d)
+    nrow = 0
+    with CellEngine(p, d) as eng:
+        for r in range(1200):
+            eng.step(100 if r else 1)
+            o.step(100 if r else 1)
+            grow, gst = eng.diag()
+            orow, ost = o.diag()
+            assert gst == ost == 100 * r, r
+            assert np.array_equal(grow[0], orow), (r, np.nonzero(grow[0] != orow))
+            nrow += 1
+            if r % 200 == 0:
+                print(f"  row {r} equal (t = {orow[0]:g} ms, V = {orow[2]:.2f} mV)", flush=True)
+        eng.step(99)
+        o.step(99)
+        _assert_same_state(eng, 0, o, "config 0 after 120000 steps")
+        assert np.array_equal(eng.counters()[0], o.counters())
+        gb = eng.beats(0)
+        assert np.array_equal(gb, o.beats())
+        # onsets at 100, 1100 and 2100 ms: two closed beats and the third in progress, each with an upstroke
+        assert gb.shape[0] == 3 and list(gb[:, 9]) == [1.0, 1.0, 0.0] and np.all(gb[:, 2] > 0.0)
+    assert nrow == 1200
 
 
 def test_fp64_bitexact_on_a_cell_of_more_than_256_chunks():
@@ -348,9 +382,15 @@ def test_tuning_and_kernel_profile():
         assert wc in (64, 128, 256) and wr in (64, 128, 256, 512) and sc in (0, 1, 2, 3)
         prof = eng.profile_kernels(120)
         assert eng.cru_steps() == 325 * d.ncru
-        for k in ("k_cyto", "k_cru_step", "k_membrane"):
+        assert eng.scalars()[0][S["STEP"]] == 325
+        for k in ("k_cyto", "k_cru_step"):
             us, n = prof[k]
             assert n == 120 and 0.5 < us < 5e4, (k, us, n)
+        # the fused schedules run the membrane step inside the next sub-voxel launch, and on its own only at
+        # output steps and at the last step
+        us, n = prof["k_membrane"]
+        assert (n == 120 if sc in (0, 2) else n == 2) and 0.5 < us < 5e4, (us, n, sc)
         assert prof["k_diag"][1] == 1              # steps 205..324 hold one output step (300)
+        assert eng.latency_floor(100) > 0.0
         eng.set_launch_shape(0, 0, 0)
         assert eng.launch_shape() == (0, 0, -1)

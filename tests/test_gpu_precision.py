@@ -1,0 +1,150 @@
+"""FP32 production mode against FP64 on the GPU, per output.
+
+The FP64 mode equals the FP64 oracle bit for bit (test_gpu_parity), so it is
+the reference the production precision is held to here.  Both modes draw the
+same Philox uniforms, so their channel counts agree until one uniform falls on
+the other side of a rate threshold; from that step on they are different
+realisations of the same stochastic model.  Each test therefore checks:
+
+  * before the first divergence of the channel counts, every output-row
+    column within its FP32 error: the relative RMS over the rows within
+    PRE_RMS_TOL (ten times the largest figure measured over 16 runs of the
+    100x20x10 cell, Ku2 1 and 2.5, seeds 1-8, 10 beats at CL 2000 ms), and
+    every row within PRE_ROW_TOL of the column's scale;
+  * after it, every whole-cell column within POST_ROW_TOL of its scale (a few
+    CRUs of 10^3-10^4 gate differently; measured at most 1.4e-3);
+  * the state fields at the end: relative RMS within FIELD_TOL_SAME (the
+    gating identical everywhere) or FIELD_TOL_DIVERGED (at most
+    MAX_DIVERGED_FRAC of the CRUs gate differently), each measured case far
+    inside these.
+"""
+import numpy as np
+import pytest
+
+from ryr_hyperactivity_amd import CellDesc, CellEngine, Params, PREC_F32, SCHEME_B
+from ryr_hyperactivity_amd._abi import DIAG_INT_COLS, DIAG_NAMES, VAR_ISO, VAR_PERM, VAR_RANDOMLCC, VAR_SLOWNCX
+
+pytestmark = pytest.mark.gpu
+
+# relative RMS (over the output rows before the first divergence) of FP32 against FP64, per column: ten times
+# the largest value over the 16 runs of the precision profile; columns not listed: 1e-5
+PRE_RMS_TOL = {"v": 5e-7, "ci": 1.5e-5, "cp": 8.5e-5, "cs": 2.4e-5, "cjsr": 6.3e-7, "cnsr": 8.4e-7,
+               "i_cal": 7.8e-7, "i_ncx": 1.9e-5, "po": 0.0, "nai": 3.1e-9, "catf": 2.1e-6, "cats": 6.1e-8,
+               "cacal": 7.8e-6, "csqn_bound": 3.3e-7, "ancx": 9.9e-6, "total_ca": 6.3e-7}
+PRE_RMS_DEFAULT = 1e-5
+PRE_ROW_TOL = 1e-4          # any single row, relative to the column's RMS over the window
+POST_ROW_TOL = 1e-2         # any row after the divergence, relative to the column's RMS
+FIELD_TOL_SAME = 2e-5       # relative RMS of a field when no CRU gates differently (measured <= 3.4e-6)
+FIELD_TOL_DIVERGED = 1e-2   # relative RMS of a field after some CRUs diverged (measured <= 1.6e-3)
+MAX_DIVERGED_FRAC = 5e-3    # CRUs whose gating state differs at the end (measured <= 3.5e-4)
+COUNT_COLS = [DIAG_NAMES.index(c) for c in ("nou", "nob", "ncu", "ncb", "tn1", "tn2", "tn3", "tn4", "tn5",
+                                            "tn6", "tn7")]
+FLOAT_COLS = [j for j in range(1, len(DIAG_NAMES)) if j not in DIAG_INT_COLS]
+FIELDS_F = ("CI", "CNSR", "CATF", "CATS", "CACAL", "CASR", "CAMYO", "CS", "CP", "CJSR", "CJSRB", "ANCX",
+            "CASAR", "CASARH", "CASARJ", "CASARHJ")
+GATING = ("NOU", "NOB", "NCU", "LCC")
+
+
+def _run(p, cells, nrows, pick):
+    rows = []
+    with CellEngine(p, cells) as eng:
+        for r in range(nrows):
+            eng.step(p.steps_per_output if r else 1)
+            rows.append(eng.diag()[0][pick].copy())
+        fields = {c: {f: eng.field(c, f) for f in FIELDS_F + GATING} for c in pick}
+        v = eng.scalars()[pick, 1]
+    return np.array(rows), fields, v         # rows: [row, cell, col]
+
+
+def _compare(r32, r64, f32, f64, label):
+    n = r64.shape[0]
+    div = np.any(r32[:, COUNT_COLS] != r64[:, COUNT_COLS], axis=1)
+    d0 = int(np.argmax(div)) if div.any() else n
+    scale = np.sqrt(np.mean(r64 ** 2, axis=0))
+    for j in FLOAT_COLS:
+        name = DIAG_NAMES[j]
+        a, b = r32[:, j], r64[:, j]
+        s = max(scale[j], 1e-300)
+        if d0 > 0:
+            rms = np.sqrt(np.mean((a[:d0] - b[:d0]) ** 2)) / s
+            tol = PRE_RMS_TOL.get(name, PRE_RMS_DEFAULT)
+            assert rms <= tol, f"{label}: {name} relative RMS {rms:.3g} > {tol:.3g} before the divergence (row {d0})"
+            worst = np.max(np.abs(a[:d0] - b[:d0])) / s
+            assert worst <= PRE_ROW_TOL, f"{label}: {name} row error {worst:.3g} before the divergence"
+        if d0 < n:
+            worst = np.max(np.abs(a[d0:] - b[d0:])) / s
+            assert worst <= POST_ROW_TOL, f"{label}: {name} row error {worst:.3g} after the divergence"
+    if d0 == n:
+        assert np.array_equal(r32[:, COUNT_COLS], r64[:, COUNT_COLS])
+    ncru = f64["NOU"].size
+    differ = (f32["NOU"] != f64["NOU"]) | (f32["NOB"] != f64["NOB"]) | (f32["NCU"] != f64["NCU"]) | \
+        np.any(f32["LCC"] != f64["LCC"], axis=0)
+    frac = differ.sum() / ncru
+    tol = FIELD_TOL_SAME if frac == 0 else FIELD_TOL_DIVERGED
+    assert frac <= MAX_DIVERGED_FRAC, f"{label}: {differ.sum()} of {ncru} CRUs gate differently"
+    for f in FIELDS_F:
+        x, y = f32[f], f64[f]
+        rel = np.sqrt(np.mean((x - y) ** 2)) / max(np.sqrt(np.mean(y * y)), 1e-300)
+        assert rel <= tol, f"{label}: field {f} relative RMS {rel:.3g} > {tol:.3g} ({differ.sum()} CRUs diverged)"
+    return d0
+
+
+def _fp32_vs_fp64(pkw, cells, nrows, pick=None, label=""):
+    pick = list(range(len(cells))) if pick is None else pick
+    r64, f64, v64 = _run(Params.default(**pkw), cells, nrows, pick)
+    r32, f32, v32 = _run(Params.default(precision=PREC_F32, **pkw), cells, nrows, pick)
+    divs = []
+    for q, c in enumerate(pick):
+        divs.append(_compare(r32[:, q], r64[:, q], f32[c], f64[c], f"{label} cell {c}"))
+    return divs, v64, v32
+
+
+@pytest.mark.parametrize("ku2", [1.0, 2.5])
+def test_fp32_tracks_fp64_on_the_benchmark_cell_through_the_upstroke(ku2):
+    """The benchmark's cell (100x20x10, I_Kr off, CL 2000 ms) stimulated at
+    2 ms: 4,000 steps (100 ms) through the upstroke, the release and the
+    plateau, seeds 1 and 2."""
+    cells = [CellDesc.default(nx=100, ny=20, nz=10, cl=2000.0, Ku2=ku2, ikr_scale=0.0, seed=s) for s in (1, 2)]
+    print(f"case FP32 vs FP64 100x20x10 Ku2 {ku2}", flush=True)
+    divs, v64, v32 = _fp32_vs_fp64(dict(stim_offset=2.0), cells, 41, label=f"Ku2 {ku2}")
+    assert np.all(v64 > 0.0) and np.allclose(v32, v64, rtol=1e-3)   # on the plateau at 100 ms
+    print(f"  first divergence rows {divs}", flush=True)
+
+
+def test_fp32_tracks_fp64_under_scheme_b():
+    cells = [CellDesc.default(nx=30, ny=10, nz=6, cl=1000.0, seed=s) for s in (1, 2)]
+    print("case FP32 vs FP64 scheme B 30x10x6", flush=True)
+    _fp32_vs_fp64(dict(stim_offset=2.0, scheme=SCHEME_B), cells, 41, label="scheme B")
+
+
+VARIANTS = [("perm", dict(variants=VAR_PERM, ci0=0.3), dict(cjsr0=900.0), 1000.0),
+            ("iso", dict(variants=VAR_ISO, iso2=0), {}, 10.0),
+            ("slowncx", dict(variants=VAR_SLOWNCX), {}, 1000.0),
+            ("randomlcc", dict(variants=VAR_RANDOMLCC), {}, 1000.0),
+            ("tapsi", dict(tapsi=1), {}, 1000.0)]
+
+
+@pytest.mark.parametrize("name,pk,dk,cl", VARIANTS, ids=[v[0] for v in VARIANTS])
+def test_fp32_tracks_fp64_for_each_variant(name, pk, dk, cl):
+    """Each variant in FP32 against FP64; iso at CL 10 ms so that its
+    stimulus window, the SERCA change and the LCC slots appearing at 5.5 CL
+    fall inside the 4,000 steps."""
+    cells = [CellDesc.default(nx=30, ny=10, nz=6, cl=cl, seed=s, **dk) for s in (1, 2)]
+    print(f"case FP32 vs FP64 variant {name}", flush=True)
+    _fp32_vs_fp64(dict(stim_offset=2.0, **pk), cells, 41, label=name)
+
+
+def test_fp32_batch_with_ikr_scaling_tracks_the_fp64_batch():
+    """The benchmark's 256-cell batch (3.07 M CRUs, lqt off so each cell's
+    I_Kr scale applies) in FP32 against the same batch in FP64, on the first
+    cell of each lattice size, through 2,000 steps with the stimulus at 2 ms."""
+    import bench
+    _, cells = bench.make_batch(0)
+    pick, seen = [], set()
+    for c, d in enumerate(cells):
+        if (d.nx, d.ny, d.nz) not in seen:
+            seen.add((d.nx, d.ny, d.nz))
+            pick.append(c)
+    assert len(pick) == 3 and len({cells[c].ikr_scale for c in pick}) == 3
+    print(f"case FP32 vs FP64 256-cell batch, cells {pick}", flush=True)
+    _fp32_vs_fp64(dict(stim_offset=2.0, lqt=0), cells, 21, pick=pick, label="batch")

@@ -110,6 +110,9 @@ VARIANT_CASES = [
     ("slowncx", dict(variants=VAR_SLOWNCX), {}),
     ("randomlcc", dict(variants=VAR_RANDOMLCC), {}),
     ("randomlcc+iso+slowncx/B", dict(variants=VAR_RANDOMLCC | VAR_ISO | VAR_SLOWNCX, scheme=1), {}),
+    ("tapsi", dict(tapsi=1), {}),                       # with lqt: cjsr0 10, SERCA 0.05
+    ("tapsi-nolqt", dict(tapsi=1, lqt=0), {}),          # SERCA 0
+    ("tapsi+iso", dict(tapsi=1, variants=VAR_ISO, iso2=0), {}),
 ]
 
 
