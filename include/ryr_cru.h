@@ -342,6 +342,22 @@ int ryr_gating_probe(int device, int64_t n, uint64_t seed, const uint32_t *cru, 
                      int32_t *counts, int32_t *lcc, const double *conc,
                      int32_t *draws, int32_t *nopen, int32_t *fails);
 
+/* ryr_gating_probe without allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as ryr_gating_probe; the device copies of the
+ * queries are staged in a fixed device array of the library (about 1.5 MiB,
+ * held by each GPU it runs on until the process ends) instead of buffers
+ * allocated per call, and the call makes no hipMalloc or hipFree.  At most
+ * RYR_PROBE_NOALLOC_MAX queries per call (more: RYR_E_ARG).  Every call on
+ * one GPU uses that GPU's one array: calls that overlap on the same GPU (from
+ * several host threads) race and must be serialised by the caller. */
+#define RYR_PROBE_NOALLOC_MAX 16384
+int ryr_gating_probe_noalloc(int device, int64_t n, uint64_t seed, const uint32_t *cru, uint64_t step,
+                             double Ku, double Ku2, double Kb, double dt, int iso2,
+                             int32_t *counts, int32_t *lcc, const double *conc,
+                             int32_t *draws, int32_t *nopen, int32_t *fails);
+
 #ifdef __cplusplus
 }
 #endif

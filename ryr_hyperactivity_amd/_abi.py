@@ -15,6 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libryr_cru.so")
 
 ABI_VERSION = 4
+PROBE_NOALLOC_MAX = 16384      # RYR_PROBE_NOALLOC_MAX
 RYR_OK = 0
 RYR_E_ARG = -1
 RYR_E_HIP = -2
@@ -161,6 +162,10 @@ def load_library(path: str | None = None):
                                      c_double, c_double, c_int, P(ctypes.c_int32), P(ctypes.c_int32),
                                      P(c_double), P(ctypes.c_int32), P(ctypes.c_int32),
                                      P(ctypes.c_int32)]),
+        "ryr_gating_probe_noalloc": (c_int, [c_int, c_i64, c_u64, P(ctypes.c_uint32), c_u64, c_double, c_double,
+                                             c_double, c_double, c_int, P(ctypes.c_int32), P(ctypes.c_int32),
+                                             P(c_double), P(ctypes.c_int32), P(ctypes.c_int32),
+                                             P(ctypes.c_int32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -179,7 +184,8 @@ EXPORTED = ["ryr_abi_version", "ryr_params_default", "ryr_cell_desc_default", "r
             "ryr_sync", "ryr_read_scalars", "ryr_read_field", "ryr_write_field", "ryr_write_scalar",
             "ryr_read_diag", "ryr_read_linescan", "ryr_read_counters", "ryr_cru_steps",
             "ryr_kernel_time", "ryr_save", "ryr_load", "ryr_destroy", "ryr_last_error",
-            "ryr_read_beats", "ryr_set_vclamp", "ryr_gating_probe", "ryr_set_launch_shape",
+            "ryr_read_beats", "ryr_set_vclamp", "ryr_gating_probe", "ryr_gating_probe_noalloc",
+            "ryr_set_launch_shape",
             "ryr_get_launch_shape", "ryr_profile_kernels"]
 # include/ryr_mlp.h
 EXPORTED_MLP = ["ryr_mlp_create", "ryr_mlp_destroy", "ryr_mlp_set_params", "ryr_mlp_get_params",

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 #include "ryr_cru.h"
@@ -1364,5 +1365,89 @@ extern "C" int ryr_gating_probe(int device, int64_t n, uint64_t seed, const uint
     hipFree(d_cru); hipFree(d_counts); hipFree(d_lcc); hipFree(d_conc);
     hipFree(d_draws); hipFree(d_nopen); hipFree(d_fails);
     if (!ok) { g_create_err = "ryr_gating_probe: HIP error"; return RYR_E_HIP; }
+    return RYR_OK;
+}
+
+/* ---- the gating probe without allocation
+ * ryr_gating_probe_noalloc stages the queries in `scratch`, a file-scope
+ * device array that every GPU the library runs on holds until the process
+ * ends, instead of device buffers allocated per call.  Host code never
+ * passes `scratch` itself to a kernel or a copy (in host code the name is a
+ * host-side stand-in, not a device address): the address of each device's
+ * copy is looked up once, with that device current, and kept per device. */
+/* RYR_PROBE_NOALLOC_MAX (queries per call) comes from the public header */
+#define RYR_PROBE_Q_BYTES (4 + 16 + 32 + 32 + 4 + 4 + 4)  /* cru, counts, lcc, conc, draws, nopen, fails */
+#define RYR_PROBE_SCRATCH_BYTES (RYR_PROBE_NOALLOC_MAX * RYR_PROBE_Q_BYTES + 7 * 256)
+static __device__ __align__(256) unsigned char scratch[RYR_PROBE_SCRATCH_BYTES];
+
+#define RYR_MAX_DEVICES 64
+static unsigned char *g_scratch_ptr[RYR_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+static int scratch_on(int device, unsigned char **out)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            g_create_err = std::string("ryr_gating_probe_noalloc: hipGetSymbolAddress(scratch): ") +
+                           hipGetErrorString(e);
+            return RYR_E_HIP;
+        }
+        g_scratch_ptr[device] = (unsigned char *)p;
+    }
+    *out = g_scratch_ptr[device];
+    return RYR_OK;
+}
+
+extern "C" int ryr_gating_probe_noalloc(int device, int64_t n, uint64_t seed, const uint32_t *cru, uint64_t step,
+                                        double Ku, double Ku2, double Kb, double dt, int iso2,
+                                        int32_t *counts, int32_t *lcc, const double *conc,
+                                        int32_t *draws, int32_t *nopen, int32_t *fails)
+{
+    if (n <= 0 || !cru || !counts || !lcc || !conc || !draws || !nopen || !fails || !(dt > 0)) {
+        g_create_err = "ryr_gating_probe_noalloc: bad arguments";
+        return RYR_E_ARG;
+    }
+    if (n > RYR_PROBE_NOALLOC_MAX) {
+        g_create_err = "ryr_gating_probe_noalloc: at most " + std::to_string(RYR_PROBE_NOALLOC_MAX) +
+                       " queries per call (" + std::to_string((long long)n) + " given); use ryr_gating_probe";
+        return RYR_E_ARG;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) { g_create_err = "no HIP device"; return RYR_E_NODEV; }
+    if (device < 0 || device >= ndev || device >= RYR_MAX_DEVICES) { g_create_err = "bad device"; return RYR_E_ARG; }
+    if (hipSetDevice(device) != hipSuccess) { g_create_err = "hipSetDevice failed"; return RYR_E_HIP; }
+    unsigned char *base = nullptr;
+    if (int rc = scratch_on(device, &base)) return rc;
+    /* the seven arrays of this call, each on a 256-byte boundary */
+    size_t off = 0;
+    auto take = [&](size_t bytes) { unsigned char *p = base + off; off += (bytes + 255) & ~(size_t)255; return p; };
+    uint32_t *d_cru = (uint32_t *)take(n * 4);
+    int32_t *d_counts = (int32_t *)take(n * 16);
+    int32_t *d_lcc = (int32_t *)take(n * 32);
+    double *d_conc = (double *)take(n * 32);
+    int32_t *d_draws = (int32_t *)take(n * 4);
+    int32_t *d_nopen = (int32_t *)take(n * 4);
+    int32_t *d_fails = (int32_t *)take(n * 4);
+    if (off > RYR_PROBE_SCRATCH_BYTES) { g_create_err = "ryr_gating_probe_noalloc: scratch overflow"; return RYR_E_ARG; }
+    /* every output is written by the kernel for each query, so nothing needs clearing */
+    bool ok = hipMemcpy(d_cru, cru, n * 4, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(d_counts, counts, n * 16, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(d_lcc, lcc, n * 32, hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(d_conc, conc, n * 32, hipMemcpyHostToDevice) == hipSuccess;
+    if (ok) {
+        hipLaunchKernelGGL(k_gating_probe, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, (long long)n,
+                           (unsigned long long)seed, d_cru, (unsigned long long)step, Ku, Ku2, Kb, dt, iso2,
+                           d_counts, d_lcc, d_conc, d_draws, d_nopen, d_fails);
+        ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(0) == hipSuccess;
+    }
+    ok = ok && hipMemcpy(counts, d_counts, n * 16, hipMemcpyDeviceToHost) == hipSuccess &&
+         hipMemcpy(lcc, d_lcc, n * 32, hipMemcpyDeviceToHost) == hipSuccess &&
+         hipMemcpy(draws, d_draws, n * 4, hipMemcpyDeviceToHost) == hipSuccess &&
+         hipMemcpy(nopen, d_nopen, n * 4, hipMemcpyDeviceToHost) == hipSuccess &&
+         hipMemcpy(fails, d_fails, n * 4, hipMemcpyDeviceToHost) == hipSuccess;
+    if (!ok) { g_create_err = "ryr_gating_probe_noalloc: HIP error"; return RYR_E_HIP; }
     return RYR_OK;
 }
