@@ -50,6 +50,8 @@ def lib():
             "orc_set_vclamp": (i, [vp, _D, i64, i]),
             "orc_read_beats": (i, [vp, _D, i, ctypes.POINTER(i)]),
             "orc_ncx": (d, [d, d, d, _D, d]),
+            "orc_ncx_slow": (d, [d, d, d, _D, d]),
+            "orc_read_scheme_b_lag": (i, [vp, _D, i64]),
             "orc_uptake": (d, [d, d]),
             "orc_leak": (d, [d, d]),
             "orc_luminal": (d, [d]),
@@ -122,6 +124,13 @@ class OracleCell:
         rc = self.L.orc_read_field(self.h, F[name], _dp(out), out.size)
         assert rc == 0
         return out.reshape(planes, n) if planes > 1 else out
+
+    def scheme_b_lag(self):
+        """Per CRU, the last scheme-B step's sum over its sub-steps of
+        (step-start cs - sub-step cs)."""
+        out = np.zeros(self.cell.nx * self.cell.ny * self.cell.nz)
+        assert self.L.orc_read_scheme_b_lag(self.h, _dp(out), out.size) == 0
+        return out
 
     def set_field(self, name, values):
         v = np.ascontiguousarray(np.asarray(values, dtype=np.float64).reshape(-1))

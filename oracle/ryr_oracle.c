@@ -178,11 +178,14 @@ double orc_canonical_sum(const double *x, int64_t n)
 
 /* ---- leaf physics: subroutine_cuda.c ------------------------------------- */
 
-/* subroutine_cuda.c:47-75 (cs in mM) */
-double orc_ncx(double cs, double v, double xnai, double *Ancx, double dt)
+/* subroutine_cuda.c:47-75 (cs in mM).  Under slowncxchange (:59-62) the
+ * exchanger runs at five times the allosteric factor; the factor itself
+ * relaxes as without it (the dead first assignment Ka = 0.025*12 is left out) */
+static double orc_ncx_v(double cs, double v, double xnai, double *Ancx, double dt, int slowncx)
 {
     double za = v * O_FARAD / O_XR / O_TEMPER;
     double Ka = *Ancx;
+    if (slowncx) Ka = *Ancx * 5;
     double t1 = O_KMCAI * P3(O_XNAO) * (1.0 + P3(xnai / O_KMNAI));
     double t2 = P3(O_KMNAO) * cs * (1.0 + cs / O_KMCAI);
     double t3 = (O_KMCAO + O_CEXT) * P3(xnai) + cs * P3(O_XNAO);
@@ -192,6 +195,16 @@ double orc_ncx(double cs, double v, double xnai, double *Ancx, double dt)
     double Ancxdot = (1.0 / (1.0 + P3(0.0003 / cs)) - *Ancx) / 150.;
     *Ancx += Ancxdot * dt;
     return Inaca;
+}
+
+double orc_ncx(double cs, double v, double xnai, double *Ancx, double dt)
+{
+    return orc_ncx_v(cs, v, xnai, Ancx, dt, 0);
+}
+
+double orc_ncx_slow(double cs, double v, double xnai, double *Ancx, double dt)
+{
+    return orc_ncx_v(cs, v, xnai, Ancx, dt, 1);
 }
 
 /* subroutine_cuda.c:128-137 */
@@ -565,6 +578,7 @@ struct orc_cell {
     double *cp, *cjsr, *cjsrb, *xire, *po, *Ancx, *xinaca, *xicat;
     double *casar, *casarh, *casarj, *casarhj;
     double *cs, *csnext;
+    double *lagB;           /* scheme B: sum over the sub-steps of (cs at step start - cs at sub-step start) */
     int *lcc, *nl, *nou, *ncu, *nob, *ncb, *nspark;
     /* per sub-voxel */
     double *ci, *cinext, *cnsr, *cnsrnext;
@@ -636,7 +650,7 @@ orc_cell *orc_create(const ryr_params *p, const ryr_cell_desc *d)
     c->xinaca = orc_alloc(n1, 8); c->xicat = orc_alloc(n1, 8);
     c->casar = orc_alloc(n1, 8); c->casarh = orc_alloc(n1, 8);
     c->casarj = orc_alloc(n1, 8); c->casarhj = orc_alloc(n1, 8);
-    c->cs = orc_alloc(n1, 8); c->csnext = orc_alloc(n1, 8);
+    c->cs = orc_alloc(n1, 8); c->csnext = orc_alloc(n1, 8); c->lagB = orc_alloc(n1, 8);
     c->lcc = orc_alloc(n8, sizeof(int)); c->nl = orc_alloc(n1, sizeof(int));
     c->nou = orc_alloc(n1, sizeof(int)); c->ncu = orc_alloc(n1, sizeof(int));
     c->nob = orc_alloc(n1, sizeof(int)); c->ncb = orc_alloc(n1, sizeof(int));
@@ -718,7 +732,7 @@ void orc_destroy(orc_cell *c)
 {
     if (!c) return;
     void *ptrs[] = {c->cp, c->cjsr, c->cjsrb, c->xire, c->po, c->Ancx, c->xinaca, c->xicat,
-                    c->casar, c->casarh, c->casarj, c->casarhj, c->cs, c->csnext, c->lcc, c->nl,
+                    c->casar, c->casarh, c->casarj, c->casarhj, c->cs, c->csnext, c->lagB, c->lcc, c->nl,
                     c->nou, c->ncu, c->nob, c->ncb, c->nspark, c->ci, c->cinext, c->cnsr,
                     c->cnsrnext, c->catf, c->cats, c->cacal, c->casr, c->camyo, c->mgmyo,
                     c->xiup, c->xileak, c->line, c->scratch, c->vclamp};
@@ -799,7 +813,8 @@ static void orc_compute_cru(orc_cell *c, int i, int j, int k, int reset, int64_t
     if (c->xire[ac] < 0) c->xire[ac] = 0;
 
     /* NCX (:1294-1304) */
-    c->xinaca[ac] = perm ? 0 : c->svncx * 1.0 * orc_ncx(cs[ac] / 1000.0, v, xnai, &c->Ancx[ac], dt);
+    c->xinaca[ac] = perm ? 0 : c->svncx * 1.0 * orc_ncx_v(cs[ac] / 1000.0, v, xnai, &c->Ancx[ac], dt,
+                                                         (var & RYR_VAR_SLOWNCX) != 0);
 
     /* SERCA and leak (:1305-1308) */
     for (int ii = 0; ii < 8; ++ii) {
@@ -895,7 +910,9 @@ static void orc_compute_cru(orc_cell *c, int i, int j, int k, int reset, int64_t
         dotcnsr[0] += diffjn0 * O_VJSR / O_VNSR;
         dotcnsr[1] += diffjn1 * O_VJSR / O_VNSR;
         csnext[ac] = cs[ac];
+        double lag = 0;       /* checker's record, not part of the step: see orc_read_scheme_b_lag */
         for (int iii = 0; iii < 5; ++iii) {
+            lag += cs[ac] - csnext[ac];
             double dpi0 = (c->cp[ac] - ci[ac * 8]) / O_TAUPI / 2.;
             double dpi1 = (c->cp[ac] - ci[ac * 8 + 1]) / O_TAUPI / 2.;
             double dsi0 = (csnext[ac] - ci[ac * 8]) / O_TAUSI / 2.;
@@ -932,6 +949,7 @@ static void orc_compute_cru(orc_cell *c, int i, int j, int k, int reset, int64_t
             if (c->casarhj[ac] < 0) c->casarhj[ac] = 0;
             c->cp[ac] += cpdot * dth;
         }
+        c->lagB[ac] = lag;
         for (int ii = 0; ii < 8; ++ii) {
             int64_t act = ac * 8 + ii;
             cinext[act] = ci[act] + dotci[ii] * dt;
@@ -1549,6 +1567,24 @@ int orc_read_field(const orc_cell *c, int f, double *out, int64_t n)
                     out[p * c->N + q] = d ? d[src] : (double)iv[src];
                 }
             }
+    return 0;
+}
+
+/* Scheme B's cp<->cs exchange uses the step-start cs on the cp side and the
+ * sub-step's cs on the cs side (:1503 vs :1515), and its cs z-diffusion the
+ * step-start neighbours against the sub-step's own cs (:1496), so it does not
+ * conserve calcium: over a step the total changes by the sarcolemmal flux
+ * plus (Vp/taups + 2 Vs/taust) dt/5 * sum over CRUs of the lag
+ * sum_m (cs_start - cs_m) (m = sub-steps 0..4, cs_0 = cs_start).  This
+ * returns each interior CRU's lag of the last scheme-B step, for the test of
+ * that identity. */
+int orc_read_scheme_b_lag(const orc_cell *c, double *out, int64_t n)
+{
+    if (n < c->N) return RYR_E_ARG;
+    for (int k = 1; k <= c->Z; ++k)
+        for (int j = 1; j <= c->Y; ++j)
+            for (int i = 1; i <= c->X; ++i)
+                out[INTERIOR(c, i, j, k)] = c->lagB[POS(c, i, j, k)];
     return 0;
 }
 

@@ -40,6 +40,12 @@ int orc_read_beats(const orc_cell *c, double *out, int maxbeats, int *nbeats);
 
 /* leaf functions (unit tests) */
 double orc_ncx(double cs_mM, double v, double xnai, double *Ancx, double dt);
+/* per interior CRU, the last scheme-B step's sum over sub-steps of
+ * (step-start cs - sub-step cs): the term by which scheme B departs from
+ * calcium conservation (test hook) */
+int orc_read_scheme_b_lag(const orc_cell *c, double *out, int64_t n);
+/* the same under slowncxchange (exchanger at 5 x the allosteric factor) */
+double orc_ncx_slow(double cs_mM, double v, double xnai, double *Ancx, double dt);
 double orc_uptake(double ci, double cnsr);
 double orc_leak(double cnsr, double ci);
 double orc_luminal(double cjsr);

@@ -1687,27 +1687,31 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L)
         } else if constexpr (sizeof(TF) != 8) {
             const RyrVF vf = vf_early ? vf0 : load_vf(M);
             const float csm = (float)cs_d * 1e-3f;
-            const float Ka = (float)(GSPLIT ? e_ancx : ancx[g]);
+            const float An = (float)(GSPLIT ? e_ancx : ancx[g]);
+            /* slowncxchange (subroutine_cuda.c:59-62): the exchanger at 5 x the allosteric factor */
+            const float Ka = (C.variants & RYR_VAR_SLOWNCX) ? An * 5.0f : An;
             const float t2 = 669921.875f * csm * (1.0f + csm * (float)(1.0 / 0.00359));
             const float t3 = (float)vf.ncx_t3a + csm * 2744000.0f;
             const float Inaca = Ka * 21.0f * ((float)vf.ncx_n1 - (float)vf.ncx_n2 * csm) *
                                 __builtin_amdgcn_rcpf(((float)vf.ncx_t1 + t2 + t3) * (float)vf.ncx_den2);
             const float r = 3e-4f * __builtin_amdgcn_rcpf(csm);
-            const float Ancxdot = (__builtin_amdgcn_rcpf(1.0f + r * r * r) - Ka) * (1.0f / 150.0f);
-            ancx_new = (TF)(Ka + Ancxdot * (float)dt);
+            const float Ancxdot = (__builtin_amdgcn_rcpf(1.0f + r * r * r) - An) * (1.0f / 150.0f);
+            ancx_new = (TF)(An + Ancxdot * (float)dt);
             xinaca = (double)((float)C.svncx * Inaca);
         } else {
             RYR_STRICT_FP
             const RyrVF vf = vf_early ? vf0 : load_vf(M);
             double cs = cs_d / 1000.0;
-            double Ka = (double)(GSPLIT ? e_ancx : ancx[g]);
+            const double An = (double)(GSPLIT ? e_ancx : ancx[g]);
+            double Ka = An;
+            if (C.variants & RYR_VAR_SLOWNCX) Ka = An * 5;  /* subroutine_cuda.c:59-62 */
             double t2 = (87.5) * (87.5) * (87.5) * cs * (1.0 + cs / 0.00359);
             double t3 = vf.ncx_t3a + cs * ((140.0) * (140.0) * (140.0));
             double Inaca = Ka * (21.0 / 1.0) * (vf.ncx_n1 - vf.ncx_n2 * cs)
                            / ((vf.ncx_t1 + t2 + t3) * vf.ncx_den2);
             double r = 0.0003 / cs;
-            double Ancxdot = (1.0 / (1.0 + r * r * r) - Ka) / 150.;
-            ancx_new = (TF)(Ka + Ancxdot * dt);
+            double Ancxdot = (1.0 / (1.0 + r * r * r) - An) / 150.;
+            ancx_new = (TF)(An + Ancxdot * dt);
             xinaca = C.svncx * 1.0 * Inaca;
         }
         if (lead) ancx[g] = ancx_new;

@@ -46,9 +46,12 @@ REGIONS = {
     "membrane_ics": [("cuda_Cell_Detailed.cu", 353, 365)],
 }
 # `#define` blocks, scanned in this order (the buffer file's volumes use the
-# main file's Vi and Vs) under the build the reference's own switches select
-# (cuda_Cell_Detailed.cu:29-40 turn on lumgate, lqt, iso2, outputlinescan)
+# main file's Vi and Vs), under the switches of the published build: run.sh:5
+# builds `make il`, whose recipe (Makefile:20-22) passes -Diso2 -Dlqt; the
+# file's own active switches (lumgate, outputlinescan) are added as they are
+# met.  The values are those of that build (e.g. svikr 0 under lqt).
 DEFINE_FILES = [("cuda_Cell_Detailed.cu", 29, 146), ("subroutine_cuda.c", 1, 45), ("buffers.c", 1, 53)]
+BUILD_SWITCHES = {"iso2", "lqt"}
 
 NUM = re.compile(r"(?<![\w.])(?:\d+\.\d*|\.\d+|\d+)(?:[eE][-+]?\d+)?(?![\w.])")
 
@@ -104,7 +107,7 @@ def main():
         for f, a, b in parts:
             lits.extend(literals(src[f][a - 1:b]))
         out["regions"][name] = {"source": [f"{f}:{a}-{b}" for f, a, b in parts], "literals": lits}
-    known, flags = {}, set()
+    known, flags = {}, set(BUILD_SWITCHES)
     for f, a, b in DEFINE_FILES:
         active = [True]
         for k, ln in enumerate(src[f][a - 1:b], start=a):

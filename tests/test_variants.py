@@ -9,6 +9,7 @@ import pytest
 from ryr_hyperactivity_amd import CellDesc, Params, FIELDS
 from ryr_hyperactivity_amd._abi import S, VAR_PERM, VAR_ISO, VAR_SLOWNCX, VAR_RANDOMLCC, DIAG_NAMES
 from oracle.binding import OracleCell
+from oracle import binding as orc
 
 FRT = 96.485 / 8.314 / 308.0
 
@@ -71,20 +72,35 @@ def test_iso_paces_from_phase_zero_for_twelve_beats_and_adds_channels():
 def test_slowncx_background_current_vanishes_at_the_calcium_reversal():
     """slowncxchange (:1266-1268): 0.003016/32 (V - ln(180000/cs)/(2 FRT)) per
     CRU.  With V at the Ca reversal of the uniform cs the first step's cs
-    equals the base model's; below it the current is inward and cs rises."""
+    equals the base model's; below it the current is inward and cs rises.
+    The exchanger is off (ncx scale 0) so that only the background current
+    differs between the two builds."""
     cs0 = 0.0944
     eca = math.log(180000 / cs0) / 2.0 / FRT
     res = {}
     for var in (0, VAR_SLOWNCX):
         for v in (eca, -80.0):
             p = Params.default(variants=var)
-            d = CellDesc.default(nx=3, ny=3, nz=3, seed=5)
+            d = CellDesc.default(nx=3, ny=3, nz=3, seed=5, ncx_scale=0.0)
             o = OracleCell(p, d)
             o.set_scalar("V", v)
             o.step(1)
             res[(var, v)] = o.field("CS").copy()
     assert np.allclose(res[(VAR_SLOWNCX, eca)], res[(0, eca)], rtol=1e-12, atol=0)
     assert np.all(res[(VAR_SLOWNCX, -80.0)] > res[(0, -80.0)])
+
+
+def test_slowncx_runs_the_exchanger_at_five_times_the_allosteric_factor():
+    """slowncxchange (subroutine_cuda.c:59-62): Ka = 5 Ancx in the exchanger
+    current, while Ancx itself relaxes as in the base model."""
+    import ctypes
+    L = orc.lib()
+    for cs, v, nai in ((1e-4, -80.0, 8.0), (5e-4, 10.0, 10.0), (2e-3, 40.0, 12.0)):
+        a0, a5 = ctypes.c_double(0.2), ctypes.c_double(0.2)
+        i0 = L.orc_ncx(cs, v, nai, ctypes.byref(a0), 0.025)
+        i5 = L.orc_ncx_slow(cs, v, nai, ctypes.byref(a5), 0.025)
+        assert i5 == pytest.approx(5.0 * i0, rel=1e-14)
+        assert a5.value == a0.value != 0.2
 
 
 def test_randomlcc_draws_zero_to_eight_channels_uniformly():
