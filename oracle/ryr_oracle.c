@@ -34,12 +34,15 @@
  *  - Cm follows ryr_params.cm_mode (density-preserving by default).
  *
  * Pinning: the tests check each part against closed forms, invariants,
- * brute force or published known answers (Philox).  Parity unpinned:
- *  - trajectories against the reference's own runs (cuRAND and CUDA libm are
- *    absent and the reference ships no outputs), so agreement with it is
- *    statistical;
- *  - the scheme B sub-step orchestration (:1475-1597), which conserves no
- *    quantity; only its uniform fixed point is checked.
+ * brute force or published known answers (Philox), and the constants and
+ * leaf formulas against the reference's own literals (a fixture taken from
+ * its sources): every define, the literal sequence of each leaf function,
+ * the LCC gate rates, NCX, SERCA, leak and the membrane currents and gates.
+ * Scheme B (:1475-1597) conserves no quantity; it is pinned by its exact
+ * calcium balance, the sarcolemmal flux plus the lag its mixed operands
+ * leave (orc_read_scheme_b_lag).  Parity unpinned: trajectories against the
+ * reference's own runs (cuRAND and CUDA libm are absent and the reference
+ * ships no outputs), so agreement with it is statistical.
  */
 #include <stdlib.h>
 #include <string.h>
