@@ -161,12 +161,19 @@ enum ryr_field {
  *   peak_ci   largest whole-cell mean ci over the beat's output rows (uM)
  *   jsr_load  whole-cell mean cjsr at the output row of the onset (uM)
  *   sparks    sum over the beat's output rows of the CRUs with a spark onset
- *   complete  1 for a finished beat, 0 for the beat in progress */
-#define RYR_BEAT_COLS 10
+ *   complete  1 for a finished beat, 0 for the beat in progress
+ *   ead_lit   1 if dV/dt > 0 above -40 mV during repolarisation, read
+ *             literally: once V has started to fall after the upstroke
+ *             (the first step after the beat's V exceeded -40 mV at which V
+ *             decreases), any later step of the beat at which V rises and
+ *             ends above -40 mV.  A notch-and-dome or any rise on the plateau
+ *             counts; ead above is the flag with a time and size floor
+ *   ead_lit_t time of that step's end from onset (ms; -1 if none) */
+#define RYR_BEAT_COLS 12
 #define RYR_MAX_BEATS 64
 enum ryr_beat_col {
     RYR_B_ONSET = 0, RYR_B_APD90, RYR_B_VMAX, RYR_B_VREST, RYR_B_EAD, RYR_B_EAD_T,
-    RYR_B_PEAK_CI, RYR_B_JSR_LOAD, RYR_B_SPARKS, RYR_B_COMPLETE
+    RYR_B_PEAK_CI, RYR_B_JSR_LOAD, RYR_B_SPARKS, RYR_B_COMPLETE, RYR_B_EAD_LIT, RYR_B_EAD_LIT_T
 };
 
 /* Soft-error counters per cell (ryr_read_counters). */

@@ -15,6 +15,7 @@ import subprocess
 import numpy as np
 
 from ryr_hyperactivity_amd._abi import CellDesc, Params, S_COUNT, DIAG_COLS, LINE_COLS, C_COUNT, F, PLANES, S
+from ryr_hyperactivity_amd._abi import BEAT_COLS
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboracle_ryr.so")
@@ -151,7 +152,7 @@ class OracleCell:
         return out
 
     def beats(self):
-        out = np.zeros((65, 10))
+        out = np.zeros((65, BEAT_COLS))
         n = ctypes.c_int()
         self.L.orc_read_beats(self.h, _dp(out), 65, ctypes.byref(n))
         return out[:n.value]

@@ -604,7 +604,7 @@ struct orc_cell {
     /* beat analyser (definitions in include/ryr_cru.h, enum ryr_beat_col) */
     int64_t beat_on;
     int beat_n, beat_flags;
-    double b_vrest, b_vmax, b_vmin, b_apd90, b_ead_t, b_peak_ci, b_jsr, b_sparks;
+    double b_vrest, b_vmax, b_vmin, b_apd90, b_ead_t, b_peak_ci, b_jsr, b_sparks, b_eadl_t;
     double beats[RYR_MAX_BEATS][RYR_BEAT_COLS];
 };
 
@@ -1202,6 +1202,7 @@ static void orc_beat_close(orc_cell *c, double complete)
         r[0] = (double)c->beat_on; r[1] = c->b_apd90; r[2] = c->b_vmax; r[3] = c->b_vrest;
         r[4] = (c->beat_flags & 2) ? 1.0 : 0.0; r[5] = c->b_ead_t; r[6] = c->b_peak_ci;
         r[7] = c->b_jsr; r[8] = c->b_sparks; r[9] = complete;
+        r[10] = (c->beat_flags & 8) ? 1.0 : 0.0; r[11] = c->b_eadl_t;
     }
     c->beat_n++;
 }
@@ -1215,9 +1216,16 @@ static void orc_beat_track(orc_cell *c, int64_t n, double vo, double vn)
         c->beat_on = n;
         c->b_vrest = vo; c->b_vmax = vo; c->b_vmin = vo;
         c->b_apd90 = -1.0; c->b_ead_t = -1.0; c->beat_flags = 0;
-        c->b_peak_ci = 0.0; c->b_jsr = -1.0; c->b_sparks = 0.0;
+        c->b_peak_ci = 0.0; c->b_jsr = -1.0; c->b_sparks = 0.0; c->b_eadl_t = -1.0;
     }
     if (c->beat_on < 0) return;
+    /* the literal flag: after the upstroke V starts to fall (bit2); then any rise ending above -40 mV */
+    if (!(c->beat_flags & 4)) {
+        if (c->b_vmax > -40.0 && vn < vo) c->beat_flags |= 4;
+    } else if (!(c->beat_flags & 8) && vn > vo && vn > -40.0) {
+        c->beat_flags |= 8;
+        c->b_eadl_t = (double)(n + 1) * dt - (double)c->beat_on * dt;
+    }
     if (vn > c->b_vmax) c->b_vmax = vn;
     double amp = c->b_vmax - c->b_vrest;
     if (!(c->beat_flags & 1)) {
@@ -1263,6 +1271,7 @@ int orc_read_beats(const orc_cell *c, double *out, int maxbeats, int *nbeats)
         r[0] = (double)c->beat_on * c->p.dt; r[1] = c->b_apd90; r[2] = c->b_vmax; r[3] = c->b_vrest;
         r[4] = (c->beat_flags & 2) ? 1.0 : 0.0; r[5] = c->b_ead_t; r[6] = c->b_peak_ci; r[7] = c->b_jsr;
         r[8] = c->b_sparks; r[9] = 0.0;
+        r[10] = (c->beat_flags & 8) ? 1.0 : 0.0; r[11] = c->b_eadl_t;
         ++k;
     }
     *nbeats = k;

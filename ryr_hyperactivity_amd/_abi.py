@@ -42,9 +42,10 @@ F = {name: i for i, name in enumerate(FIELDS)}
 PLANES = {name: (8 if name in ("CI", "CNSR", "CATF", "CATS", "CACAL", "CASR", "CAMYO", "MGMYO", "LCC")
                  else 1) for name in FIELDS}
 
-BEAT_COLS = 10
+BEAT_COLS = 12
 MAX_BEATS = 64
-BEAT_NAMES = ["onset", "apd90", "vmax", "vrest", "ead", "ead_t", "peak_ci", "jsr_load", "sparks", "complete"]
+BEAT_NAMES = ["onset", "apd90", "vmax", "vrest", "ead", "ead_t", "peak_ci", "jsr_load", "sparks", "complete",
+              "ead_lit", "ead_lit_t"]
 
 COUNTERS = ["GAUSS_FALLBACK", "CLAMP_CP", "CLAMP_CS", "CI_RUNAWAY", "NONFINITE_V"]
 C_COUNT = len(COUNTERS)

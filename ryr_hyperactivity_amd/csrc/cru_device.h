@@ -110,8 +110,9 @@ struct RyrMem {
     double ncx_n1, ncx_n2, ncx_t1, ncx_t3a, ncx_den2;
     /* beat tracker (definitions in the public header, enum ryr_beat_col) */
     long long beat_on;      /* onset step of the beat in progress, -1 before the first */
-    int beat_n, beat_flags; /* beats completed; bit0 repolarising, bit1 EAD seen */
-    double b_vrest, b_vmax, b_vmin, b_apd90, b_ead_t, b_peak_ci, b_jsr, b_sparks;
+    int beat_n, beat_flags; /* beats completed; bit0 repolarising, bit1 EAD seen,
+                               bit2 V fell after the upstroke, bit3 literal EAD seen */
+    double b_vrest, b_vmax, b_vmin, b_apd90, b_ead_t, b_peak_ci, b_jsr, b_sparks, b_eadl_t;
 };
 
 struct RyrDD { double hi, lo; };
@@ -168,7 +169,7 @@ struct RyrKArgs {
 
 #define RYR_NCNT 5
 #define RYR_MAXB 64
-#define RYR_BCOLS 10
+#define RYR_BCOLS 12
 
 /* ------------------------------------------------------------- reductions */
 RYR_DEV RyrDD dd_add(RyrDD a, RyrDD b)
@@ -1084,6 +1085,7 @@ RYR_DEV void beat_close(RyrMem *M, double *beats, double complete)
         r[0] = (double)M->beat_on; r[1] = M->b_apd90; r[2] = M->b_vmax; r[3] = M->b_vrest;
         r[4] = (M->beat_flags & 2) ? 1.0 : 0.0; r[5] = M->b_ead_t; r[6] = M->b_peak_ci;
         r[7] = M->b_jsr; r[8] = M->b_sparks; r[9] = complete;
+        r[10] = (M->beat_flags & 8) ? 1.0 : 0.0; r[11] = M->b_eadl_t;
     }
     M->beat_n++;
 }
@@ -1102,9 +1104,16 @@ RYR_DEV void beat_track(RyrMem *M, const RyrCellInfo &C, long long n, double dt,
         M->beat_on = n;
         M->b_vrest = vo; M->b_vmax = vo; M->b_vmin = vo;
         M->b_apd90 = -1.0; M->b_ead_t = -1.0; M->beat_flags = 0;
-        M->b_peak_ci = 0.0; M->b_jsr = -1.0; M->b_sparks = 0.0;
+        M->b_peak_ci = 0.0; M->b_jsr = -1.0; M->b_sparks = 0.0; M->b_eadl_t = -1.0;
     }
     if (M->beat_on < 0) return;
+    /* the literal flag: after the upstroke V starts to fall (bit2); then any rise ending above -40 mV */
+    if (!(M->beat_flags & 4)) {
+        if (M->b_vmax > -40.0 && vn < vo) M->beat_flags |= 4;
+    } else if (!(M->beat_flags & 8) && vn > vo && vn > -40.0) {
+        M->beat_flags |= 8;
+        M->b_eadl_t = (double)(n + 1) * dt - (double)M->beat_on * dt;
+    }
     if (vn > M->b_vmax) M->b_vmax = vn;
     const double amp = M->b_vmax - M->b_vrest;
     if (!(M->beat_flags & 1)) {

@@ -1181,6 +1181,7 @@ extern "C" int ryr_read_beats(ryr_handle *h, int cell, double *out, int maxbeats
         r[0] = (double)M.beat_on * h->p.dt; r[1] = M.b_apd90; r[2] = M.b_vmax; r[3] = M.b_vrest;
         r[4] = (M.beat_flags & 2) ? 1.0 : 0.0; r[5] = M.b_ead_t; r[6] = M.b_peak_ci; r[7] = M.b_jsr;
         r[8] = M.b_sparks; r[9] = 0.0;
+        r[10] = (M.beat_flags & 8) ? 1.0 : 0.0; r[11] = M.b_eadl_t;
         ++k;
     }
     *nbeats = k;

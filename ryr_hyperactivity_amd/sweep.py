@@ -43,8 +43,9 @@ class Point:
 @dataclass
 class PointResult:
     point: Point
-    beats: list                      # per replica: (nbeats x 10) completed-beat rows
+    beats: list                      # per replica: (nbeats x BEAT_COLS) completed-beat rows
     ead: float = float("nan")        # incidence over beats 2..n, mean over replicas
+    ead_lit: float = float("nan")    # incidence of the literal flag (dV/dt > 0 above -40 mV in repolarisation)
     ead_lo: float = float("nan")     # 95% interval (Wilson, on the pooled beats)
     ead_hi: float = float("nan")
     apd90: float = float("nan")
@@ -84,7 +85,7 @@ def wilson(k: int, n: int, z: float = 1.959964):
 
 
 def summarize(res: PointResult, ncru: int) -> PointResult:
-    per_rep, apds, pk, jl, sp = [], [], [], [], []
+    per_rep, per_rep_lit, apds, pk, jl, sp = [], [], [], [], [], []
     k = n = 0
     for b in res.beats:
         tail = b[1:]
@@ -92,6 +93,7 @@ def summarize(res: PointResult, ncru: int) -> PointResult:
             continue
         e = (tail[:, 4] > 0).astype(float)
         per_rep.append(e.mean())
+        per_rep_lit.append(float((tail[:, 10] > 0).mean()))
         k += int(e.sum())
         n += len(e)
         ok = tail[:, 1] > 0
@@ -102,6 +104,7 @@ def summarize(res: PointResult, ncru: int) -> PointResult:
         sp.append(1000.0 * tail[:, 8].mean() / ncru)
     if per_rep:
         res.ead = float(np.mean(per_rep))
+        res.ead_lit = float(np.mean(per_rep_lit))
         res.ead_lo, res.ead_hi = wilson(k, n)
     if apds:
         res.apd90 = float(np.mean(apds))
@@ -299,7 +302,8 @@ def main(argv=None):
     report = {"lattice": lat, "replicas": args.replicas, "nbeats": args.nbeats, "grid": len(grid)}
     if args.mode == "plain":
         res, cost = simulate_points(grid, args.replicas, args.nbeats, lat, seed0=args.seed, progress=log)
-        report["points"] = [dict(ku2=p.ku2, ikr=p.ikr, cl=p.cl, ead=r.ead, ead_lo=r.ead_lo, ead_hi=r.ead_hi,
+        report["points"] = [dict(ku2=p.ku2, ikr=p.ikr, cl=p.cl, ead=r.ead, ead_lit=r.ead_lit, ead_lo=r.ead_lo,
+                                 ead_hi=r.ead_hi,
                                  apd90=r.apd90, apd90_ci=r.apd90_ci) for p, r in res.items()]
         report["cost"] = cost.__dict__
     else:

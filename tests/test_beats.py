@@ -55,6 +55,10 @@ def test_apd90_and_ead_on_synthetic_trace():
     assert list(beats[:, 4]) == [0.0, 1.0, 0.0]
     assert 230.0 < beats[1, 5] < 240.0
     assert np.allclose(beats[:, 2], 40.0) and np.allclose(beats[:, 3], -86.0)
+    # the literal flag (dV/dt > 0 above -40 mV once repolarisation began): only the hump, from its first step
+    # (at +230 ms the ramp is at -25.8 mV and the hump's slope 10 pi / 20 mV/ms beats the ramp's -0.86)
+    assert list(beats[:, 10]) == [0.0, 1.0, 0.0]
+    assert beats[1, 11] == pytest.approx(230.0 + DT, abs=DT / 2) and beats[0, 11] == -1.0
 
 
 def test_no_ead_for_spike_notch_dome():
@@ -68,13 +72,19 @@ def test_no_ead_for_spike_notch_dome():
         tr[notch] -= 15.0 * np.sin(np.pi * (s[notch] - 2.0) / 18.0)
     beats, _ = run_clamped(tr, tr.size)
     assert list(beats[:, 4]) == [0.0, 0.0]
+    # read literally, the dome's rise after the notch (far above -40 mV) is dV/dt > 0 during repolarisation:
+    # the literal flag counts it, the flag with the 40% / 50 ms floor does not.  The notch's minimum is where
+    # 40/198 = (15 pi/18) (-cos(pi (s-2)/18)); the first rising step ends within a step after it
+    s_min = 2.0 + 18.0 / np.pi * np.arccos(-(40.0 / 198.0) / (15.0 * np.pi / 18.0))
+    assert list(beats[:, 10]) == [1.0, 1.0]
+    assert np.all((beats[:, 11] > s_min) & (beats[:, 11] <= s_min + 2 * DT))
 
 
 def test_failed_beat_has_no_apd():
     tr = np.full(int(round((100 + 2 * CL) / DT)), -86.0)
     beats, _ = run_clamped(tr, tr.size)
     assert beats.shape[0] == 2
-    assert np.all(beats[:, 1] == -1.0) and np.all(beats[:, 4] == 0.0)
+    assert np.all(beats[:, 1] == -1.0) and np.all(beats[:, 4] == 0.0) and np.all(beats[:, 10] == 0.0)
 
 
 def test_reference_clamp_protocols():

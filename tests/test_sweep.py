@@ -25,12 +25,13 @@ def test_wilson_interval():
 
 
 def test_summarize_counts_beats_after_the_first():
-    def beat(apd, ead, peak=1.0, jsr=700.0, sparks=10.0):
-        return [0, apd, 30, -80, ead, 60.0 if ead else -1, peak, jsr, sparks, 1]
+    def beat(apd, ead, peak=1.0, jsr=700.0, sparks=10.0, lit=1):
+        return [0, apd, 30, -80, ead, 60.0 if ead else -1, peak, jsr, sparks, 1, lit, 12.0 if lit else -1]
     reps = [np.array([beat(300, 1), beat(310, 1), beat(320, 0)]),        # first beat ignored
-            np.array([beat(250, 0), beat(260, 0), beat(-1, 1)])]         # a beat that never repolarised
+            np.array([beat(250, 0), beat(260, 0, lit=0), beat(-1, 1)])]  # a beat that never repolarised
     r = summarize(PointResult(Point(2.0, 0.0, 1000.0), reps), ncru=1000)
     assert r.ead == pytest.approx(0.5 * (0.5 + 0.5))
+    assert r.ead_lit == pytest.approx(0.5 * (1.0 + 0.5))
     assert (r.ead_lo, r.ead_hi) == pytest.approx(wilson(2, 4))
     assert r.apd90 == pytest.approx(0.5 * (315.0 + 260.0))
     assert r.sparks == pytest.approx(10.0)
