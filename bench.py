@@ -181,7 +181,7 @@ def roofline(r, workload):
     alg = r["ncru"] * r["unit_bytes"]
     ach = alg / (us * 1e-6) / 1e9
     return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles(k, workload),
+            "traffic": traffic_from_profiles(workload),
             "kernel": k, "kernel_us": us, "bytes_per_launch": alg, "unit_bytes": r["unit_bytes"],
             "step_GBs": r["step_GBs"], "step_frac": r["step_GBs"] / HBM_PEAK_GBS,
             "stored_bytes_per_cru_step": r["stored_bytes_per_cru_step"],
@@ -236,16 +236,20 @@ def ead_map_from_profiles():
     return d
 
 
-def traffic_from_profiles(kernel, workload):
-    """HBM bytes per launch of `kernel` on `workload` from the committed PMC
-    summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the factor
-    the same counter shows on a copy kernel of known size), or None."""
+def traffic_from_profiles(workload):
+    """HBM bytes of one step's per-CRU kernels (the sub-voxel sweep and the
+    CRU kernel, each one launch per step) on `workload`, from the committed
+    PMC summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the
+    factor the same counter shows on a copy kernel of known size), or None.
+    It is the measured counterpart of the unit's algorithmic bytes, which
+    also cover the whole step."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         t = json.load(f)
-    return t.get(f"{kernel}/{workload}", {}).get("hbm_bytes_per_launch")
+    parts = [t.get(f"{k}/{workload}", {}).get("hbm_bytes_per_launch") for k in ("k_cyto", "k_cru_step")]
+    return None if None in parts else sum(parts)
 
 
 def sub_record(r, workload, steps, warmup):
