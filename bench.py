@@ -75,7 +75,7 @@ def kernel_bytes(precision, sched=0):
     return b
 
 
-SCHED_NAMES = ["sequential", "membrane-fused", "concurrent", "membrane-and-gating-fused"]
+SCHED_NAMES = ["sequential", "membrane-fused", "concurrent", "membrane-and-gating-fused", "one-launch"]
 
 
 def make_cell(prec):

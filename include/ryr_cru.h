@@ -282,6 +282,13 @@ int ryr_kernel_time(ryr_handle *h, double *ms, int64_t *launches);
  *   RYR_SCHED_FUSED_GATING as FUSED, and the RyR gating of step n (which reads
  *                         only step-start values) rides in the sub-voxel launch
  *                         of step n too, off the CRU kernel (CRU size 128/256/512)
+ *   RYR_SCHED_ONE         one launch per step (cleft scheme A, CRU size 256;
+ *                         the sub-voxel size is not used): the sweep, the CRU
+ *                         update and a membrane workgroup per cell side by side.
+ *                         The CRU workgroups form V themselves from the previous
+ *                         step's sums and the membrane values its launch left,
+ *                         so no launch waits for the membrane step; the
+ *                         membrane workgroup finishes the previous step
  * No reference counterpart (the reference hard-codes one 48x448 launch,
  * :1701).  A new handle tunes the shape itself: its first 9 steps alternate
  * the sizes with each kernel timed by its own events (fastest median kept);
@@ -297,6 +304,7 @@ int ryr_kernel_time(ryr_handle *h, double *ms, int64_t *launches);
 #define RYR_SCHED_FUSED      1
 #define RYR_SCHED_CONCURRENT 2
 #define RYR_SCHED_FUSED_GATING 3
+#define RYR_SCHED_ONE        4
 int ryr_set_launch_shape(ryr_handle *h, int wg_cyto, int wg_cru, int sched);
 int ryr_get_launch_shape(const ryr_handle *h, int *wg_cyto, int *wg_cru, int *sched);
 

@@ -117,6 +117,26 @@ struct RyrMem {
 
 struct RyrDD { double hi, lo; };
 
+/* The membrane step of step n split for the one-launch schedule: the state
+ * at the start of step n and what the four current groups form from it alone
+ * (the gates of step n+1 and the ionic currents of step n, slots RYR_MS_*);
+ * [Na]i and V of step n+1 then need only the step's whole-cell sums.  Two
+ * per cell, by step parity. */
+#define RYR_PEND_MS 24
+struct RyrPend {
+    double s[13];           /* v xm xh xj xr xs1 xs2 qks xtos ytos xtof ytof xnai at the start of step n */
+    double ms[RYR_PEND_MS];
+};
+
+/* One per cell: V of step `step` and its V-only factors (RyrVF order), which
+ * the membrane workgroup of the one-launch schedule publishes for the CRU
+ * workgroups of the same launch.  `step` is written last, with release. */
+struct RyrVBox {
+    double v, f[13];
+    long long step;
+    long long pad_[1];
+};
+
 /* The V-only factors of the step that the CRU kernel reads (copied from
  * RyrMem at the kernel's start, so that their loads issue with the state
  * loads instead of inside the LCC and NCX branches) */
@@ -155,8 +175,10 @@ struct RyrKArgs {
                                     workgroup reaches its cell parameters in one load */
     long long *gstep;       /* step index a launch's offsets count from (one per handle) */
     RyrMem *mem;
+    RyrPend *pend;          /* per cell x 2 (step parity): the one-launch schedule's pending membrane values */
+    RyrVBox *vbox;          /* per cell: V published within a launch of the one-launch schedule */
     const int *block_cell;
-    RyrDD *partial;         /* whole-cell sums: 3 per wave, 4 waves per chunk */
+    RyrDD *partial;         /* whole-cell sums: 3 per wave, 4 waves per chunk; two sets, by step parity */
     unsigned long long *counters;  /* per cell x RYR_C_COUNT */
     double *diag;           /* per cell x 45 */
     long long *diag_step;   /* per cell */
@@ -166,6 +188,13 @@ struct RyrKArgs {
     RyrDD *dpartial;        /* output-row chunk sums: per block x 21 */
     unsigned long long *dcount;   /* output-row integer counts: per chunk x 13 */
 };
+
+/* the whole-cell partial sums of step n: the CRU kernel of step n writes set
+ * n mod 2, so that a launch can read step n-1's sums while it writes step n's */
+RYR_DEV RyrDD *step_partials(const RyrKArgs &a, long long nstep)
+{
+    return a.partial + (nstep & 1) * 12LL * (a.P >> 8);
+}
 
 #define RYR_NCNT 5
 #define RYR_MAXB 64
@@ -622,6 +651,7 @@ RYR_DEV void ryr_gating(uint64_t seed, uint32_t cru, uint64_t step, double Ku, d
  * wave or lane computed it.  Each group leaves its new gates and currents in
  * shared slots RYR_MS_*; every wave then forms [Na]i and V from them. */
 #define RYR_MS_N 24
+static_assert(RYR_MS_N == RYR_PEND_MS, "pending membrane slots");
 enum {
     RYR_MS_V = 0, RYR_MS_XM, RYR_MS_XH, RYR_MS_XJ, RYR_MS_XR, RYR_MS_XS1, RYR_MS_XS2, RYR_MS_QKS,
     RYR_MS_XTOS, RYR_MS_YTOS, RYR_MS_XTOF, RYR_MS_YTOF, RYR_MS_NAI,
@@ -985,8 +1015,8 @@ RYR_DEV void mem_finish(const double *s, const double *sh, double dt, double sti
 }
 
 /* The V-only factors of the next step (vfactors): the exponentials and the
- * divisions spread over the lanes; lane 0 stores them. */
-RYR_DEV void vfactors_wave(RyrMem *M, int iso2, int lane, const double v, const double xnai)
+ * divisions spread over the lanes; every lane gets them. */
+RYR_DEV RyrVF vfactors_regs(int iso2, int lane, const double v, const double xnai)
 {
     RYR_STRICT_FP
     double dv5 = 5, dvk = 8, fv5 = -22.8, fvk = 9.1;
@@ -1036,21 +1066,33 @@ RYR_DEV void vfactors_wave(RyrMem *M, int iso2, int lane, const double v, const 
     const double finf = 1. - lanev(q3, 2);
     const double taufin = (0.02 - 0.007 * e3);
     const double r = lanev(q1, 3);
-    if (lane != 0) return;
-    M->alphad = dinf * taudin;
-    M->betad = (1 - dinf) * taudin;
-    M->alphaf = (finf) * taufin;
-    M->betaf = (1 - finf) * taufin;
-    M->ghk_small = __builtin_fabs(za) < 0.001;
-    M->ghk_e2 = e4;
-    M->ghk_A = 4.0 * 11.9 * za * 96.485 * 0.341;
-    M->ghk_D = M->ghk_e2 - 1.0;
-    M->ncx_t1 = 0.00359 * ((140.0) * (140.0) * (140.0)) * (1.0 + r * r * r);
+    RyrVF f;
+    f.alphad = dinf * taudin;
+    f.betad = (1 - dinf) * taudin;
+    f.alphaf = (finf) * taufin;
+    f.betaf = (1 - finf) * taufin;
+    f.ghk_small = __builtin_fabs(za) < 0.001;
+    f.ghk_e2 = e4;
+    f.ghk_A = 4.0 * 11.9 * za * 96.485 * 0.341;
+    f.ghk_D = f.ghk_e2 - 1.0;
+    f.ncx_t1 = 0.00359 * ((140.0) * (140.0) * (140.0)) * (1.0 + r * r * r);
     double p3 = xnai * xnai * xnai;
-    M->ncx_t3a = (1.3 + (1 * 1.8)) * p3;
-    M->ncx_n1 = e5 * p3 * (1 * 1.8);
-    M->ncx_n2 = e6 * ((140.0) * (140.0) * (140.0));
-    M->ncx_den2 = 1.0 + 0.27 * e6;
+    f.ncx_t3a = (1.3 + (1 * 1.8)) * p3;
+    f.ncx_n1 = e5 * p3 * (1 * 1.8);
+    f.ncx_n2 = e6 * ((140.0) * (140.0) * (140.0));
+    f.ncx_den2 = 1.0 + 0.27 * e6;
+    return f;
+}
+
+/* the same, stored into M by lane 0 */
+RYR_DEV void vfactors_wave(RyrMem *M, int iso2, int lane, const double v, const double xnai)
+{
+    const RyrVF f = vfactors_regs(iso2, lane, v, xnai);
+    if (lane != 0) return;
+    M->alphad = f.alphad; M->betad = f.betad; M->alphaf = f.alphaf; M->betaf = f.betaf;
+    M->ghk_small = f.ghk_small; M->ghk_e2 = f.ghk_e2; M->ghk_A = f.ghk_A; M->ghk_D = f.ghk_D;
+    M->ncx_t1 = f.ncx_t1; M->ncx_t3a = f.ncx_t3a; M->ncx_n1 = f.ncx_n1; M->ncx_n2 = f.ncx_n2;
+    M->ncx_den2 = f.ncx_den2;
 }
 
 RYR_DEV double stim_of(const RyrCellInfo &C, long long n, double amp)
@@ -1162,6 +1204,8 @@ RYR_DEV float fbuf(float b, float c, float dt, float sat, float kon, float koff,
  * membrane code's registers, hence a separate instance. */
 template <typename TF, typename TS, int WG>
 RYR_DEV void membrane_block(const RyrKArgs &a, int c, long long nstep, double stim_amp);
+RYR_DEV void cru_v_next_get(const RyrKArgs &a, const RyrCellInfo &C, long long nstep, double stim_amp,
+                            double *v_out, RyrVF *vf_out);
 #define RYR_GATE_LPC 4   /* lanes per CRU in the gating part: each takes two transitions */
 template <typename TF, typename TS, int WG>
 RYR_DEV void gating_block(const RyrKArgs &a, int L, long long nstep);
@@ -1484,8 +1528,8 @@ RYR_DEV void gating_block(const RyrKArgs &a, int L, long long nstep)
 
 /* ------------------------------------------------------------ step kernel */
 /* The CRU update for logical workgroup L of a launch (step gstep + kc). */
-template <typename TF, typename TS, int WG, bool SCHB, int LPC, bool GSPLIT>
-RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L)
+template <typename TF, typename TS, int WG, bool SCHB, int LPC, bool GSPLIT, bool VNEXT = false>
+RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L, double stim_amp = 0.0)
 {
     static_assert(LPC == 1 || (LPC == 2 && WG >= 128), "paired lanes need two waves per workgroup");
 
@@ -1503,51 +1547,66 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L)
     const bool diag = (nstep % spo) == 0;
     const int n = (bid - C.first_block) * RYR_BLOCK + part * CPW + (int)threadIdx.x / LPC;
     const int hl = LPC == 1 ? 0 : (int)threadIdx.x & (LPC - 1);   /* lane half of a shared CRU */
-    const bool lead = hl == 0;                                     /* the lane that stores */
     const bool active = n < C.N;
+    /* the lane that stores: under VNEXT every lane runs the update (see
+     * below), and only the leads of real CRUs store */
+    const bool lead = hl == 0 && active;
     const long long P = a.P;
     const double dt = a.dt;
 
     double r_icat = 0.0, r_inaca = 0.0, r_cs = 0.0;
 
-    if (active) {
-        const long long g = C.off + n;
-        const int X = C.X, Y = C.Y, Z = C.Z;
-        const int sz = X * Y;
-        const TF *__restrict__ ci0 = (const TF *)a.ci[par];
-        TF *__restrict__ ci1 = (TF *)a.ci[par ^ 1];
-        const TS *__restrict__ cn0 = (const TS *)a.cnsr[par];
-        TS *__restrict__ cn1 = (TS *)a.cnsr[par ^ 1];
-        const TF *__restrict__ cs0 = (const TF *)a.cs[par];
-        TF *__restrict__ cs1 = (TF *)a.cs[par ^ 1];
-        TF *catf = (TF *)a.catf, *cacal = (TF *)a.cacal, *casr = (TF *)a.casr;
-        TF *camyo = (TF *)a.camyo, *mgmyo = (TF *)a.mgmyo;
-        TS *cats = (TS *)a.cats;
-        const TF *cp0 = (const TF *)a.cp[par];
-        TF *cp1 = (TF *)a.cp[par ^ 1], *ancx = (TF *)a.Ancx;
-        TF *casar = (TF *)a.casar, *casarh = (TF *)a.casarh, *casarj = (TF *)a.casarj, *casarhj = (TF *)a.casarhj;
-        const TS *cjsr0 = (const TS *)a.cjsr[par];
-        TS *cjsr1 = (TS *)a.cjsr[par ^ 1], *cjsrbp = (TS *)a.cjsrb;
-
-        const double v = M->m[0];
-        /* The V-only factors and the NSR values of sub-voxels 0-1 are loaded
-         * with the rest of the state where registers allow (the instances that
-         * run one wave per SIMD anyway), and at their use sites in the
-         * instances held to 168 VGPRs, where keeping them live spills */
-        constexpr bool vf_early = WG >= 256 || SCHB;
-        RyrVF vf0{};
-        if constexpr (vf_early) vf0 = load_vf(M);
-        TF cil[2];
-        TS cnl[2] = {TS(0), TS(0)};
+    /* The CRU's slot.  A lane past the cell's last CRU (active false) points
+     * at a padding slot of the same 256-slot chunk, which exists, so every
+     * lane issues the state loads; under VNEXT they are then in flight while
+     * the wave forms V, which needs every lane. */
+    const long long g = C.off + n;
+    const int X = C.X, Y = C.Y, Z = C.Z;
+    const int sz = X * Y;
+    const TF *__restrict__ ci0 = (const TF *)a.ci[par];
+    TF *__restrict__ ci1 = (TF *)a.ci[par ^ 1];
+    const TS *__restrict__ cn0 = (const TS *)a.cnsr[par];
+    TS *__restrict__ cn1 = (TS *)a.cnsr[par ^ 1];
+    const TF *__restrict__ cs0 = (const TF *)a.cs[par];
+    TF *__restrict__ cs1 = (TF *)a.cs[par ^ 1];
+    TF *catf = (TF *)a.catf, *cacal = (TF *)a.cacal, *casr = (TF *)a.casr;
+    TF *camyo = (TF *)a.camyo, *mgmyo = (TF *)a.mgmyo;
+    TS *cats = (TS *)a.cats;
+    const TF *cp0 = (const TF *)a.cp[par];
+    TF *cp1 = (TF *)a.cp[par ^ 1], *ancx = (TF *)a.Ancx;
+    TF *casar = (TF *)a.casar, *casarh = (TF *)a.casarh, *casarj = (TF *)a.casarj, *casarhj = (TF *)a.casarhj;
+    const TS *cjsr0 = (const TS *)a.cjsr[par];
+    TS *cjsr1 = (TS *)a.cjsr[par ^ 1], *cjsrbp = (TS *)a.cjsrb;
+    /* The V-only factors and the NSR values of sub-voxels 0-1 are loaded
+     * with the rest of the state where registers allow (the instances that
+     * run one wave per SIMD anyway), and at their use sites in the
+     * instances held to 168 VGPRs, where keeping them live spills */
+    constexpr bool vf_early = WG >= 256 || SCHB || VNEXT;
+    TF cil[2];
+    TS cnl[2] = {TS(0), TS(0)};
 #pragma unroll
-        for (int ii = 0; ii < 2; ++ii) {
-            cil[ii] = ci0[ii * P + g];
-            if constexpr (vf_early) cnl[ii] = cn0[ii * P + g];
+    for (int ii = 0; ii < 2; ++ii) {
+        cil[ii] = ci0[ii * P + g];
+        if constexpr (vf_early) cnl[ii] = cn0[ii * P + g];
+    }
+    const TF csv = cs0[g];
+    TF cpv = cp0[g];
+    const TS cjsrv = cjsr0[g];
+    uint32_t lw = a.lccw[g], rw = a.ryrw[g], ax = a.aux[g];
+
+    /* VNEXT (one-launch schedule, after the run's first step): V of this
+     * step and its V-only factors come from this launch's membrane workgroup
+     * or, failing that, are formed by each wave (cru_v_next_get), which needs
+     * every lane; so every lane runs the update (a lane past the cell's last
+     * CRU on its padding slot, stores masked by `lead`), and the RyR gating,
+     * which needs no V, runs first, while V is being formed */
+    if (active || VNEXT) {
+        double v = 0.0;
+        RyrVF vf0{};
+        if constexpr (!VNEXT) {
+            v = M->m[0];
+            if constexpr (vf_early) vf0 = load_vf(M);
         }
-        const TF csv = cs0[g];
-        TF cpv = cp0[g];
-        const TS cjsrv = cjsr0[g];
-        uint32_t lw = a.lccw[g], rw = a.ryrw[g], ax = a.aux[g];
         /* the draw blocks of LCC channels 0-3 need only the key, so in the
          * instances that have the registers they are formed while the state
          * loads are in flight */
@@ -1584,10 +1643,29 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L)
                 }
         }
 
+        const double cp_d = (double)cpv, cs_d = (double)csv;
+        /* RyR gating (:1274-1284; subroutine_cuda.c:251-669): reads only
+         * step-start values, so where it runs does not change a result */
+        int nou = (int)(rw & 255u), ncu = (int)((rw >> 8) & 255u), nob = (int)((rw >> 16) & 255u),
+            ncb = (int)(rw >> 24);
+        const int prevsp = nou + nob;
+        double roo2;
+        const double mono = csqn_mono((double)cjsrv, &roo2);
+        auto gate = [&]() {
+            int ndraw = 0;
+            ryr_gating<LPC>(C.seed, (uint32_t)n, (uint64_t)nstep, C.Ku, C.Ku2, C.Kb, cp_d, (double)cjsrv, mono, dt,
+                            nou, ncu, nob, ncb, &nfail, &ndraw, hl);
+            if constexpr (LPC == 2) nfail += (int)lane_pair_u32((uint32_t)nfail);
+            if (prevsp < 5 && (nou + nob) >= 5) nspark = 1;
+        };
+        if constexpr (VNEXT) {
+            gate();
+            cru_v_next_get(a, C, nstep, stim_amp, &v, &vf0);
+        }
+
         /* LCC gating and I_CaL (cuda_Cell_Detailed.cu:1248-1272): the slots
          * up to the first absent one, as the reference's loop; channel l
          * draws static slot l */
-        const double cp_d = (double)cpv, cs_d = (double)csv;
         double icat1 = 0.0, xicat, bcgcur = 0.0;
         {
             RYR_STRICT_FP
@@ -1663,19 +1741,9 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L)
             xicat = icat1 + 0.0 + bcgcur * ((0.025) / (0.00126));
         }
 
-        /* RyR gating, spark, release (:1274-1293) */
-        int nou = (int)(rw & 255u), ncu = (int)((rw >> 8) & 255u), nob = (int)((rw >> 16) & 255u),
-            ncb = (int)(rw >> 24);
-        const int prevsp = nou + nob;
-        double roo2;
-        const double mono = csqn_mono((double)cjsrv, &roo2);
-        if constexpr (!GSPLIT) {                    /* GSPLIT: the counts are this step's already */
-            int ndraw = 0;
-            ryr_gating<LPC>(C.seed, (uint32_t)n, (uint64_t)nstep, C.Ku, C.Ku2, C.Kb, cp_d, (double)cjsrv, mono, dt,
-                            nou, ncu, nob, ncb, &nfail, &ndraw, hl);
-            if constexpr (LPC == 2) nfail += (int)lane_pair_u32((uint32_t)nfail);
-            if (prevsp < 5 && (nou + nob) >= 5) nspark = 1;
-        }
+        /* RyR gating (unless done above, or by the gating launch: GSPLIT),
+         * spark, release (:1274-1293) */
+        if constexpr (!GSPLIT && !VNEXT) gate();
         double xire;
         {
             RYR_STRICT_FP
@@ -1909,7 +1977,7 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L)
         t2 = wave_tree(t2);
         if constexpr (LPC == 1) {
             if (lane == 0) {
-                RyrDD *pp = a.partial + 12LL * bid + 3 * (part * (WG / 64) + wave);
+                RyrDD *pp = step_partials(a, nstep) + 12LL * bid + 3 * (part * (WG / 64) + wave);
                 pp[0] = t0; pp[1] = t1; pp[2] = t2;
             }
         } else {
@@ -1917,7 +1985,7 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L)
             if (lane == 0) { s_half[wave][0] = t0; s_half[wave][1] = t1; s_half[wave][2] = t2; }
             __syncthreads();
             if (lane == 0 && !(wave & 1)) {
-                RyrDD *pp = a.partial + 12LL * bid + 3 * ((part * (WG / 64) + wave) / 2);
+                RyrDD *pp = step_partials(a, nstep) + 12LL * bid + 3 * ((part * (WG / 64) + wave) / 2);
 #pragma unroll
                 for (int q = 0; q < 3; ++q) pp[q] = dd_add(s_half[wave][q], s_half[wave + 1][q]);
             }
@@ -1955,7 +2023,7 @@ RYR_DEV void membrane_block(const RyrKArgs &a, int c, long long nstep, double st
     const double dt = a.dt;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     RyrDD acc[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
-    const RyrDD *pp = a.partial + 12LL * C.first_block;
+    const RyrDD *pp = step_partials(a, nstep) + 12LL * C.first_block;
     __shared__ double s_ms[RYR_MS_N];
     MemIn I;
     I.s = M->m; I.svikr = C.svikr; I.Cm = C.Cm; I.dN = C.dN; I.dt = dt;
@@ -2079,11 +2147,304 @@ RYR_DEV void membrane_block(const RyrKArgs &a, int c, long long nstep, double st
     if (lane == 0) M->step = nstep + 1;
 }
 
+/* ------------------------------------------- the one-launch schedule's parts
+ * A launch of step n runs, side by side, the sub-voxel sweep of step n, the
+ * CRU update of step n and one membrane workgroup per cell.  The CRU
+ * workgroups need V of step n, which the membrane step of step n-1 forms
+ * from step n-1's whole-cell sums; instead of waiting for it they form it
+ * themselves from those sums and from the values the membrane workgroup of
+ * the previous launch left pending (RyrPend: the state at the start of step
+ * n-1 and its current groups), with the same expressions in the same order,
+ * so every workgroup gets the same bits.  The membrane workgroup of the launch
+ * finishes step n-1 (stores the state, tracks the beat) and forms the current
+ * groups of step n for the next launch.  The first launch of a run, whose
+ * membrane state is complete, only forms those groups. */
+
+/* The three whole-cell sums of step nstep (I_CaL, I_NCX, cs) over the cell's
+ * chunk partials: the tree and order of membrane_block.  Every thread of the
+ * workgroup calls it; each gets the sums. */
+template <int NW>
+RYR_DEV void reduce_cell_sums(const RyrKArgs &a, const RyrCellInfo &C, long long nstep, RyrDD acc[3])
+{
+    __shared__ RyrDD s_red2[3][4];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const RyrDD *pp = step_partials(a, nstep) + 12LL * C.first_block;
+    for (int q = 0; q < 3; ++q) { acc[q].hi = 0.0; acc[q].lo = 0.0; }
+    for (int g0 = 0; g0 < C.nblocks; g0 += 256) {
+        for (int sl = wv; sl < 4; sl += NW) {
+            const int bb = g0 + sl * 64 + lane;
+            if (g0 + sl * 64 >= C.nblocks) {
+                if (lane == 0)
+                    for (int q = 0; q < 3; ++q) { s_red2[q][sl].hi = 0.0; s_red2[q][sl].lo = 0.0; }
+                continue;
+            }
+            RyrDD t[3];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                if (bb < C.nblocks) {
+                    const RyrDD *w = pp + 12LL * bb + q;
+                    t[q] = dd_add(dd_add(w[0], w[3]), dd_add(w[6], w[9]));
+                } else { t[q].hi = 0.0; t[q].lo = 0.0; }
+                t[q] = wave_tree(t[q]);
+            }
+            if (lane == 0) { s_red2[0][sl] = t[0]; s_red2[1][sl] = t[1]; s_red2[2][sl] = t[2]; }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+            acc[q] = dd_add(acc[q], dd_add(dd_add(s_red2[q][0], s_red2[q][1]), dd_add(s_red2[q][2], s_red2[q][3])));
+        __syncthreads();
+    }
+}
+
+/* Wave 0: the sums-dependent part of the membrane step of step np from the
+ * pending values pd and the sums acc: fills s_ms's sums slots (csubt, qks,
+ * I_CaL, I_NCX), and returns V and [Na]i after it (V replaced by a clamp
+ * protocol's value), as membrane_block does. */
+RYR_DEV void pend_finish(const RyrCellInfo &C, const RyrPend *pd, const RyrDD acc[3], long long np, double dt,
+                         double stim_amp, int lane, double *s_ms, double *v_out, double *nai_out)
+{
+    MemIn I;
+    I.s = pd->s; I.svikr = C.svikr; I.Cm = C.Cm; I.dN = C.dN; I.dt = dt;
+    I.slowncx = (C.variants & RYR_VAR_SLOWNCX) != 0;
+    I.sum_icat = acc[0].hi; I.sum_inaca = acc[1].hi; I.sum_cs = acc[2].hi;
+    mem_sums(I, lane, s_ms);
+    const double stim = stim_of(C, np, stim_amp);
+    double v, nai;
+    mem_finish(pd->s, s_ms, dt, stim, &v, &nai);
+    if (C.vclamp_n > 0) {
+        long long q = C.vclamp_periodic ? np % C.vclamp_n : (np < C.vclamp_n ? np : C.vclamp_n - 1);
+        v = C.vclamp[q];
+    }
+    *v_out = v;
+    *nai_out = nai;
+}
+
+/* The current groups of the state s (13 values) on the workgroup's waves,
+ * into pending slot pd (all threads call; s must stay unchanged meanwhile). */
+template <int NW>
+RYR_DEV void pend_groups(const RyrCellInfo &C, const double *s, double dt, RyrPend *pd)
+{
+    __shared__ double s_ms[RYR_MS_N];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    MemIn I;
+    I.s = s; I.svikr = C.svikr; I.Cm = C.Cm; I.dN = C.dN; I.dt = dt;
+    I.slowncx = (C.variants & RYR_VAR_SLOWNCX) != 0;
+    I.sum_icat = 0.0; I.sum_inaca = 0.0; I.sum_cs = 0.0;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        if (g % NW != wv) continue;
+        if (g == 0) mem_ina(I, lane, s_ms);
+        else if (g == 1) mem_ikr_iks(I, lane, s_ms);
+        else if (g == 2) mem_ik1_ito(I, lane, s_ms);
+        else mem_inak(I, lane, s_ms);
+    }
+    __syncthreads();
+    for (int t = (int)threadIdx.x; t < 13 + RYR_MS_N; t += (int)blockDim.x) {
+        if (t < 13) pd->s[t] = s[t];
+        else pd->ms[t - 13] = s_ms[t - 13];
+    }
+}
+
+/* Membrane workgroup of the first launch of a run (step nstep): the current
+ * groups of the complete state, pending for the next launch. */
+template <int NW>
+RYR_DEV void membrane_first_block(const RyrKArgs &a, int c, long long nstep)
+{
+    const RyrCellInfo C = a.cells[c];
+    __shared__ double s_st[13];
+    if (threadIdx.x < 13) s_st[threadIdx.x] = a.mem[c].m[threadIdx.x];
+    /* no V is published within this launch: a value left by an earlier run
+     * must not be taken for this run's */
+    if (threadIdx.x == 0) __hip_atomic_store(&a.vbox[c].step, -1LL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    pend_groups<NW>(C, s_st, a.dt, a.pend + 2LL * c + (nstep & 1));
+}
+
+/* Membrane workgroup of a later launch (step nstep): finishes the membrane
+ * step of nstep-1 from its pending values and sums, stores the state and the
+ * step's currents, tracks the beat (as membrane_block), then forms the
+ * current groups of the new state, pending for the next launch. */
+template <int NW>
+RYR_DEV void membrane_next_block(const RyrKArgs &a, int c, long long nstep, double stim_amp)
+{
+    const RyrCellInfo C = a.cells[c];
+    RyrMem *M = a.mem + c;
+    const long long np = nstep - 1;
+    const RyrPend *pd = a.pend + 2LL * c + (np & 1);
+    const double dt = a.dt;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    RyrDD acc[3];
+    reduce_cell_sums<NW>(a, C, np, acc);
+    __shared__ double s_ms[RYR_MS_N];
+    __shared__ double s_st[13];
+    if (threadIdx.x < RYR_MS_N) s_ms[threadIdx.x] = pd->ms[threadIdx.x];
+    __syncthreads();
+    if (wv == 0) {
+        double vnew, nai;
+        pend_finish(C, pd, acc, np, dt, stim_amp, lane, s_ms, &vnew, &nai);
+        /* publish V of step nstep and its V-only factors for this launch's CRU
+         * workgroups first: they wait for it after their RyR gating */
+        const RyrVF f = vfactors_regs(a.iso2, lane, vnew, nai);
+        if (lane == 0) {
+            RyrVBox *bx = a.vbox + c;
+            const double fv[13] = {f.alphad, f.betad, f.alphaf, f.betaf, f.ghk_A, f.ghk_e2, f.ghk_D, f.ncx_n1,
+                                   f.ncx_n2, f.ncx_t1, f.ncx_t3a, f.ncx_den2, (double)f.ghk_small};
+            __hip_atomic_store(&bx->v, vnew, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+            for (int q = 0; q < 13; ++q) __hip_atomic_store(&bx->f[q], fv[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&bx->step, nstep, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (lane == 0) {
+            double mo[13];
+            mo[0] = vnew;
+            for (int q = 1; q < 12; ++q) mo[q] = s_ms[q];
+            mo[12] = nai;
+            const double vold = pd->s[0];
+            const double stim = stim_of(C, np, stim_amp);
+            for (int q = 0; q < 13; ++q) { M->m[q] = mo[q]; s_st[q] = mo[q]; }
+            for (int q = 0; q < 10; ++q) M->cur[q] = s_ms[RYR_MS_CSUBT + q];
+            M->stim = stim;
+            M->out_ica += M->cur[1];
+            M->out_ncx += M->cur[2];
+            M->out_ina += M->cur[3];
+            beat_track(M, C, np, dt, vold, vnew, a.beats + (long long)c * RYR_MAXB * RYR_BCOLS);
+            if (!(vnew == vnew) || vnew == __builtin_inf() || vnew == -__builtin_inf())
+                atomicAdd(a.counters + (long long)c * RYR_NCNT + 4, 1ull);
+            M->step = nstep;
+        }
+    }
+    __syncthreads();
+    pend_groups<NW>(C, s_st, dt, a.pend + 2LL * c + (nstep & 1));
+}
+
+/* In each wave of a CRU workgroup of a later launch (step nstep), with
+ * every lane active: V of step nstep and its V-only factors, formed from
+ * step nstep-1's pending values and sums with no LDS and no barrier, so that
+ * the workgroup's state loads stay in flight meanwhile.  The sums are the
+ * same tree as reduce_cell_sums (each 64-chunk slice a wave tree, slices
+ * pairwise, groups of 256 chunks in order), and [Na]i and V the same
+ * expressions as mem_sums and mem_finish. */
+RYR_DEV void cru_v_next_wave(const RyrKArgs &a, const RyrCellInfo &C, long long nstep, double stim_amp,
+                             double *v_out, RyrVF *vf_out)
+{
+    RYR_STRICT_FP
+    const long long np = nstep - 1;
+    const RyrPend *pd = a.pend + 2LL * C.cell + (np & 1);
+    const int lane = threadIdx.x & 63;
+    const RyrDD *pp = step_partials(a, np) + 12LL * C.first_block;
+    RyrDD acc[2] = {{0.0, 0.0}, {0.0, 0.0}};
+    for (int g0 = 0; g0 < C.nblocks; g0 += 256) {
+        RyrDD s[2][4];
+#pragma unroll
+        for (int sl = 0; sl < 4; ++sl) {
+            const int bb = g0 + sl * 64 + lane;
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                if (g0 + sl * 64 >= C.nblocks) { s[q][sl].hi = 0.0; s[q][sl].lo = 0.0; continue; }
+                RyrDD t;
+                if (bb < C.nblocks) {
+                    const RyrDD *w = pp + 12LL * bb + q;
+                    t = dd_add(dd_add(w[0], w[3]), dd_add(w[6], w[9]));
+                } else { t.hi = 0.0; t.lo = 0.0; }
+                s[q][sl] = wave_tree(t);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+            acc[q] = dd_add(acc[q], dd_add(dd_add(s[q][0], s[q][1]), dd_add(s[q][2], s[q][3])));
+    }
+    /* I_CaL and I_NCX of the sums (mem_sums), then [Na]i and V (mem_finish) */
+    double n = 1.0, d = 1.0;
+    if (lane == 1) { n = acc[0].hi; d = C.Cm; }
+    if (lane == 2) { n = acc[1].hi; d = C.Cm; }
+    const double q1 = n / d;
+    double sh[RYR_MS_N];
+#pragma unroll
+    for (int k = 0; k < RYR_MS_N; ++k) sh[k] = 0.0;
+    sh[RYR_MS_INA] = pd->ms[RYR_MS_INA]; sh[RYR_MS_IK1] = pd->ms[RYR_MS_IK1];
+    sh[RYR_MS_IKR] = pd->ms[RYR_MS_IKR]; sh[RYR_MS_IKS] = pd->ms[RYR_MS_IKS];
+    sh[RYR_MS_ITOS] = pd->ms[RYR_MS_ITOS]; sh[RYR_MS_ITOF] = pd->ms[RYR_MS_ITOF];
+    sh[RYR_MS_INAK] = pd->ms[RYR_MS_INAK];
+    sh[RYR_MS_ICAT] = lanev(q1, 1) * 0.0965 * (0.00126) * 2.0 * ((0.5) + 1.);
+    sh[RYR_MS_INACA] = lanev(q1, 2) * 0.0965 * (0.025);
+    double v, nai;
+    mem_finish(pd->s, sh, a.dt, stim_of(C, np, stim_amp), &v, &nai);
+    if (C.vclamp_n > 0) {
+        long long q = C.vclamp_periodic ? np % C.vclamp_n : (np < C.vclamp_n ? np : C.vclamp_n - 1);
+        v = C.vclamp[q];
+    }
+    *v_out = v;
+    *vf_out = vfactors_regs(a.iso2, lane, v, nai);
+}
+
+/* V of step nstep for a CRU wave of a later launch (every lane active): the
+ * membrane workgroup's published values once it has released them, read
+ * past the L2 with agent-scope loads; if they do not appear within a bounded
+ * number of polls the wave forms them itself.  Either way the bits are the
+ * same. */
+#define RYR_VBOX_POLLS 4096
+RYR_DEV void cru_v_next_get(const RyrKArgs &a, const RyrCellInfo &C, long long nstep, double stim_amp,
+                            double *v_out, RyrVF *vf_out)
+{
+    const RyrVBox *bx = a.vbox + C.cell;
+    long long seen = -1;
+    for (int t = 0; t < RYR_VBOX_POLLS; ++t) {
+        seen = __hip_atomic_load(&bx->step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (seen == nstep) break;
+        __builtin_amdgcn_s_sleep(2);
+    }
+    /* the loads below stay after the poll */
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (seen == nstep) {
+        double f[13];
+#pragma unroll
+        for (int q = 0; q < 13; ++q) f[q] = __hip_atomic_load(&bx->f[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        *v_out = __hip_atomic_load(&bx->v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        vf_out->alphad = f[0]; vf_out->betad = f[1]; vf_out->alphaf = f[2]; vf_out->betaf = f[3];
+        vf_out->ghk_A = f[4]; vf_out->ghk_e2 = f[5]; vf_out->ghk_D = f[6]; vf_out->ncx_n1 = f[7];
+        vf_out->ncx_n2 = f[8]; vf_out->ncx_t1 = f[9]; vf_out->ncx_t3a = f[10]; vf_out->ncx_den2 = f[11];
+        vf_out->ghk_small = f[12] != 0.0;
+        return;
+    }
+    cru_v_next_wave(a, C, nstep, stim_amp, v_out, vf_out);
+}
+
 /* the membrane step on its own: one workgroup per cell, step gstep + km */
 template <typename TF, typename TS>
 __global__ __launch_bounds__(RYR_BLOCK) void k_membrane(RyrKArgs a, int km, double stim_amp)
 {
     membrane_block<TF, TS, RYR_BLOCK>(a, (int)blockIdx.x, *a.gstep + km, stim_amp);
+}
+
+/* One launch per step (cleft scheme A): membrane workgroups (one per cell),
+ * then the CRU update of every 256-CRU chunk (RyR gating included), then the
+ * sub-voxel sweep; NEXT: the membrane step of the previous step is pending
+ * (see the one-launch schedule's parts above).  The head (membrane and CRU
+ * workgroups, which set the step's length) comes first in dispatch order and
+ * round-robin over the XCDs; it is padded to a multiple of eight blocks so
+ * that the sweep behind it keeps the XCD-by-XCD layout (xcd_remap) in which
+ * a chunk's eight planes share one L2. */
+__host__ __device__ inline int one_head(int ncells, int nchunks) { return (ncells + nchunks + 7) & ~7; }
+
+template <typename TF, typename TS, bool NEXT>
+__global__ __launch_bounds__(RYR_BLOCK) void k_one(RyrKArgs a, int kc, double stim_amp)
+{
+    const long long nstep = *a.gstep + kc;
+    const int nchunks = (int)(a.P >> 8);
+    const int b = (int)blockIdx.x;
+    const int head = one_head(a.ncells, nchunks);
+    if (b < a.ncells) {
+        if constexpr (NEXT) membrane_next_block<RYR_BLOCK / 64>(a, b, nstep, stim_amp);
+        else membrane_first_block<RYR_BLOCK / 64>(a, b, nstep);
+        return;
+    }
+    if (b < a.ncells + nchunks) {
+        cru_block<TF, TS, RYR_BLOCK, false, 1, false, NEXT>(a, kc, b - a.ncells, stim_amp);
+        return;
+    }
+    if (b < head) return;                          /* padding */
+    cyto_block<TF, TS, RYR_BLOCK>(a, xcd_remap(b - head, (int)gridDim.x - head), nstep);
 }
 
 /* ------------------------------------------------------- output-row kernel */

@@ -24,6 +24,8 @@
                                     hipStream_t s, hipEvent_t *ev);                             \
     hipError_t ryr_launch_diag_##sfx(const RyrKArgs &a, int nchunks, hipStream_t s,             \
                                      hipEvent_t *ev);                                           \
+    hipError_t ryr_launch_one_##sfx(const RyrKArgs &a, int nchunks, int kc, bool next,           \
+                                    double amp, hipStream_t s, hipEvent_t *ev);                 \
     hipError_t ryr_launch_init_##sfx(const RyrKArgs &a, const RyrInit *init, int nblocks,        \
                                      hipStream_t s);                                             \
     hipError_t ryr_launch_export_##sfx(const RyrKArgs &a, int cell, int field, int planes,       \
@@ -126,6 +128,16 @@ static hipError_t ryr_go_cru(const RyrKArgs &a, int nchunks, int wg, int kc, boo
                                     hipStream_t s, hipEvent_t *ev)                              \
     {                                                                                            \
         return ryr_go_cru<TF, TS>(a, nchunks, wg, kc, split, s, ev);                             \
+    }                                                                                            \
+    hipError_t ryr_launch_one_##sfx(const RyrKArgs &a, int nchunks, int kc, bool next,           \
+                                    double amp, hipStream_t s, hipEvent_t *ev)                  \
+    {                                                                                            \
+        /* membrane workgroups and one CRU workgroup per chunk (padded to a multiple of eight),  \
+         * then eight sweep workgroups per chunk */                                             \
+        const dim3 grid((unsigned)one_head(a.ncells, nchunks) + 8u * (unsigned)nchunks);         \
+        if (next) RYR_GO((k_one<TF, TS, true>), grid, dim3(RYR_BLOCK), s, ev, a, kc, amp);       \
+        else RYR_GO((k_one<TF, TS, false>), grid, dim3(RYR_BLOCK), s, ev, a, kc, amp);           \
+        return hipGetLastError();                                                                \
     }                                                                                            \
     hipError_t ryr_launch_mem_##sfx(const RyrKArgs &a, int ncells, int km, double amp,           \
                                     hipStream_t s, hipEvent_t *ev)                              \

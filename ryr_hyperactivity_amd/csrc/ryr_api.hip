@@ -322,7 +322,9 @@ extern "C" int ryr_create(int device, const ryr_params *p, const ryr_cell_desc *
     ALLOCT(d_cells, sizeof(RyrCellInfo) * ncells);
     ALLOCT(d_chunks, sizeof(RyrCellInfo) * h->nblocks);
     ALLOCT(d_bc, sizeof(int) * h->nblocks);
-    ALLOCT(d_part, sizeof(RyrDD) * 12 * h->nblocks);
+    ALLOCT(d_part, sizeof(RyrDD) * 2 * 12 * h->nblocks);
+    ALLOCT(a.pend, sizeof(RyrPend) * 2 * ncells);
+    ALLOCT(a.vbox, sizeof(RyrVBox) * ncells);
     ALLOCT(d_loff, sizeof(long long) * ncells);
     ALLOCT(a.gstep, sizeof(long long));
     ALLOC(a.beats, sizeof(double) * ncells * RYR_MAXB * RYR_BCOLS);
@@ -464,6 +466,15 @@ static hipError_t go_mem(ryr_handle *h, int km, hipEvent_t *ev)
     return RYR_F32(h) ? ryr_launch_mem_f32(h->a, h->ncells, km, h->p.stim_amp, h->stream, ev)
                       : ryr_launch_mem_f64(h->a, h->ncells, km, h->p.stim_amp, h->stream, ev);
 }
+/* one-launch schedule: the launch of step gstep+k (next: the membrane step of
+ * step gstep+k-1 is pending) */
+static hipError_t go_one(ryr_handle *h, int k, bool next, hipEvent_t *ev)
+{
+    if (h->null_launch) return go_null(h, (unsigned)one_head(h->ncells, h->nblocks) + 8u * (unsigned)h->nblocks, RYR_BLOCK);
+    return RYR_F32(h) ? ryr_launch_one_f32(h->a, h->nblocks, k, next, h->p.stim_amp, h->stream, ev)
+                      : ryr_launch_one_f64(h->a, h->nblocks, k, next, h->p.stim_amp, h->stream, ev);
+}
+
 static hipError_t go_diag(ryr_handle *h, hipEvent_t *ev)
 {
     if (h->null_launch) {
@@ -498,6 +509,17 @@ static hipError_t enqueue_run(ryr_handle *h, long long n, int sched)
         const long long m = h->step + i;
         const int k = (int)(m - h->gbase);
         const bool out = m % spo == 0;
+        if (sched == RYR_SCHED_ONE) {
+            e = go_one(h, k, pending, nullptr);
+            if (e == hipSuccess && (out || i == n - 1)) {
+                e = go_mem(h, k, nullptr);
+                if (e == hipSuccess && out) e = go_diag(h, nullptr);
+                pending = false;
+            } else {
+                pending = true;
+            }
+            continue;
+        }
         if (sched == RYR_SCHED_CONCURRENT) {
             e = hipEventRecord(h->fork, h->stream);
             if (e == hipSuccess) e = hipStreamWaitEvent(h->stream2, h->fork, 0);
@@ -556,7 +578,21 @@ static hipError_t enqueue_profiled_step(ryr_handle *h, bool last, bool *pending,
     const int sc = h->sched;
     for (int q = 0; q < 4; ++q) ran[q] = false;
     hipError_t e = hipSuccess;
-    if (sc == RYR_SCHED_CONCURRENT) {
+    if (sc == RYR_SCHED_ONE) {
+        /* the one launch carries the sweep, the CRU update and the membrane
+         * workgroups: stamped as the sub-voxel launch */
+        e = go_one(h, k, *pending, h->kev);
+        ran[0] = true;
+        *tail = h->kev[1];
+        if (e == hipSuccess && (out || last)) {
+            e = go_mem(h, k, h->kev + 4);
+            ran[2] = true;
+            *tail = h->kev[5];
+            *pending = false;
+        } else {
+            *pending = true;
+        }
+    } else if (sc == RYR_SCHED_CONCURRENT) {
         e = hipEventRecord(h->fork, h->stream);
         if (e == hipSuccess) e = hipStreamWaitEvent(h->stream2, h->fork, 0);
         if (e == hipSuccess) e = go_cru(h, h->wg_cru, k, h->kev + 2, h->stream2);
@@ -652,6 +688,7 @@ static int tune_step(ryr_handle *h)
 static bool shape_ok(const ryr_handle *h, int wg_cru, int sched)
 {
     if (sched == RYR_SCHED_CONCURRENT) return h->p.scheme == RYR_SCHEME_A;
+    if (sched == RYR_SCHED_ONE) return h->p.scheme == RYR_SCHEME_A && wg_cru == 256;
     if (sched == RYR_SCHED_FUSED_GATING) return wg_cru == 128 || wg_cru == 256 || wg_cru == 512;
     return sched == RYR_SCHED_SEQ || sched == RYR_SCHED_FUSED;
 }
@@ -678,6 +715,7 @@ static void sched_candidates(const ryr_handle *h, std::vector<std::array<int, 3>
     add(256, 512, RYR_SCHED_FUSED_GATING);
     add(h->wg_cyto, 256, RYR_SCHED_FUSED_GATING);
     if (h->p.scheme == RYR_SCHEME_A) {
+        add(256, 256, RYR_SCHED_ONE);
         add(h->wg_cyto, h->wg_cru, RYR_SCHED_CONCURRENT);
         add(128, 64, RYR_SCHED_CONCURRENT);
         add(256, 64, RYR_SCHED_CONCURRENT);

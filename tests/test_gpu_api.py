@@ -43,7 +43,7 @@ def test_step_counts_are_exact_through_the_tuning():
         _same(eng, 0, o, f"after {done} steps")
 
 
-@pytest.mark.parametrize("sched", [0, 1, 2, 3])
+@pytest.mark.parametrize("sched", [0, 1, 2, 3, 4])
 def test_prepared_segments_replay_bit_exactly(sched):
     """Segments of any length and phase, captured by ryr_prepare or at their
     second use, replay the same values as eager steps (FP64 == oracle)."""

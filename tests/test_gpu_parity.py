@@ -149,19 +149,23 @@ def test_config0_three_beats_from_baseline_ics_match_the_oracle():
     assert nrow == 1200
 
 
-def test_fp64_bitexact_on_a_cell_of_more_than_256_chunks():
+@pytest.mark.parametrize("shape", [None, (256, 256, 4)], ids=["tuned", "one-launch"])
+def test_fp64_bitexact_on_a_cell_of_more_than_256_chunks(shape):
     """A 160x20x21 cell (67,200 CRUs, 263 chunks): the whole-cell sums and
     the output-row sums run their loops over groups of 256 chunks more than
-    once, and the sums of the second group join the first in chunk order."""
+    once, and the sums of the second group join the first in chunk order
+    (under the one-launch schedule in every CRU workgroup too)."""
     p = Params.default(stim_offset=1.0)
     d = CellDesc.default(nx=160, ny=20, nz=21, cl=1000.0, seed=13)
-    print("case 160x20x21 FP64, 263 chunks", flush=True)
+    print(f"case 160x20x21 FP64, 263 chunks, shape {shape}", flush=True)
     o = OracleCell(p, d)
     with CellEngine(p, d) as eng:
-        for chunk in (1, 100):                        # an eager step, then a graph-replayed interval
+        if shape:
+            eng.set_launch_shape(*shape)
+        for chunk in (1, 100, 37):                        # an eager step, then a graph-replayed interval
             eng.step(chunk)
             o.step(chunk)
-        _assert_same_state(eng, 0, o, "after 101 steps")
+        _assert_same_state(eng, 0, o, "after 138 steps")
         grow, gst = eng.diag()
         orow, ost = o.diag()
         assert gst == ost == 100
@@ -197,12 +201,15 @@ def test_gating_probe_bitexact_including_gaussian_branch():
                 assert np.all(g[2][many] >= 4 + 3 + 7)
 
 
-def test_batch_equals_cells_run_alone():
+@pytest.mark.parametrize("shape", [None, (256, 256, 4)], ids=["tuned", "one-launch"])
+def test_batch_equals_cells_run_alone(shape):
     p = Params.default(stim_offset=2.0)
     cells = [CellDesc.default(nx=9, ny=4, nz=3, cl=1000.0, seed=1, Ku2=1.0),
              CellDesc.default(nx=30, ny=10, nz=2, cl=2000.0, seed=2, Ku2=2.5, ikr_scale=0.3),
              CellDesc.default(nx=5, ny=5, nz=5, cl=1000.0, seed=3, Ku=3.333, Kb=18.0)]
     with CellEngine(p, cells) as batch:
+        if shape:
+            batch.set_launch_shape(*shape)
         batch.step(230)
         bs = batch.scalars()
         for c, d in enumerate(cells):
@@ -336,7 +343,7 @@ def test_beats_and_clamp_match_the_oracle():
 
 @pytest.mark.parametrize("shape", [(64, 64, 1), (128, 256, 0), (256, 128, 1), (256, 256, 0), (64, 128, 2),
                                    (256, 64, 2), (256, 512, 1), (128, 512, 0), (64, 512, 2), (256, 256, 3),
-                                   (64, 128, 3), (128, 512, 3)])
+                                   (64, 128, 3), (128, 512, 3), (256, 256, 4)])
 def test_every_launch_shape_is_bitexact(shape):
     """Workgroup sizes and step schedules change no value: FP64 against the
     oracle, and FP32 against the FP32 run at another shape (whole-cell sums
@@ -379,7 +386,7 @@ def test_tuning_and_kernel_profile():
         assert eng.launch_shape() == (0, 0, -1)
         eng.step(200)                              # sizes after 9 steps, then the schedule candidates
         wc, wr, sc = eng.launch_shape()
-        assert wc in (64, 128, 256) and wr in (64, 128, 256, 512) and sc in (0, 1, 2, 3)
+        assert wc in (64, 128, 256) and wr in (64, 128, 256, 512) and sc in (0, 1, 2, 3, 4)
         prof = eng.profile_kernels(120)
         assert eng.cru_steps() == 325 * d.ncru
         assert eng.scalars()[0][S["STEP"]] == 325
