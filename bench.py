@@ -64,7 +64,8 @@ def kernel_bytes(precision, sched=0):
     FP64: 116 + 84 = 200 B.
     Schedule 3 (RyR gating in the sub-voxel launch): the sub-voxel launch also
     reads and writes the RyR-count and flag words, +16 B; the CRU kernel no
-    longer writes the RyR-count word, -4 B."""
+    longer writes the RyR-count word, -4 B.
+    Schedule 4 (one launch per step, k_one): the sum of the two."""
     if precision == 1:
         b = {"k_cyto": 656, "k_cru_step": 136}
     else:
@@ -72,6 +73,8 @@ def kernel_bytes(precision, sched=0):
     if sched == 3:
         b["k_cyto"] += 16
         b["k_cru_step"] -= 4
+    if sched == 4:
+        b = {"k_one": b["k_cyto"] + b["k_cru_step"]}
     return b
 
 
@@ -141,6 +144,10 @@ def run_gpu(p, cells, steps, warmup, prof_steps, shape=None, floor=False):
         shape = eng.launch_shape()
         prof = eng.profile_kernels(prof_steps)
         floor_us = eng.latency_floor(spo) if floor else None
+    if shape[2] == 4:
+        # one launch per step (k_one) carries the membrane step, the gating,
+        # the CRU update and the sweep; it is stamped in the sub-voxel slot
+        prof = {("k_one" if k == "k_cyto" else k): v for k, v in prof.items() if k != "k_cru_step"}
     sim_ms = len(cells) * steps * p.dt
     step_s = ev_ms * 1e-3 / max(launches, 1)
     kb = kernel_bytes(p.precision, shape[2])
@@ -155,7 +162,7 @@ def run_gpu(p, cells, steps, warmup, prof_steps, shape=None, floor=False):
     # concurrent schedule the two overlap; under the fused ones the sub-voxel launch carries the membrane step
     # and the gating).  The unit of work's algorithmic bytes (UNIT_BYTES per CRU-step, the whole step's state)
     # over that kernel's duration is the roofline's achieved rate.
-    dom = max(("k_cyto", "k_cru_step"), key=lambda k: kernels[k]["avg_us"])
+    dom = "k_one" if "k_one" in kernels else max(("k_cyto", "k_cru_step"), key=lambda k: kernels[k]["avg_us"])
     return {
         "cell_ms_per_s": sim_ms / wall,
         "wall_s": wall,
@@ -181,7 +188,7 @@ def roofline(r, workload):
     alg = r["ncru"] * r["unit_bytes"]
     ach = alg / (us * 1e-6) / 1e9
     return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles(workload),
+            "traffic": traffic_from_profiles(workload, k == "k_one"),
             "kernel": k, "kernel_us": us, "bytes_per_launch": alg, "unit_bytes": r["unit_bytes"],
             "step_GBs": r["step_GBs"], "step_frac": r["step_GBs"] / HBM_PEAK_GBS,
             "stored_bytes_per_cru_step": r["stored_bytes_per_cru_step"],
@@ -236,19 +243,20 @@ def ead_map_from_profiles():
     return d
 
 
-def traffic_from_profiles(workload):
+def traffic_from_profiles(workload, one_launch=False):
     """HBM bytes of one step's per-CRU kernels (the sub-voxel sweep and the
-    CRU kernel, each one launch per step) on `workload`, from the committed
-    PMC summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the
-    factor the same counter shows on a copy kernel of known size), or None.
-    It is the measured counterpart of the unit's algorithmic bytes, which
-    also cover the whole step."""
+    CRU kernel, each one launch per step, or the one-launch kernel) on
+    `workload`, from the committed PMC summary (FETCH_SIZE and WRITE_SIZE per
+    dispatch, each scaled by the factor the same counter shows on a copy
+    kernel of known size), or None.  It is the measured counterpart of the
+    unit's algorithmic bytes, which also cover the whole step."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         t = json.load(f)
-    parts = [t.get(f"{k}/{workload}", {}).get("hbm_bytes_per_launch") for k in ("k_cyto", "k_cru_step")]
+    names = ("k_one",) if one_launch else ("k_cyto", "k_cru_step")
+    parts = [t.get(f"{k}/{workload}", {}).get("hbm_bytes_per_launch") for k in names]
     return None if None in parts else sum(parts)
 
 

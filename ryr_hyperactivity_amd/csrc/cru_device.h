@@ -2427,20 +2427,20 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_membrane(RyrKArgs a, int km, doub
  * a chunk's eight planes share one L2. */
 __host__ __device__ inline int one_head(int ncells, int nchunks) { return (ncells + nchunks + 7) & ~7; }
 
-template <typename TF, typename TS, bool NEXT>
+template <typename TF, typename TS, bool NEXT, int LPC>
 __global__ __launch_bounds__(RYR_BLOCK) void k_one(RyrKArgs a, int kc, double stim_amp)
 {
     const long long nstep = *a.gstep + kc;
     const int nchunks = (int)(a.P >> 8);
     const int b = (int)blockIdx.x;
-    const int head = one_head(a.ncells, nchunks);
+    const int head = one_head(a.ncells, LPC * nchunks);
     if (b < a.ncells) {
         if constexpr (NEXT) membrane_next_block<RYR_BLOCK / 64>(a, b, nstep, stim_amp);
         else membrane_first_block<RYR_BLOCK / 64>(a, b, nstep);
         return;
     }
-    if (b < a.ncells + nchunks) {
-        cru_block<TF, TS, RYR_BLOCK, false, 1, false, NEXT>(a, kc, b - a.ncells, stim_amp);
+    if (b < a.ncells + LPC * nchunks) {
+        cru_block<TF, TS, RYR_BLOCK, false, LPC, false, NEXT>(a, kc, b - a.ncells, stim_amp);
         return;
     }
     if (b < head) return;                          /* padding */

@@ -24,7 +24,7 @@
                                     hipStream_t s, hipEvent_t *ev);                             \
     hipError_t ryr_launch_diag_##sfx(const RyrKArgs &a, int nchunks, hipStream_t s,             \
                                      hipEvent_t *ev);                                           \
-    hipError_t ryr_launch_one_##sfx(const RyrKArgs &a, int nchunks, int kc, bool next,           \
+    hipError_t ryr_launch_one_##sfx(const RyrKArgs &a, int nchunks, int lpc, int kc, bool next,  \
                                     double amp, hipStream_t s, hipEvent_t *ev);                 \
     hipError_t ryr_launch_init_##sfx(const RyrKArgs &a, const RyrInit *init, int nblocks,        \
                                      hipStream_t s);                                             \
@@ -129,14 +129,20 @@ static hipError_t ryr_go_cru(const RyrKArgs &a, int nchunks, int wg, int kc, boo
     {                                                                                            \
         return ryr_go_cru<TF, TS>(a, nchunks, wg, kc, split, s, ev);                             \
     }                                                                                            \
-    hipError_t ryr_launch_one_##sfx(const RyrKArgs &a, int nchunks, int kc, bool next,           \
+    hipError_t ryr_launch_one_##sfx(const RyrKArgs &a, int nchunks, int lpc, int kc, bool next,  \
                                     double amp, hipStream_t s, hipEvent_t *ev)                  \
     {                                                                                            \
-        /* membrane workgroups and one CRU workgroup per chunk (padded to a multiple of eight),  \
-         * then eight sweep workgroups per chunk */                                             \
-        const dim3 grid((unsigned)one_head(a.ncells, nchunks) + 8u * (unsigned)nchunks);         \
-        if (next) RYR_GO((k_one<TF, TS, true>), grid, dim3(RYR_BLOCK), s, ev, a, kc, amp);       \
-        else RYR_GO((k_one<TF, TS, false>), grid, dim3(RYR_BLOCK), s, ev, a, kc, amp);           \
+        /* membrane workgroups and lpc CRU workgroups per chunk (lpc lanes per CRU; padded to a  \
+         * multiple of eight), then eight sweep workgroups per chunk */                         \
+        if (lpc != 1 && lpc != 2) return hipErrorInvalidValue;                                   \
+        const dim3 grid((unsigned)one_head(a.ncells, lpc * nchunks) + 8u * (unsigned)nchunks);   \
+        if (lpc == 2) {                                                                          \
+            if (next) RYR_GO((k_one<TF, TS, true, 2>), grid, dim3(RYR_BLOCK), s, ev, a, kc, amp); \
+            else RYR_GO((k_one<TF, TS, false, 2>), grid, dim3(RYR_BLOCK), s, ev, a, kc, amp);    \
+        } else {                                                                                 \
+            if (next) RYR_GO((k_one<TF, TS, true, 1>), grid, dim3(RYR_BLOCK), s, ev, a, kc, amp); \
+            else RYR_GO((k_one<TF, TS, false, 1>), grid, dim3(RYR_BLOCK), s, ev, a, kc, amp);    \
+        }                                                                                        \
         return hipGetLastError();                                                                \
     }                                                                                            \
     hipError_t ryr_launch_mem_##sfx(const RyrKArgs &a, int ncells, int km, double amp,           \

@@ -23,12 +23,14 @@ from __future__ import annotations
 import argparse
 import json
 import math
+import os
+import sys
 import time
 from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._abi import PREC_F32
+from ._abi import BEAT_COLS, PREC_F32
 from .engine import CellEngine
 from ._abi import CellDesc, Params
 
@@ -149,20 +151,82 @@ def plan_batches(points, replicas: int, nbeats: int, ncru: int, dt: float = 0.02
     return groups
 
 
+class SweepPaused(Exception):
+    """The time budget ran out before every requested replica was simulated;
+    what was done is in the ReplicaStore."""
+
+
+class ReplicaStore:
+    """Per-replica results kept across processes: the completed-beat rows of
+    each simulated (point, replica) and its share of the batch's cost, in a
+    JSON file rewritten after every batch, so that a sweep longer than one
+    process continues where it stopped.  Keys hold the run's settings, and a
+    tag keeps apart runs that must not share results (a hold-out re-simulates
+    points on purpose).  Rows round-trip through JSON exactly."""
+
+    def __init__(self, path=None):
+        self.path = path
+        self.data = {"replicas": {}, "notes": {}}
+        if path and os.path.exists(path):
+            with open(path) as f:
+                self.data = json.load(f)
+
+    @staticmethod
+    def key(tag, pt, r, nbeats, lattice, seed0, precision):
+        return (f"{tag}|{lattice[0]}x{lattice[1]}x{lattice[2]}|{nbeats}|{seed0}|{precision}|"
+                f"{pt.ku2!r}|{pt.ikr!r}|{pt.cl!r}|{r}")
+
+    def get(self, k):
+        e = self.data["replicas"].get(k)
+        if e is None:
+            return None
+        return np.array(e["beats"], dtype=np.float64).reshape(-1, BEAT_COLS), e["cell_steps"], e["gpu_s"]
+
+    def put(self, k, beats, cell_steps, gpu_s):
+        self.data["replicas"][k] = {"beats": np.asarray(beats, dtype=np.float64).tolist(),
+                                    "cell_steps": int(cell_steps), "gpu_s": float(gpu_s)}
+
+    def note(self, name, value=None):
+        if value is not None:
+            self.data["notes"][name] = value
+        return self.data["notes"].get(name)
+
+    def save(self):
+        if not self.path:
+            return
+        tmp = self.path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(self.data, f)
+        os.replace(tmp, self.path)
+
+
 def simulate_points(points, replicas: int, nbeats: int, lattice=(20, 8, 4), precision: int = PREC_F32,
                     seed0: int = 1, device: int = 0, max_cells: int = 4096, chunk_steps: int = 20000,
-                    progress=None):
-    """Plain method on `points`: returns ({point: PointResult}, Cost)."""
+                    progress=None, r0: int = 0, store: ReplicaStore | None = None, tag: str = "",
+                    deadline: float | None = None):
+    """Plain method on `points` with replicas r0..replicas-1: returns
+    ({point: PointResult}, Cost).  Replicas already in `store` (same tag and
+    settings) are taken from it with their recorded cost; the others are
+    simulated in batches of at most `max_cells` cells and recorded.  A batch
+    is started only if its estimated duration (step_time_model) ends before
+    `deadline` (time.time() seconds); otherwise SweepPaused is raised."""
     nx, ny, nz = lattice
     ncru = nx * ny * nz
     p = Params.default(precision=precision, lqt=0)           # lqt off: the I_Kr scale applies
-    out, cost = {}, Cost()
-    for pts, nsteps in plan_batches(points, replicas, nbeats, ncru, p.dt, p.stim_offset):
+    store = store or ReplicaStore()
+    key = lambda pt, r: store.key(tag, pt, r, nbeats, lattice, seed0, precision)
+    missing = [pt for pt in points if any(store.get(key(pt, r)) is None for r in range(r0, replicas))]
+    for pts, nsteps in plan_batches(missing, replicas - r0, nbeats, ncru, p.dt, p.stim_offset):
         cl = max(q.cl for q in pts)
-        for g0 in range(0, len(pts), max(1, max_cells // replicas)):
-            grp = pts[g0:g0 + max(1, max_cells // replicas)]
+        todo = [(pt, r) for pt in pts for r in range(r0, replicas) if store.get(key(pt, r)) is None]
+        for g0 in range(0, len(todo), max_cells):
+            grp = todo[g0:g0 + max_cells]
+            est = step_time_model(len(grp) * ncru) * nsteps
+            if deadline is not None and time.time() + est > deadline:
+                store.save()
+                raise SweepPaused(f"{len(todo) - g0} replicas left at cl {cl:g} (next batch ~{est:.0f} s)")
             cells = [CellDesc.default(nx=nx, ny=ny, nz=nz, Ku2=pt.ku2, ikr_scale=pt.ikr, cl=pt.cl,
-                                      seed=point_seed(pt, r, seed0)) for pt in grp for r in range(replicas)]
+                                      seed=point_seed(pt, r, seed0)) for pt, r in grp]
             with CellEngine(p, cells, device=device) as eng:
                 eng.kernel_time()
                 done = 0
@@ -174,14 +238,19 @@ def simulate_points(points, replicas: int, nbeats: int, lattice=(20, 8, 4), prec
                         progress(f"cl {cl:g}: {done}/{nsteps} steps, {len(cells)} cells")
                 eng.sync()
                 ms, _ = eng.kernel_time()
-                for i, pt in enumerate(grp):
-                    reps = []
-                    for r in range(replicas):
-                        b = eng.beats(i * replicas + r)
-                        reps.append(b[b[:, 9] > 0][:nbeats].copy())
-                    out[pt] = summarize(PointResult(pt, reps), ncru)
-            cost.cell_steps += len(cells) * nsteps
-            cost.gpu_s += ms * 1e-3
+                for i, (pt, r) in enumerate(grp):
+                    b = eng.beats(i)
+                    store.put(key(pt, r), b[b[:, 9] > 0][:nbeats], nsteps, ms * 1e-3 / len(grp))
+            store.save()
+    out, cost = {}, Cost()
+    for pt in points:
+        reps = []
+        for r in range(r0, replicas):
+            b, cs, gs = store.get(key(pt, r))
+            reps.append(b)
+            cost.cell_steps += cs
+            cost.gpu_s += gs
+        out[pt] = summarize(PointResult(pt, reps), ncru)
     return out, cost
 
 
@@ -193,56 +262,98 @@ def point_cost(pt: Point, replicas: int, nbeats: int, dt: float = 0.025, stim_of
 @dataclass
 class SteeredMap:
     values: dict                     # point -> EAD incidence
-    source: dict                     # point -> "train" | "plain" | "surrogate"
+    source: dict                     # point -> "train" | "train+plain" | "plain" | "surrogate"
     cost: Cost
     predicted: dict = field(default_factory=dict)   # point -> (incidence, spread) from the surrogate
 
 
 def steered_map(grid, train_points, replicas: int, nbeats: int, lattice=(20, 8, 4), band=(0.1, 0.9),
                 spread_max: float = 0.1, n_models: int = 3, train_steps: int = 3000, device: int = 0,
-                seed0: int = 1, simulate=None, progress=None, n_near: int = 2) -> SteeredMap:
+                seed0: int = 1, simulate=None, progress=None, n_near: int = 2, train_replicas: int | None = None,
+                store: ReplicaStore | None = None, deadline: float | None = None) -> SteeredMap:
     """Surrogate-steered EAD map over `grid` (see the module docstring).
-    `simulate(points)` may replace the plain runs (it must return what
-    simulate_points returns); by default simulate_points is called."""
+
+    The training points are simulated with the first `train_replicas`
+    replicas (all of them by default).  After training, a point is simulated
+    with all `replicas` replicas if the surrogate's incidence lies in `band`
+    or its networks' spread exceeds `spread_max`; a training point also if its
+    own incidence over its training replicas lies in `band`, and then only its
+    remaining replicas are run.  A simulated point therefore has exactly the
+    plain method's value (same seeds, same replicas); an unflagged training
+    point keeps the incidence of its training replicas.
+
+    `simulate(points, r0, r1)` may replace the plain runs of replicas
+    r0..r1-1 (it must return what simulate_points returns).  With a `store`,
+    the steering decisions are recorded beside the replicas, so a run resumed
+    after SweepPaused takes the same decisions."""
     from .surrogate import BeatSurrogate, training_set
 
-    sim = simulate or (lambda pts: simulate_points(pts, replicas, nbeats, lattice, seed0=seed0, device=device,
-                                                   progress=progress))
+    tr = replicas if train_replicas is None else int(train_replicas)
+    assert 1 <= tr <= replicas
+    sim = simulate or (lambda pts, r0, r1: simulate_points(pts, r1, nbeats, lattice, seed0=seed0, device=device,
+                                                           progress=progress, r0=r0, store=store, tag="steered",
+                                                           deadline=deadline))
     ncru = lattice[0] * lattice[1] * lattice[2]
-    train_res, cost = sim(list(train_points))
-    recs = [(pt.ku2, pt.ikr, pt.cl, ncru, b) for pt, r in train_res.items() for b in r.beats if len(b) > 1]
-    x, y, t, mask = training_set(recs)
-    sur = BeatSurrogate(n_models=n_models, device=device, seed=seed0)
-    t0 = time.perf_counter()
-    sur.fit(x, y, t, mask, steps=train_steps, seed=seed0)
-    cost.train_s += time.perf_counter() - t0
-
-    # previous-beat records for a new point: the beats (all but the last of
-    # each replica) of its `n_near` nearest training points
+    train_res, cost = sim(list(train_points), 0, tr)
     tp = list(train_res)
-    tkey = np.array([[q.ku2, q.ikr, q.cl / 1000.0] for q in tp])
-    scale = np.maximum(tkey.std(0), 1e-9)
-    prev = {q: [row for b in train_res[q].beats for row in b[:-1]] for q in tp}
+    dkey = (f"decisions|{lattice[0]}x{lattice[1]}x{lattice[2]}|{replicas}|{tr}|{nbeats}|{seed0}|{band}|{spread_max}|"
+            f"{n_models}|{train_steps}|{n_near}|" + ";".join(f"{q.ku2!r},{q.ikr!r},{q.cl!r}" for q in grid)
+            + "|" + ";".join(f"{q.ku2!r},{q.ikr!r},{q.cl!r}" for q in tp))
+    saved = store.note(dkey) if store is not None else None
+    predicted, todo = {}, []
+    if saved is not None:
+        for row in saved["predicted"]:
+            predicted[Point(*row[:3])] = (row[3], row[4])
+        todo = [Point(*row) for row in saved["todo"]]
+        cost.train_s += saved["train_s"]
+    else:
+        recs = [(pt.ku2, pt.ikr, pt.cl, ncru, b) for pt, r in train_res.items() for b in r.beats if len(b) > 1]
+        x, y, t, mask = training_set(recs)
+        sur = BeatSurrogate(n_models=n_models, device=device, seed=seed0)
+        t0 = time.perf_counter()
+        sur.fit(x, y, t, mask, steps=train_steps, seed=seed0)
+        train_s = time.perf_counter() - t0
+        cost.train_s += train_s
+        # previous-beat records for a point: the beats (all but the last of
+        # each replica) of its `n_near` nearest training points
+        tkey = np.array([[q.ku2, q.ikr, q.cl / 1000.0] for q in tp])
+        scale = np.maximum(tkey.std(0), 1e-9)
+        prev = {q: [row for b in train_res[q].beats for row in b[:-1]] for q in tp}
+        for pt in grid:
+            d = np.sum(((tkey - [pt.ku2, pt.ikr, pt.cl / 1000.0]) / scale) ** 2, axis=1)
+            recs = [row for j in np.argsort(d)[:n_near] for row in prev[tp[int(j)]]]
+            inc, spr = sur.predict_point(pt.ku2, pt.ikr, pt.cl, recs, ncru)
+            predicted[pt] = (inc, spr)
+            unsure = band[0] <= inc <= band[1] or spr > spread_max
+            if pt in train_res:
+                unsure = tr < replicas and (unsure or band[0] <= train_res[pt].ead <= band[1])
+            if unsure:
+                todo.append(pt)
+        sur.close()
+        if store is not None:
+            store.note(dkey, {"predicted": [[q.ku2, q.ikr, q.cl, v[0], v[1]] for q, v in predicted.items()],
+                              "todo": [[q.ku2, q.ikr, q.cl] for q in todo], "train_s": train_s})
+            store.save()
 
-    values, source, predicted, todo = {}, {}, {}, []
+    values, source = {}, {}
     for pt in grid:
         if pt in train_res:
             values[pt], source[pt] = train_res[pt].ead, "train"
-            continue
-        d = np.sum(((tkey - [pt.ku2, pt.ikr, pt.cl / 1000.0]) / scale) ** 2, axis=1)
-        recs = [row for j in np.argsort(d)[:n_near] for row in prev[tp[int(j)]]]
-        inc, spr = sur.predict_point(pt.ku2, pt.ikr, pt.cl, recs, ncru)
-        predicted[pt] = (inc, spr)
-        if band[0] <= inc <= band[1] or spr > spread_max:
-            todo.append(pt)
-        else:
-            values[pt], source[pt] = inc, "surrogate"
-    sur.close()
-    if todo:
-        res, c = sim(todo)
+        elif pt not in todo:
+            values[pt], source[pt] = predicted[pt][0], "surrogate"
+    new = [pt for pt in todo if pt not in train_res]
+    top = [pt for pt in todo if pt in train_res]
+    if new:
+        res, c = sim(new, 0, replicas)
         cost.add(c)
-        for pt in todo:
+        for pt in new:
             values[pt], source[pt] = res[pt].ead, "plain"
+    if top:
+        res, c = sim(top, tr, replicas)
+        cost.add(c)
+        for pt in top:
+            full = summarize(PointResult(pt, list(train_res[pt].beats) + list(res[pt].beats)), ncru)
+            values[pt], source[pt] = full.ead, "train+plain"
     return SteeredMap(values, source, cost, predicted)
 
 
@@ -258,13 +369,13 @@ def holdout_summary(grid, sm: SteeredMap, hold: dict, hold_cost: Cost, replicas:
     GPU time against the plain map's.  The plain map is not run: its GPU time
     is the hold-out's GPU time per cell-step times the plain map's cell-steps
     (the hold-out runs in batches of the same kind).  Points the steered map
-    simulated itself carry the same seeds, so their values must equal the
-    hold-out's exactly; `resimulated_equal` reports it."""
+    simulated with all replicas carry the same seeds, so their values must
+    equal the hold-out's exactly; `resimulated_equal` reports it."""
     plain_cells = sum(point_cost(p, replicas, nbeats) for p in grid)
     plain_gpu_s = hold_cost.gpu_s / hold_cost.cell_steps * plain_cells
     err = np.array([abs(sm.values[p] - hold[p].ead) for p in hold])
     sur = np.array([sm.source[p] == "surrogate" for p in hold])
-    same = [sm.values[p] == hold[p].ead for p in hold if sm.source[p] != "surrogate"]
+    same = [sm.values[p] == hold[p].ead for p in hold if sm.source[p] in ("plain", "train+plain")]
     return dict(holdout=len(hold), holdout_surrogate=int(sur.sum()),
                 map_err_mean=float(err.mean()), map_err_max=float(err.max()),
                 map_err_surrogate_mean=float(err[sur].mean()) if sur.any() else 0.0,
@@ -292,6 +403,13 @@ def main(argv=None):
     ap.add_argument("--holdout", type=int, default=200, help="hold-out size of --mode holdout")
     ap.add_argument("--train-every", default="2",
                     help="training subset: every k-th value on each axis (one k, or k_ku2,k_ikr,k_cl)")
+    ap.add_argument("--train-replicas", type=int, default=None,
+                    help="replicas of a training point before steering (default: all)")
+    ap.add_argument("--max-cells", type=int, default=4096, help="cells per batched handle")
+    ap.add_argument("--store", default=None, help="JSON file of per-replica results; a paused run resumes from it")
+    ap.add_argument("--budget-s", type=float, default=None,
+                    help="start no batch that would end later than this many seconds after the start; the run "
+                         "then pauses (exit status 3) and a later run with the same arguments continues")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--out", default=None)
     args = ap.parse_args(argv)
@@ -299,49 +417,63 @@ def main(argv=None):
     ax = [[float(v) for v in s.split(",")] for s in (args.ku2, args.ikr, args.cl)]
     grid = grid_points(*ax)
     log = lambda s: print(s, flush=True)
+    store = ReplicaStore(args.store)
+    deadline = time.time() + args.budget_s if args.budget_s else None
+    run = dict(seed0=args.seed, progress=log, store=store, deadline=deadline, max_cells=args.max_cells)
     report = {"lattice": lat, "replicas": args.replicas, "nbeats": args.nbeats, "grid": len(grid)}
-    if args.mode == "plain":
-        res, cost = simulate_points(grid, args.replicas, args.nbeats, lat, seed0=args.seed, progress=log)
-        report["points"] = [dict(ku2=p.ku2, ikr=p.ikr, cl=p.cl, ead=r.ead, ead_lit=r.ead_lit, ead_lo=r.ead_lo,
-                                 ead_hi=r.ead_hi,
-                                 apd90=r.apd90, apd90_ci=r.apd90_ci) for p, r in res.items()]
-        report["cost"] = cost.__dict__
-    else:
-        ks = [int(v) for v in args.train_every.split(",")]
-        ks = ks * 3 if len(ks) == 1 else ks
-        sub = [a[::k] + ([a[-1]] if (len(a) - 1) % k else []) for a, k in zip(ax, ks)]
-        train = grid_points(*sub)
-        sm = steered_map(grid, train, args.replicas, args.nbeats, lat, seed0=args.seed, progress=log)
-        n_sur = sum(1 for s in sm.source.values() if s == "surrogate")
-        plain_cells = sum(point_cost(p, args.replicas, args.nbeats) for p in grid)
-        report.update(steered_cost=sm.cost.__dict__, plain_cell_steps=plain_cells,
-                      cell_step_ratio=sm.cost.cell_steps / plain_cells, n_surrogate=n_sur,
-                      n_plain=sum(1 for s in sm.source.values() if s == "plain"), n_train=len(train))
-        if args.mode == "holdout":
-            hp = holdout_points(grid, args.holdout, args.seed)
-            hold, hcost = simulate_points(hp, args.replicas, args.nbeats, lat, seed0=args.seed, progress=log)
-            report.update(holdout_cost=hcost.__dict__,
-                          **holdout_summary(grid, sm, hold, hcost, args.replicas, args.nbeats),
-                          points=[dict(ku2=p.ku2, ikr=p.ikr, cl=p.cl, steered=sm.values[p], source=sm.source[p],
-                                       predicted=sm.predicted.get(p, (None, None))[0],
-                                       plain=hold[p].ead if p in hold else None) for p in grid])
-        if args.mode == "compare":
-            # the plain method on every grid point (same seeds), timed the same way
-            res, cost = simulate_points(grid, args.replicas, args.nbeats, lat, seed0=args.seed, progress=log)
-            err = np.array([abs(sm.values[p] - res[p].ead) for p in grid])
-            sur = np.array([sm.source[p] == "surrogate" for p in grid])
-            report.update(plain_cost=cost.__dict__, map_err_max=float(err.max()), map_err_mean=float(err.mean()),
-                          map_err_surrogate_max=float(err[sur].max()) if sur.any() else 0.0,
-                          map_err_surrogate_mean=float(err[sur].mean()) if sur.any() else 0.0,
-                          gpu_time_ratio=(sm.cost.gpu_s + sm.cost.train_s) / cost.gpu_s,
-                          points=[dict(ku2=p.ku2, ikr=p.ikr, cl=p.cl, plain=res[p].ead, plain_lo=res[p].ead_lo,
-                                       plain_hi=res[p].ead_hi, steered=sm.values[p], source=sm.source[p],
-                                       apd90=res[p].apd90) for p in grid])
+    try:
+        if args.mode == "plain":
+            res, cost = simulate_points(grid, args.replicas, args.nbeats, lat, tag="plain", **run)
+            report["points"] = [dict(ku2=p.ku2, ikr=p.ikr, cl=p.cl, ead=r.ead, ead_lit=r.ead_lit, ead_lo=r.ead_lo,
+                                     ead_hi=r.ead_hi,
+                                     apd90=r.apd90, apd90_ci=r.apd90_ci) for p, r in res.items()]
+            report["cost"] = cost.__dict__
+        else:
+            ks = [int(v) for v in args.train_every.split(",")]
+            ks = ks * 3 if len(ks) == 1 else ks
+            sub = [a[::k] + ([a[-1]] if (len(a) - 1) % k else []) for a, k in zip(ax, ks)]
+            train = grid_points(*sub)
+            sim = lambda pts, r0, r1: simulate_points(pts, r1, args.nbeats, lat, r0=r0, tag="steered", **run)
+            sm = steered_map(grid, train, args.replicas, args.nbeats, lat, seed0=args.seed, progress=log,
+                             train_replicas=args.train_replicas, store=store, simulate=sim)
+            n_sur = sum(1 for s in sm.source.values() if s == "surrogate")
+            plain_cells = sum(point_cost(p, args.replicas, args.nbeats) for p in grid)
+            report.update(steered_cost=sm.cost.__dict__, plain_cell_steps=plain_cells,
+                          cell_step_ratio=sm.cost.cell_steps / plain_cells, n_surrogate=n_sur,
+                          n_plain=sum(1 for s in sm.source.values() if s == "plain"),
+                          n_train_topped_up=sum(1 for s in sm.source.values() if s == "train+plain"),
+                          n_train=len(train), train_replicas=args.train_replicas or args.replicas)
+            if args.mode == "holdout":
+                hp = holdout_points(grid, args.holdout, args.seed)
+                hold, hcost = simulate_points(hp, args.replicas, args.nbeats, lat, tag="holdout", **run)
+                report.update(holdout_cost=hcost.__dict__,
+                              **holdout_summary(grid, sm, hold, hcost, args.replicas, args.nbeats),
+                              points=[dict(ku2=p.ku2, ikr=p.ikr, cl=p.cl, steered=sm.values[p], source=sm.source[p],
+                                           predicted=sm.predicted.get(p, (None, None))[0],
+                                           plain=hold[p].ead if p in hold else None,
+                                           plain_lit=hold[p].ead_lit if p in hold else None) for p in grid])
+            if args.mode == "compare":
+                # the plain method on every grid point (same seeds), timed the same way
+                res, cost = simulate_points(grid, args.replicas, args.nbeats, lat, tag="plain", **run)
+                err = np.array([abs(sm.values[p] - res[p].ead) for p in grid])
+                sur = np.array([sm.source[p] == "surrogate" for p in grid])
+                report.update(plain_cost=cost.__dict__, map_err_max=float(err.max()), map_err_mean=float(err.mean()),
+                              map_err_surrogate_max=float(err[sur].max()) if sur.any() else 0.0,
+                              map_err_surrogate_mean=float(err[sur].mean()) if sur.any() else 0.0,
+                              gpu_time_ratio=(sm.cost.gpu_s + sm.cost.train_s) / cost.gpu_s,
+                              points=[dict(ku2=p.ku2, ikr=p.ikr, cl=p.cl, plain=res[p].ead, plain_lo=res[p].ead_lo,
+                                           plain_hi=res[p].ead_hi, plain_lit=res[p].ead_lit, steered=sm.values[p],
+                                           source=sm.source[p], apd90=res[p].apd90) for p in grid])
+    except SweepPaused as e:
+        store.save()
+        print(json.dumps({"paused": str(e), "store": args.store}), flush=True)
+        return 3
     print(json.dumps(report, indent=1), flush=True)
     if args.out:
         with open(args.out, "w") as f:
             json.dump(report, f, indent=1)
+    return 0
 
 
 if __name__ == "__main__":
-    main()
+    sys.exit(main())

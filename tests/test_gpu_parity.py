@@ -149,7 +149,7 @@ def test_config0_three_beats_from_baseline_ics_match_the_oracle():
     assert nrow == 1200
 
 
-@pytest.mark.parametrize("shape", [None, (256, 256, 4)], ids=["tuned", "one-launch"])
+@pytest.mark.parametrize("shape", [None, (256, 256, 4), (256, 512, 4)], ids=["tuned", "one-launch", "one-launch-2"])
 def test_fp64_bitexact_on_a_cell_of_more_than_256_chunks(shape):
     """A 160x20x21 cell (67,200 CRUs, 263 chunks): the whole-cell sums and
     the output-row sums run their loops over groups of 256 chunks more than
@@ -201,7 +201,7 @@ def test_gating_probe_bitexact_including_gaussian_branch():
                 assert np.all(g[2][many] >= 4 + 3 + 7)
 
 
-@pytest.mark.parametrize("shape", [None, (256, 256, 4)], ids=["tuned", "one-launch"])
+@pytest.mark.parametrize("shape", [None, (256, 256, 4), (256, 512, 4)], ids=["tuned", "one-launch", "one-launch-2"])
 def test_batch_equals_cells_run_alone(shape):
     p = Params.default(stim_offset=2.0)
     cells = [CellDesc.default(nx=9, ny=4, nz=3, cl=1000.0, seed=1, Ku2=1.0),
@@ -343,7 +343,7 @@ def test_beats_and_clamp_match_the_oracle():
 
 @pytest.mark.parametrize("shape", [(64, 64, 1), (128, 256, 0), (256, 128, 1), (256, 256, 0), (64, 128, 2),
                                    (256, 64, 2), (256, 512, 1), (128, 512, 0), (64, 512, 2), (256, 256, 3),
-                                   (64, 128, 3), (128, 512, 3), (256, 256, 4)])
+                                   (64, 128, 3), (128, 512, 3), (256, 256, 4), (256, 512, 4)])
 def test_every_launch_shape_is_bitexact(shape):
     """Workgroup sizes and step schedules change no value: FP64 against the
     oracle, and FP32 against the FP32 run at another shape (whole-cell sums

@@ -5,8 +5,10 @@ import math
 import numpy as np
 import pytest
 
-from ryr_hyperactivity_amd.sweep import (Cost, Point, PointResult, SteeredMap, grid_points, holdout_points,
-                                         holdout_summary, plan_batches, point_cost, point_seed, summarize, wilson)
+from ryr_hyperactivity_amd import sweep as sw
+from ryr_hyperactivity_amd.sweep import (Cost, Point, PointResult, ReplicaStore, SteeredMap, SweepPaused, grid_points,
+                                         holdout_points, holdout_summary, plan_batches, point_cost, point_seed,
+                                         simulate_points, steered_map, summarize, wilson)
 
 
 def test_wilson_interval():
@@ -79,6 +81,106 @@ def test_holdout_draw_and_summary():
     assert not holdout_summary(grid, sm, hold, Cost(cell_steps=500, gpu_s=2.0), 4, 3)["resimulated_equal"]
 
 
+def _beats(rng, p, nb):
+    """Synthetic completed-beat rows with EAD probability p per beat."""
+    rows = np.zeros((nb, 12))
+    rows[:, 1] = 300.0
+    rows[:, 4] = (rng.random(nb) < p).astype(float)
+    rows[:, 9] = 1.0
+    rows[:, 10] = rows[:, 4]
+    return rows
+
+
+def test_replica_store_resumes_and_pauses(tmp_path):
+    """Replicas in the store are taken with their recorded cost and the same
+    rows (JSON keeps every bit); a missing replica whose batch would end past
+    the deadline pauses the run before any engine is made."""
+    path = str(tmp_path / "store.json")
+    st = ReplicaStore(path)
+    pts = [Point(1.0, 0.0, 1000.0), Point(2.0, 0.5, 1000.0)]
+    rng = np.random.default_rng(0)
+    rows = {}
+    for pt in pts:
+        for r in range(3):
+            rows[pt, r] = _beats(rng, 0.5, 4) + rng.random((4, 12)) * 1e-9
+            st.put(st.key("t", pt, r, 4, (5, 4, 3), 1, 0), rows[pt, r], 100, 0.25)
+    st.save()
+    again = ReplicaStore(path)
+    res, cost = simulate_points(pts, 3, 4, (5, 4, 3), precision=0, store=again, tag="t")
+    assert cost.cell_steps == 600 and cost.gpu_s == pytest.approx(1.5)
+    for pt in pts:
+        assert all(np.array_equal(res[pt].beats[r], rows[pt, r]) for r in range(3))
+        assert res[pt].ead == summarize(PointResult(pt, [rows[pt, r] for r in range(3)]), 60).ead
+    res, cost = simulate_points(pts, 3, 4, (5, 4, 3), precision=0, store=again, tag="t", r0=1)
+    assert cost.cell_steps == 400 and len(res[pts[0]].beats) == 2
+    with pytest.raises(SweepPaused):
+        simulate_points(pts, 4, 4, (5, 4, 3), precision=0, store=again, tag="t", deadline=0.0)
+    with pytest.raises(SweepPaused):        # another tag shares nothing
+        simulate_points(pts, 3, 4, (5, 4, 3), precision=0, store=again, tag="u", deadline=0.0)
+
+
+class _StubSurrogate:
+    """Stands in for the GPU networks: incidence as a fixed function of the
+    point, spread 0."""
+
+    def __init__(self, *a, **k):
+        pass
+
+    def fit(self, *a, **k):
+        return []
+
+    def predict_point(self, ku2, ikr, cl, recs, ncru):
+        return _truth(ku2, ikr), 0.0
+
+    def close(self):
+        pass
+
+
+def _truth(ku2, ikr):
+    return float(np.clip(0.9 - 2.0 * ikr + 0.05 * ku2, 0.0, 1.0))
+
+
+def test_steered_map_tops_up_uncertain_training_points(monkeypatch):
+    """Training points run on half the replicas; a training point whose own
+    incidence or prediction is uncertain gets the other half, and its value
+    is then the plain method's exactly; an uncertain new point runs every
+    replica; the rest take the prediction.  Cost counts the replicas run."""
+    import ryr_hyperactivity_amd.surrogate as surmod
+    monkeypatch.setattr(surmod, "BeatSurrogate", _StubSurrogate)
+    R, NB, lat = 8, 5, (5, 4, 3)
+    grid = grid_points([1.0, 2.0, 3.0], [0.0, 0.1, 0.2, 0.3, 0.4, 0.5, 0.6], [1000.0])
+    train = grid_points([1.0, 3.0], [0.0, 0.2, 0.4, 0.6], [1000.0])
+    rng = np.random.default_rng(3)
+    plain = {p: [_beats(rng, _truth(p.ku2, p.ikr), NB) for _ in range(R)] for p in grid}
+    calls = []
+
+    def sim(pts, r0, r1):
+        calls.append((sorted(pts, key=lambda q: (q.ku2, q.ikr)), r0, r1))
+        return ({p: summarize(PointResult(p, plain[p][r0:r1]), 60) for p in pts},
+                Cost(cell_steps=sum(point_cost(p, r1 - r0, NB) for p in pts)))
+
+    sm = steered_map(grid, train, R, NB, lat, simulate=sim, train_replicas=R // 2)
+    full = {p: summarize(PointResult(p, plain[p]), 60).ead for p in grid}
+    half = {p: summarize(PointResult(p, plain[p][:R // 2]), 60).ead for p in grid}
+    assert calls[0][1:] == (0, R // 2) and len(calls[0][0]) == len(train)
+    for p in grid:
+        inc = _truth(p.ku2, p.ikr)
+        unsure = 0.1 <= inc <= 0.9
+        if p in train:
+            unsure = unsure or 0.1 <= half[p] <= 0.9
+            assert sm.source[p] == ("train+plain" if unsure else "train")
+            assert sm.values[p] == (full[p] if unsure else half[p])
+        else:
+            assert sm.source[p] == ("plain" if unsure else "surrogate")
+            assert sm.values[p] == (full[p] if unsure else inc)
+    top = [c for c in calls if c[1] == R // 2]
+    assert len(top) == 1 and all(p in train for p in top[0][0])
+    n_new = sum(1 for s in sm.source.values() if s == "plain")
+    n_top = sum(1 for s in sm.source.values() if s == "train+plain")
+    per = point_cost(grid[0], 1, NB)
+    assert sm.cost.cell_steps == per * (len(train) * R // 2 + n_top * R // 2 + n_new * R)
+
+
 @pytest.mark.gpu
 def test_steered_map_spends_less_than_the_plain_map():
     """The learned part against the method without it, on a small grid: the
@@ -96,9 +198,10 @@ def test_steered_map_spends_less_than_the_plain_map():
     train = grid_points([1.0, 4.0], ikr[::2], [1000.0])
     calls = []
 
-    def cached(pts):
+    def cached(pts, r0, r1):
         calls.append(len(pts))
-        return {p: plain[p] for p in pts}, Cost(cell_steps=sum(point_cost(p, R, NB) for p in pts))
+        return ({p: summarize(PointResult(p, plain[p].beats[r0:r1]), 16 * 6 * 4) for p in pts},
+                Cost(cell_steps=sum(point_cost(p, r1 - r0, NB) for p in pts)))
 
     sm = steered_map(grid, train, R, NB, lat, simulate=cached, train_steps=2000)
     srcs = [sm.source[p] for p in grid]

@@ -133,8 +133,8 @@ VARIANT_CASES = [
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("shape", [None, (256, 256, 3), (128, 64, 1), (256, 256, 4)],
-                         ids=["tuned", "gating-fused", "fused", "one-launch"])
+@pytest.mark.parametrize("shape", [None, (256, 256, 3), (128, 64, 1), (256, 256, 4), (256, 512, 4)],
+                         ids=["tuned", "gating-fused", "fused", "one-launch", "one-launch-2"])
 @pytest.mark.parametrize("name,pk,dk", VARIANT_CASES, ids=[c[0] for c in VARIANT_CASES])
 def test_variant_fp64_bitexact_against_the_oracle(name, pk, dk, shape):
     """Every variant, FP64 against the oracle, at the tuned shape and forced
