@@ -128,13 +128,18 @@ struct RyrPend {
     double ms[RYR_PEND_MS];
 };
 
-/* One per cell: V of step `step` and its V-only factors (RyrVF order), which
+/* One per cell: V of a step and its 13 V-only factors (RyrVF order), which
  * the membrane workgroup of the one-launch schedule publishes for the CRU
- * workgroups of the same launch.  `step` is written last, with release. */
+ * workgroups of the same launch.  The 14 doubles are held as 28 words, each
+ * (tag << 32) | one 32-bit half, with tag the step's low 32 bits, written and
+ * read with agent-scope 64-bit atomics.  A 64-bit access is single-copy
+ * atomic, so a word shows either the old step's half with the old tag or the
+ * new one's with the new tag: a reader that finds all 28 tags equal to its
+ * step holds that step's values, and the writer needs no ordering between
+ * its stores (waiting for them to complete, or a release, took about 2 us). */
+#define RYR_VBOX_WORDS 28
 struct RyrVBox {
-    double v, f[13];
-    long long step;
-    long long pad_[1];
+    unsigned long long w[32];
 };
 
 /* The V-only factors of the step that the CRU kernel reads (copied from
@@ -1206,6 +1211,8 @@ template <typename TF, typename TS, int WG>
 RYR_DEV void membrane_block(const RyrKArgs &a, int c, long long nstep, double stim_amp);
 RYR_DEV void cru_v_next_get(const RyrKArgs &a, const RyrCellInfo &C, long long nstep, double stim_amp,
                             double *v_out, RyrVF *vf_out);
+RYR_DEV void cru_v_next_wave(const RyrKArgs &a, const RyrCellInfo &C, long long nstep, double stim_amp,
+                             double *v_out, RyrVF *vf_out);
 #define RYR_GATE_LPC 4   /* lanes per CRU in the gating part: each takes two transitions */
 template <typename TF, typename TS, int WG>
 RYR_DEV void gating_block(const RyrKArgs &a, int L, long long nstep);
@@ -2162,16 +2169,18 @@ RYR_DEV void membrane_block(const RyrKArgs &a, int c, long long nstep, double st
 
 /* The three whole-cell sums of step nstep (I_CaL, I_NCX, cs) over the cell's
  * chunk partials: the tree and order of membrane_block.  Every thread of the
- * workgroup calls it; each gets the sums. */
-template <int NW>
+ * workgroup calls it; each gets the sums.  The 64-chunk slices go to waves
+ * W0..NW-1 (a wave below W0 only meets the barriers, so it may arrive late). */
+template <int NW, int W0 = 0>
 RYR_DEV void reduce_cell_sums(const RyrKArgs &a, const RyrCellInfo &C, long long nstep, RyrDD acc[3])
 {
+    static_assert(W0 >= 0 && W0 < NW, "at least one wave forms the slices");
     __shared__ RyrDD s_red2[3][4];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const RyrDD *pp = step_partials(a, nstep) + 12LL * C.first_block;
     for (int q = 0; q < 3; ++q) { acc[q].hi = 0.0; acc[q].lo = 0.0; }
     for (int g0 = 0; g0 < C.nblocks; g0 += 256) {
-        for (int sl = wv; sl < 4; sl += NW) {
+        for (int sl = wv - W0; sl >= 0 && sl < 4; sl += NW - W0) {
             const int bb = g0 + sl * 64 + lane;
             if (g0 + sl * 64 >= C.nblocks) {
                 if (lane == 0)
@@ -2254,9 +2263,12 @@ RYR_DEV void membrane_first_block(const RyrKArgs &a, int c, long long nstep)
     const RyrCellInfo C = a.cells[c];
     __shared__ double s_st[13];
     if (threadIdx.x < 13) s_st[threadIdx.x] = a.mem[c].m[threadIdx.x];
-    /* no V is published within this launch: a value left by an earlier run
-     * must not be taken for this run's */
-    if (threadIdx.x == 0) __hip_atomic_store(&a.vbox[c].step, -1LL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    /* no V is published within this launch, and a value left by an earlier
+     * run must not be taken for this run's: every tag becomes this step's,
+     * which no later launch of the run waits for */
+    if (threadIdx.x < RYR_VBOX_WORDS)
+        __hip_atomic_store(&a.vbox[c].w[threadIdx.x], (unsigned long long)(unsigned)nstep << 32, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     pend_groups<NW>(C, s_st, a.dt, a.pend + 2LL * c + (nstep & 1));
 }
@@ -2274,27 +2286,39 @@ RYR_DEV void membrane_next_block(const RyrKArgs &a, int c, long long nstep, doub
     const RyrPend *pd = a.pend + 2LL * c + (np & 1);
     const double dt = a.dt;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    /* the pending current groups, loaded while the sums form */
+    const double pm = threadIdx.x < RYR_MS_N ? pd->ms[threadIdx.x] : 0.0;
+    if (wv == 0) {
+        /* V of step nstep and its V-only factors first, in wave 0 alone (the
+         * tree and expressions of the block's own sums and ODE below, so the
+         * same bits), published for this launch's CRU workgroups, which wait
+         * for them after their RyR gating.  Waves 1-3 form the block's sums
+         * meanwhile. */
+        double vp;
+        RyrVF f;
+        cru_v_next_wave(a, C, nstep, stim_amp, &vp, &f);
+        const double val[14] = {vp, f.alphad, f.betad, f.alphaf, f.betaf, f.ghk_A, f.ghk_e2, f.ghk_D, f.ncx_n1,
+                                f.ncx_n2, f.ncx_t1, f.ncx_t3a, f.ncx_den2, (double)f.ghk_small};
+        if (lane < RYR_VBOX_WORDS) {
+            double x = val[0];
+#pragma unroll
+            for (int k = 1; k < 14; ++k)
+                if ((lane >> 1) == k) x = val[k];
+            const unsigned long long bits = (unsigned long long)__double_as_longlong(x);
+            const unsigned half = (lane & 1) ? (unsigned)(bits >> 32) : (unsigned)bits;
+            __hip_atomic_store(&a.vbox[c].w[lane], ((unsigned long long)(unsigned)nstep << 32) | half,
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
     RyrDD acc[3];
-    reduce_cell_sums<NW>(a, C, np, acc);
+    reduce_cell_sums<NW, 1>(a, C, np, acc);
     __shared__ double s_ms[RYR_MS_N];
     __shared__ double s_st[13];
-    if (threadIdx.x < RYR_MS_N) s_ms[threadIdx.x] = pd->ms[threadIdx.x];
+    if (threadIdx.x < RYR_MS_N) s_ms[threadIdx.x] = pm;
     __syncthreads();
     if (wv == 0) {
         double vnew, nai;
         pend_finish(C, pd, acc, np, dt, stim_amp, lane, s_ms, &vnew, &nai);
-        /* publish V of step nstep and its V-only factors for this launch's CRU
-         * workgroups first: they wait for it after their RyR gating */
-        const RyrVF f = vfactors_regs(a.iso2, lane, vnew, nai);
-        if (lane == 0) {
-            RyrVBox *bx = a.vbox + c;
-            const double fv[13] = {f.alphad, f.betad, f.alphaf, f.betaf, f.ghk_A, f.ghk_e2, f.ghk_D, f.ncx_n1,
-                                   f.ncx_n2, f.ncx_t1, f.ncx_t3a, f.ncx_den2, (double)f.ghk_small};
-            __hip_atomic_store(&bx->v, vnew, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-#pragma unroll
-            for (int q = 0; q < 13; ++q) __hip_atomic_store(&bx->f[q], fv[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&bx->step, nstep, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        }
         if (lane == 0) {
             double mo[13];
             mo[0] = vnew;
@@ -2332,6 +2356,13 @@ RYR_DEV void cru_v_next_wave(const RyrKArgs &a, const RyrCellInfo &C, long long 
     const long long np = nstep - 1;
     const RyrPend *pd = a.pend + 2LL * C.cell + (np & 1);
     const int lane = threadIdx.x & 63;
+    /* the pending state and current groups, loaded with the partial sums */
+    double ps[13];
+#pragma unroll
+    for (int k = 0; k < 13; ++k) ps[k] = pd->s[k];
+    const double m_ina = pd->ms[RYR_MS_INA], m_ik1 = pd->ms[RYR_MS_IK1], m_ikr = pd->ms[RYR_MS_IKR],
+                 m_iks = pd->ms[RYR_MS_IKS], m_itos = pd->ms[RYR_MS_ITOS], m_itof = pd->ms[RYR_MS_ITOF],
+                 m_inak = pd->ms[RYR_MS_INAK];
     const RyrDD *pp = step_partials(a, np) + 12LL * C.first_block;
     RyrDD acc[2] = {{0.0, 0.0}, {0.0, 0.0}};
     for (int g0 = 0; g0 < C.nblocks; g0 += 256) {
@@ -2362,14 +2393,14 @@ RYR_DEV void cru_v_next_wave(const RyrKArgs &a, const RyrCellInfo &C, long long 
     double sh[RYR_MS_N];
 #pragma unroll
     for (int k = 0; k < RYR_MS_N; ++k) sh[k] = 0.0;
-    sh[RYR_MS_INA] = pd->ms[RYR_MS_INA]; sh[RYR_MS_IK1] = pd->ms[RYR_MS_IK1];
-    sh[RYR_MS_IKR] = pd->ms[RYR_MS_IKR]; sh[RYR_MS_IKS] = pd->ms[RYR_MS_IKS];
-    sh[RYR_MS_ITOS] = pd->ms[RYR_MS_ITOS]; sh[RYR_MS_ITOF] = pd->ms[RYR_MS_ITOF];
-    sh[RYR_MS_INAK] = pd->ms[RYR_MS_INAK];
+    sh[RYR_MS_INA] = m_ina; sh[RYR_MS_IK1] = m_ik1;
+    sh[RYR_MS_IKR] = m_ikr; sh[RYR_MS_IKS] = m_iks;
+    sh[RYR_MS_ITOS] = m_itos; sh[RYR_MS_ITOF] = m_itof;
+    sh[RYR_MS_INAK] = m_inak;
     sh[RYR_MS_ICAT] = lanev(q1, 1) * 0.0965 * (0.00126) * 2.0 * ((0.5) + 1.);
     sh[RYR_MS_INACA] = lanev(q1, 2) * 0.0965 * (0.025);
     double v, nai;
-    mem_finish(pd->s, sh, a.dt, stim_of(C, np, stim_amp), &v, &nai);
+    mem_finish(ps, sh, a.dt, stim_of(C, np, stim_amp), &v, &nai);
     if (C.vclamp_n > 0) {
         long long q = C.vclamp_periodic ? np % C.vclamp_n : (np < C.vclamp_n ? np : C.vclamp_n - 1);
         v = C.vclamp[q];
@@ -2379,32 +2410,41 @@ RYR_DEV void cru_v_next_wave(const RyrKArgs &a, const RyrCellInfo &C, long long 
 }
 
 /* V of step nstep for a CRU wave of a later launch (every lane active): the
- * membrane workgroup's published values once it has released them, read
- * past the L2 with agent-scope loads; if they do not appear within a bounded
- * number of polls the wave forms them itself.  Either way the bits are the
- * same. */
+ * membrane workgroup's published values once all 28 mailbox words carry this
+ * step's tag (agent-scope loads, so the L2 holds no stale copy); if they do
+ * not within a bounded number of polls, the wave forms them itself.  Either
+ * way the bits are the same. */
 #define RYR_VBOX_POLLS 4096
 RYR_DEV void cru_v_next_get(const RyrKArgs &a, const RyrCellInfo &C, long long nstep, double stim_amp,
                             double *v_out, RyrVF *vf_out)
 {
     const RyrVBox *bx = a.vbox + C.cell;
-    long long seen = -1;
+    const int lane = threadIdx.x & 63;
+    const unsigned tag = (unsigned)nstep;
+    /* lane k < 28 polls word k; the wave takes the words when every tag is
+     * this step's */
+    unsigned long long w = (unsigned long long)tag << 32;
+    bool ok = false;
     for (int t = 0; t < RYR_VBOX_POLLS; ++t) {
-        seen = __hip_atomic_load(&bx->step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (seen == nstep) break;
+        if (lane < RYR_VBOX_WORDS) w = __hip_atomic_load(&bx->w[lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok = __all((unsigned)(w >> 32) == tag);
+        if (ok) break;
         __builtin_amdgcn_s_sleep(2);
     }
-    /* the loads below stay after the poll */
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (seen == nstep) {
-        double f[13];
+    if (ok) {
+        const int half = (int)(unsigned)w;
+        double f[14];
 #pragma unroll
-        for (int q = 0; q < 13; ++q) f[q] = __hip_atomic_load(&bx->f[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        *v_out = __hip_atomic_load(&bx->v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        vf_out->alphad = f[0]; vf_out->betad = f[1]; vf_out->alphaf = f[2]; vf_out->betaf = f[3];
-        vf_out->ghk_A = f[4]; vf_out->ghk_e2 = f[5]; vf_out->ghk_D = f[6]; vf_out->ncx_n1 = f[7];
-        vf_out->ncx_n2 = f[8]; vf_out->ncx_t1 = f[9]; vf_out->ncx_t3a = f[10]; vf_out->ncx_den2 = f[11];
-        vf_out->ghk_small = f[12] != 0.0;
+        for (int k = 0; k < 14; ++k) {
+            const unsigned lo = (unsigned)__builtin_amdgcn_readlane(half, 2 * k);
+            const unsigned hi = (unsigned)__builtin_amdgcn_readlane(half, 2 * k + 1);
+            f[k] = __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+        }
+        *v_out = f[0];
+        vf_out->alphad = f[1]; vf_out->betad = f[2]; vf_out->alphaf = f[3]; vf_out->betaf = f[4];
+        vf_out->ghk_A = f[5]; vf_out->ghk_e2 = f[6]; vf_out->ghk_D = f[7]; vf_out->ncx_n1 = f[8];
+        vf_out->ncx_n2 = f[9]; vf_out->ncx_t1 = f[10]; vf_out->ncx_t3a = f[11]; vf_out->ncx_den2 = f[12];
+        vf_out->ghk_small = f[13] != 0.0;
         return;
     }
     cru_v_next_wave(a, C, nstep, stim_amp, v_out, vf_out);

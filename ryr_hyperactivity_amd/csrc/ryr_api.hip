@@ -717,9 +717,9 @@ static void sched_candidates(const ryr_handle *h, std::vector<std::array<int, 3>
     add(256, 512, RYR_SCHED_FUSED_GATING);
     add(h->wg_cyto, 256, RYR_SCHED_FUSED_GATING);
     if (h->p.scheme == RYR_SCHEME_A) {
-        /* one launch per step; its two-lanes-per-CRU form (256, 512) is
-         * slower on every size measured and is left to explicit requests */
+        /* one launch per step, one or two lanes per CRU */
         add(256, 256, RYR_SCHED_ONE);
+        add(256, 512, RYR_SCHED_ONE);
         add(h->wg_cyto, h->wg_cru, RYR_SCHED_CONCURRENT);
         add(128, 64, RYR_SCHED_CONCURRENT);
         add(256, 64, RYR_SCHED_CONCURRENT);
