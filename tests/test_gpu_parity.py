@@ -392,6 +392,9 @@ def test_tuning_and_kernel_profile():
         assert eng.scalars()[0][S["STEP"]] == 325
         for k in ("k_cyto", "k_cru_step"):
             us, n = prof[k]
+            if sc == 4 and k == "k_cru_step":
+                assert n == 0                      # one launch per step, stamped in the sub-voxel slot
+                continue
             assert n == 120 and 0.5 < us < 5e4, (k, us, n)
         # the fused schedules run the membrane step inside the next sub-voxel launch, and on its own only at
         # output steps and at the last step

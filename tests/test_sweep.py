@@ -218,7 +218,7 @@ def test_steered_map_spends_less_than_the_plain_map():
     assert calls[0] == len(train) and srcs.count("train") == len(train)
     assert n_sur >= 3 and ratio < 0.85
     assert np.mean(err) < 0.1 and max(err) < 0.25
-    # every uncertain prediction went to the simulator
+    # every uncertain prediction at a new point went to the simulator (training points ran every replica)
     for p, (inc, spr) in sm.predicted.items():
-        if 0.1 <= inc <= 0.9 or spr > 0.1:
+        if p not in train and (0.1 <= inc <= 0.9 or spr > 0.1):
             assert sm.source[p] == "plain"
