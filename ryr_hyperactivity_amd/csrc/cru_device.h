@@ -305,6 +305,30 @@ RYR_DEV RyrDD wave_tree(RyrDD v)
     return dd_add(dd_add(g0, g1), dd_add(g2, g3));
 }
 
+/* N wave trees level by level, so that their dependent chains interleave;
+ * each value gets exactly wave_tree's operations. */
+template <int N>
+RYR_DEV void wave_trees(RyrDD (&v)[N])
+{
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = dd_dpp_add<0xB1>(v[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = dd_dpp_add<0x4E>(v[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = dd_dpp_add<0x141>(v[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = dd_dpp_add<0x140>(v[i]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        RyrDD g0, g1, g2, g3;
+        g0.hi = lanev(v[i].hi, 0);  g0.lo = lanev(v[i].lo, 0);
+        g1.hi = lanev(v[i].hi, 16); g1.lo = lanev(v[i].lo, 16);
+        g2.hi = lanev(v[i].hi, 32); g2.lo = lanev(v[i].lo, 32);
+        g3.hi = lanev(v[i].hi, 48); g3.lo = lanev(v[i].lo, 48);
+        v[i] = dd_add(dd_add(g0, g1), dd_add(g2, g3));
+    }
+}
+
 /* ------------------------------------------------------------- buffers.c */
 template <typename T>
 RYR_DEV T buf_simple(T b, T c, T dt, T kon, T koff, T B)
@@ -2369,17 +2393,26 @@ RYR_DEV void cru_v_next_wave(const RyrKArgs &a, const RyrCellInfo &C, long long 
         RyrDD s[2][4];
 #pragma unroll
         for (int sl = 0; sl < 4; ++sl) {
+            if (g0 + sl * 64 >= C.nblocks) {
+                s[0][sl].hi = s[0][sl].lo = s[1][sl].hi = s[1][sl].lo = 0.0;
+                continue;
+            }
+            /* both sums' trees side by side, with no branch between them (a
+             * lane past the cell's last chunk reads that chunk and adds
+             * zero), so that their dependent chains interleave */
             const int bb = g0 + sl * 64 + lane;
+            const bool in = bb < C.nblocks;
+            const RyrDD *w = pp + 12LL * (in ? bb : C.nblocks - 1);
+            RyrDD t[2];
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
-                if (g0 + sl * 64 >= C.nblocks) { s[q][sl].hi = 0.0; s[q][sl].lo = 0.0; continue; }
-                RyrDD t;
-                if (bb < C.nblocks) {
-                    const RyrDD *w = pp + 12LL * bb + q;
-                    t = dd_add(dd_add(w[0], w[3]), dd_add(w[6], w[9]));
-                } else { t.hi = 0.0; t.lo = 0.0; }
-                s[q][sl] = wave_tree(t);
+                const RyrDD x = dd_add(dd_add(w[q], w[q + 3]), dd_add(w[q + 6], w[q + 9]));
+                t[q].hi = in ? x.hi : 0.0;
+                t[q].lo = in ? x.lo : 0.0;
             }
+            wave_trees<2>(t);
+            s[0][sl] = t[0];
+            s[1][sl] = t[1];
         }
 #pragma unroll
         for (int q = 0; q < 2; ++q)
