@@ -44,11 +44,11 @@ UNIT_BYTES = {1: 600, 0: 1184}
 
 def stored_bytes_per_cru_step(precision):
     """HBM bytes of one interior CRU-step as this build stores the state:
-    FP32 mode keeps 55 words in float, 18 in double (8 cnsr, 8 slow troponin,
-    cjsr, CSQN-bound) and 3 packed 32-bit gating/flag words; FP64 mode 73
-    doubles and the 3 packed words.  Each read once and written once."""
+    FP32 mode keeps all 73 concentration and buffer words in float (the slow
+    pools rounded stochastically) and 3 packed 32-bit gating/flag words; FP64
+    mode 73 doubles and the 3 packed words.  Each read once and written once."""
     if precision == 1:
-        return 2 * (55 * 4 + 18 * 8 + 3 * 4)
+        return 2 * (73 * 4 + 3 * 4)
     return 2 * (73 * 8 + 3 * 4)
 
 
@@ -57,17 +57,17 @@ def kernel_bytes(precision, sched=0):
     every word counted once per kernel.
     k_cyto (sub-voxel kernel): reads 8 ci, 8 cnsr, 5x8 fast buffers, 8 slow
     troponin, cjsr, cs, cp; writes 8 ci, 8 cnsr, 5x8 fast buffers, 8 slow
-    troponin.  FP32: 336 + 320 = 656 B; FP64: 536 + 512 = 1048 B.
+    troponin.  FP32: 268 + 256 = 524 B; FP64: 536 + 512 = 1048 B.
     k_cru_step (CRU kernel): reads ci/cnsr of sub-voxels 0-1, cs, cp, cjsr,
     cjsrb, Ancx, 4 sarcolemmal buffers, 3 packed words; writes Ancx, the 4
-    buffers, cp, cs, cjsr, cjsrb, the packed words.  FP32: 80 + 56 = 136 B;
+    buffers, cp, cs, cjsr, cjsrb, the packed words.  FP32: 64 + 48 = 112 B;
     FP64: 116 + 84 = 200 B.
     Schedule 3 (RyR gating in the sub-voxel launch): the sub-voxel launch also
     reads and writes the RyR-count and flag words, +16 B; the CRU kernel no
     longer writes the RyR-count word, -4 B.
     Schedule 4 (one launch per step, k_one): the sum of the two."""
     if precision == 1:
-        b = {"k_cyto": 656, "k_cru_step": 136}
+        b = {"k_cyto": 524, "k_cru_step": 112}
     else:
         b = {"k_cyto": 1048, "k_cru_step": 200}
     if sched == 3:

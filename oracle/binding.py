@@ -73,6 +73,9 @@ def lib():
             "orc_dm_pow": (d, [d, d]),
             "orc_philox": (None, [ctypes.POINTER(ctypes.c_uint32)] * 3),
             "orc_slot_uniform": (d, [ctypes.c_uint32, ctypes.c_uint32]),
+            "orc_sr_bits": (ctypes.c_uint32, [u64, u64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]),
+            "orc_mix32": (ctypes.c_uint32, [ctypes.c_uint32]),
+            "orc_sr_f32": (ctypes.c_float, [d, ctypes.c_uint32]),
             "orc_stream": (None, [u64, ctypes.c_uint32, u64, i, _D, ctypes.POINTER(ctypes.c_uint32)]),
             "orc_stencil_probe": (i, [ctypes.POINTER(Params), ctypes.POINTER(CellDesc), _D, d, d, _D]),
         }

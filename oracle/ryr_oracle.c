@@ -50,6 +50,7 @@
 #include "ryr_oracle.h"
 #include "../ryr_hyperactivity_amd/csrc/common/philox.h"
 #include "../ryr_hyperactivity_amd/csrc/common/detmath.h"
+#include "../ryr_hyperactivity_amd/csrc/common/srround.h"
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -1662,6 +1663,14 @@ void orc_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out)
     for (int q = 0; q < 4; ++q) out[q] = o.v[q];
 }
 double orc_slot_uniform(uint32_t a, uint32_t b) { return ryr_slot_uniform(a, b); }
+/* the FP32 production mode's stochastic rounding (the oracle itself is FP64
+ * and does not round; these expose the shared helpers to the tests) */
+uint32_t orc_sr_bits(uint64_t seed, uint64_t step, uint32_t field, uint32_t n, uint32_t ii)
+{
+    return ryr_sr_bits(ryr_sr_key(seed, step, field), n, ii);
+}
+uint32_t orc_mix32(uint32_t x) { return ryr_mix32(x); }
+float orc_sr_f32(double v, uint32_t r) { return ryr_sr_f32(v, r); }
 void orc_stream(uint64_t seed, uint32_t cru, uint64_t step, int n, double *u, uint32_t *w)
 {
     ryr_stream s;

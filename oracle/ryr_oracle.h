@@ -74,6 +74,9 @@ double orc_dm_cos(double x);
 double orc_dm_pow(double x, double y);
 void orc_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out);
 double orc_slot_uniform(uint32_t a, uint32_t b);
+uint32_t orc_sr_bits(uint64_t seed, uint64_t step, uint32_t field, uint32_t n, uint32_t ii);
+uint32_t orc_mix32(uint32_t x);
+float orc_sr_f32(double v, uint32_t r);
 void orc_stream(uint64_t seed, uint32_t cru, uint64_t step, int n, double *u, uint32_t *w);
 int orc_stencil_probe(const ryr_params *p, const ryr_cell_desc *d, const double *field,
                       double tt, double tl, double *out);

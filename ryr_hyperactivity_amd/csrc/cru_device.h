@@ -33,6 +33,7 @@
 #include <stdint.h>
 #include "common/philox.h"
 #include "common/detmath.h"
+#include "common/srround.h"
 
 #define RYR_BLOCK 256
 #define RYR_NRYR 100
@@ -1368,8 +1369,9 @@ RYR_DEV void cyto_block(const RyrKArgs &a, int L, long long nstep)
         if (diag) { a.d_up[e] = xiup; a.d_leak[e] = xileak; }
     } else {
         /* FP32 production path: the same equations with divisions by
-         * constants replaced by reciprocal products; cnsr and slow troponin
-         * keep double storage */
+         * constants replaced by reciprocal products.  cnsr and slow troponin
+         * change by less than a float ulp per step at rest, so their updates
+         * are formed in double and rounded stochastically to float */
         const float dt = (float)a.dt;
         const float sat = 0.95f * __builtin_amdgcn_rcpf(dt);
         const float rIT = (float)(1.0 / ((0.33 * 2.) * 0.7)), rIL = (float)(1.0 / ((0.7 * 2.) * 0.7));
@@ -1406,13 +1408,21 @@ RYR_DEV void cyto_block(const RyrKArgs &a, int L, long long nstep)
             }
         }
         if (ii >= 2 || schemeA) ci1[e] = (TF)fmaxf(cf + dotci * dt, 0.0f);
-        cn1[e] = cn + (TS)(dotcnsr * dt);
+        const double cn_new = (double)cn + (double)dotcnsr * a.dt;
+        if constexpr (sizeof(TS) == 4)
+            cn1[e] = ryr_sr_f32(cn_new, ryr_sr_bits(ryr_sr_key(C.seed, (uint64_t)nstep, RYR_SR_CNSR), (uint32_t)n, (uint32_t)ii));
+        else
+            cn1[e] = cn_new;
         const float n_camyo = fmaxf(v_camyo + f_myo * dt, 0.0f);
         const float on_mg = 0.0157f * 500.0f;
         const float f_mg = (on_mg * dt > 1.0f ? sat : on_mg) * (140.0f - n_camyo - v_mgmyo) - 0.057f * v_mgmyo;
         catf[e] = (TF)fmaxf(v_catf + f_tf * dt, 0.0f);
-        const TS n_cats = v_cats + (TS)(f_ts * dt);
-        cats[e] = n_cats < TS(0) ? TS(0) : n_cats;
+        double n_cats = (double)v_cats + (double)f_ts * a.dt;
+        if (n_cats < 0.0) n_cats = 0.0;
+        if constexpr (sizeof(TS) == 4)
+            cats[e] = ryr_sr_f32(n_cats, ryr_sr_bits(ryr_sr_key(C.seed, (uint64_t)nstep, RYR_SR_CATS), (uint32_t)n, (uint32_t)ii));
+        else
+            cats[e] = n_cats;
         cacal[e] = (TF)fmaxf(v_cacal + f_cal * dt, 0.0f);
         casr[e] = (TF)fmaxf(v_casr + f_sr * dt, 0.0f);
         camyo[e] = (TF)n_camyo;
@@ -1837,6 +1847,7 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L, double stim_amp = 0.0)
             diffjn1 = (cjsrv - cnl[1]) * TS(1.0 / (25.0 * 1.0 / 4.) / 2.);
         }
         TS dotcjsr, cjsrb_new;
+        double dotcjsr_d, cjsrb_new_d;
         {
             RYR_STRICT_FP
             const double cj = (double)cjsrv;
@@ -1846,8 +1857,10 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L, double stim_amp = 0.0)
             double beta = 1.0 / (1.0 + (460.0 * 600.0 * ene + 460.0 * enedot * cj * (cj + 600.0))
                                        / ((600.0 + cj) * (600.0 + cj)));
             double flux = ((-(double)diffjn0 - (double)diffjn1) - xire * (0.00126) * 1.0 / (0.02 * 1));
-            dotcjsr = (TS)(beta * flux);
-            cjsrb_new = (TS)((double)(GSPLIT ? e_cjb : cjsrbp[g]) + (1 - beta) * flux * dt);
+            dotcjsr_d = beta * flux;
+            dotcjsr = (TS)dotcjsr_d;
+            cjsrb_new_d = (double)(GSPLIT ? e_cjb : cjsrbp[g]) + (1 - beta) * flux * dt;
+            cjsrb_new = (TS)cjsrb_new_d;
         }
 
         /* cleft/sub-membrane sub-steps (:1356-1473 / :1475-1597); the
@@ -1967,8 +1980,16 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L, double stim_amp = 0.0)
             casar[g] = v_casar; casarh[g] = v_casarh; casarj[g] = v_casarj; casarhj[g] = v_casarhj;
             cp1[g] = cpv;
             cs1[g] = csn;
-            cjsr1[g] = cjsrv + dotcjsr * (TS)dtT;
-            cjsrbp[g] = cjsrb_new;
+            if constexpr (sizeof(TS) == 4) {
+                /* float JSR pools (FP32 production): stochastic rounding of the double update */
+                cjsr1[g] = ryr_sr_f32((double)cjsrv + dotcjsr_d * dt,
+                                      ryr_sr_bits(ryr_sr_key(C.seed, (uint64_t)nstep, RYR_SR_CJSR), (uint32_t)n, 0u));
+                cjsrbp[g] = ryr_sr_f32(cjsrb_new_d,
+                                       ryr_sr_bits(ryr_sr_key(C.seed, (uint64_t)nstep, RYR_SR_CJSRB), (uint32_t)n, 0u));
+            } else {
+                cjsr1[g] = cjsrv + dotcjsr * (TS)dtT;
+                cjsrbp[g] = cjsrb_new;
+            }
             /* the gating words' bases pass through late_base, so that these
              * stores form their addresses here instead of keeping the loads'
              * 64-bit addresses live across the kernel (which spills in the

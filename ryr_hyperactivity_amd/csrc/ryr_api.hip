@@ -286,7 +286,7 @@ extern "C" int ryr_create(int device, const ryr_params *p, const ryr_cell_desc *
     h->P = (long long)h->nblocks * RYR_BLOCK;
     const long long P = h->P;
     const bool f32 = p->precision == RYR_PREC_F32;
-    const size_t tf = f32 ? 4 : 8, ts = 8;
+    const size_t tf = f32 ? 4 : 8, ts = tf;
 
     RyrKArgs &a = h->a;
     a.P = P;

@@ -9,14 +9,20 @@ realisations of the same stochastic model.  Each test therefore checks:
   * before the first divergence of the channel counts, every output-row
     column within its FP32 error: the relative RMS over the rows within
     PRE_RMS_TOL (ten times the largest figure measured over 16 runs of the
-    100x20x10 cell, Ku2 1 and 2.5, seeds 1-8, 10 beats at CL 2000 ms), and
-    every row within PRE_ROW_TOL of the column's scale;
+    100x20x10 cell, Ku2 1 and 2.5, seeds 1-8, 10 beats at CL 2000 ms, and
+    over the cases of this file), and every row within PRE_ROW_TOL of the
+    column's scale;
   * after it, every whole-cell column within POST_ROW_TOL of its scale (a few
-    CRUs of 10^3-10^4 gate differently; measured at most 1.4e-3);
+    CRUs of 10^3-10^4 gate differently; measured at most 8.7e-5);
   * the state fields at the end: relative RMS within FIELD_TOL_SAME (the
-    gating identical everywhere) or FIELD_TOL_DIVERGED (at most
-    MAX_DIVERGED_FRAC of the CRUs gate differently), each measured case far
-    inside these.
+    channel counts never diverged) or FIELD_TOL_DIVERGED (they diverged at
+    some row, and at most MAX_DIVERGED_FRAC of the CRUs gate differently at
+    the end), each measured case far inside these.
+
+FP32 mode stores the slow pools (cnsr, slow troponin, cjsr, CSQN-bound) as
+floats rounded stochastically from their double updates; the rounding noise
+moves the first divergence earlier (median 41 ms over the 16 runs) but leaves
+the per-beat statistics within each other's 95% intervals.
 """
 import numpy as np
 import pytest
@@ -27,16 +33,17 @@ from ryr_hyperactivity_amd._abi import DIAG_INT_COLS, DIAG_NAMES, VAR_ISO, VAR_P
 pytestmark = pytest.mark.gpu
 
 # relative RMS (over the output rows before the first divergence) of FP32 against FP64, per column: ten times
-# the largest value over the 16 runs of the precision profile; columns not listed: 1e-5
-PRE_RMS_TOL = {"v": 5e-7, "ci": 1.5e-5, "cp": 8.5e-5, "cs": 2.4e-5, "cjsr": 6.3e-7, "cnsr": 8.4e-7,
-               "i_cal": 7.8e-7, "i_ncx": 1.9e-5, "po": 0.0, "nai": 3.1e-9, "catf": 2.1e-6, "cats": 6.1e-8,
-               "cacal": 7.8e-6, "csqn_bound": 3.3e-7, "ancx": 9.9e-6, "total_ca": 6.3e-7}
+# the largest value over the 16 runs of the precision profile and the cases below; columns not listed: 1e-5
+# (largest measured 2.9e-6, jrel)
+PRE_RMS_TOL = {"v": 6.6e-7, "ci": 1.8e-5, "cp": 9.9e-6, "cs": 1.8e-5, "cjsr": 1.5e-6, "cnsr": 6.4e-7,
+               "i_cal": 4.4e-7, "i_ncx": 1.3e-5, "po": 0.0, "nai": 7.4e-10, "catf": 3.0e-6, "cats": 6.9e-7,
+               "cacal": 1.2e-5, "csqn_bound": 1.8e-6, "ancx": 8.3e-6, "total_ca": 6.6e-7}
 PRE_RMS_DEFAULT = 1e-5
-PRE_ROW_TOL = 1e-4          # any single row, relative to the column's RMS over the window
+PRE_ROW_TOL = 1e-4          # any single row, relative to the column's RMS over the window (measured <= 1.8e-5)
 POST_ROW_TOL = 1e-2         # any row after the divergence, relative to the column's RMS
-FIELD_TOL_SAME = 2e-5       # relative RMS of a field when no CRU gates differently (measured <= 3.4e-6)
-FIELD_TOL_DIVERGED = 1e-2   # relative RMS of a field after some CRUs diverged (measured <= 1.6e-3)
-MAX_DIVERGED_FRAC = 5e-3    # CRUs whose gating state differs at the end (measured <= 3.5e-4)
+FIELD_TOL_SAME = 2e-5       # relative RMS of a field when the counts never diverged (measured <= 4.6e-6)
+FIELD_TOL_DIVERGED = 1e-2   # relative RMS of a field after the counts diverged (measured <= 1.7e-4)
+MAX_DIVERGED_FRAC = 5e-3    # CRUs whose gating state differs at the end (measured <= 4e-4)
 COUNT_COLS = [DIAG_NAMES.index(c) for c in ("nou", "nob", "ncu", "ncb", "tn1", "tn2", "tn3", "tn4", "tn5",
                                             "tn6", "tn7")]
 FLOAT_COLS = [j for j in range(1, len(DIAG_NAMES)) if j not in DIAG_INT_COLS]
@@ -80,7 +87,7 @@ def _compare(r32, r64, f32, f64, label):
     differ = (f32["NOU"] != f64["NOU"]) | (f32["NOB"] != f64["NOB"]) | (f32["NCU"] != f64["NCU"]) | \
         np.any(f32["LCC"] != f64["LCC"], axis=0)
     frac = differ.sum() / ncru
-    tol = FIELD_TOL_SAME if frac == 0 else FIELD_TOL_DIVERGED
+    tol = FIELD_TOL_SAME if frac == 0 and d0 == n else FIELD_TOL_DIVERGED
     assert frac <= MAX_DIVERGED_FRAC, f"{label}: {differ.sum()} of {ncru} CRUs gate differently"
     for f in FIELDS_F:
         x, y = f32[f], f64[f]

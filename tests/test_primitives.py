@@ -163,3 +163,81 @@ def test_detmath_pow():
         bound = (abs(1.787 * math.log(x)) + 3) * 2.3e-16 * ref if x > 0 else 0.0
         assert abs(L.orc_dm_pow(x, 1.787) - ref) <= bound
     assert L.orc_dm_pow(0.0, 1.787) == 0.0
+
+
+# ---- stochastic rounding of the FP32 production mode's slow pools ----------
+def _lo_hi(v):
+    """The floats just below/at and just above a double v > 0."""
+    lo = np.float32(v)
+    if float(lo) > v:
+        lo = np.nextafter(lo, np.float32(0))
+    return lo, np.nextafter(lo, np.float32(np.inf))
+
+
+def test_stochastic_rounding_brackets_and_keeps_exact_floats():
+    L = orc.lib()
+    rng = np.random.default_rng(5)
+    rs = [0, 8, 0xFFFFFFF8, 0xFFFFFFFF, 0x80000000] + [int(x) for x in rng.integers(0, 2**32, 20)]
+    for scale in (1e-3, 0.1, 1.0, 118.0, 750.0, 1000.0, 3.0e4):
+        for v in scale * (1.0 + rng.random(20)):
+            lo, hi = _lo_hi(float(v))
+            for r in rs:
+                out = np.float32(L.orc_sr_f32(float(v), r))
+                assert out == lo or out == hi, (v, r, out, lo, hi)
+            f = float(lo)                      # an exact float rounds to itself whatever the bits
+            assert all(np.float32(L.orc_sr_f32(f, r)) == lo for r in rs)
+    assert all(L.orc_sr_f32(0.0, r) == 0.0 for r in rs)
+    # rounding up across a binade boundary lands on the next binade's first float
+    below2 = float(np.nextafter(np.float32(2.0), np.float32(0)))
+    v = below2 + 0.75 * (2.0 - below2)
+    assert {float(L.orc_sr_f32(v, r)) for r in (0, 0xFFFFFFF8)} == {below2, 2.0}
+
+
+@pytest.mark.parametrize("frac", [0.001, 0.25, 0.5, 0.875, 0.999])
+def test_stochastic_rounding_rounds_up_with_the_fractional_probability(frac):
+    """Stratified random words (the 29 bits used step through [0, 2^29) in
+    steps of 2^13): the share rounded up equals the fraction of an ulp the
+    value lies above the lower float, to the stratification's 2^-16."""
+    L = orc.lib()
+    for base in (118.0, 750.0, 0.0944):
+        lo, hi = _lo_hi(base)
+        v = float(lo) + frac * (float(hi) - float(lo))
+        ups = sum(L.orc_sr_f32(v, ((k << 13) | 1) << 3) == float(hi) for k in range(65536))
+        assert abs(ups / 65536 - frac) <= 2.0 / 65536, (base, frac, ups / 65536)
+
+
+def test_stochastic_rounding_accumulates_increments_below_an_ulp():
+    """Slow troponin at rest: 2.5e-6 uM per step on 118 uM (a float ulp there
+    is 7.6e-6).  Rounding to nearest never moves; stochastic rounding with the
+    model's own random bits tracks the exact sum within its random-walk
+    spread (5 sigma)."""
+    L = orc.lib()
+    x, xn, d, n = np.float32(118.0), np.float32(118.0), 2.5e-6, 20000
+    for k in range(n):
+        x = np.float32(L.orc_sr_f32(float(x) + d, L.orc_sr_bits(7, k, 1, 123, 5)))
+        xn = np.float32(float(xn) + d)
+    ulp = float(np.spacing(np.float32(118.0)))
+    p = d / ulp
+    sigma = ulp * math.sqrt(n * p * (1 - p))
+    assert xn == np.float32(118.0)
+    assert abs(float(x) - (118.0 + n * d)) < 5 * sigma, (float(x), 118.0 + n * d, sigma)
+
+
+def test_stochastic_rounding_bits_are_uniform_and_independent():
+    L = orc.lib()
+    # the mixer is a bijection: no two of 2^18 consecutive inputs collide
+    m = {L.orc_mix32(x) for x in range(1 << 18)}
+    assert len(m) == 1 << 18
+    # over the sub-voxels of one key the words are uniform (64-bin chi-square)
+    w = np.array([L.orc_sr_bits(3, 1000, 0, n, ii) for n in range(8192) for ii in range(8)], dtype=np.float64)
+    h = np.histogram(w / 2**32, bins=64, range=(0, 1))[0]
+    e = len(w) / 64
+    chi2 = float(((h - e) ** 2 / e).sum())
+    assert chi2 < 120, chi2                    # 63 dof: p ~ 3e-5 at 120
+    # one sub-voxel over successive steps: no lag-1 correlation
+    s = np.array([L.orc_sr_bits(3, k, 0, 77, 2) for k in range(20000)], dtype=np.float64) / 2**32
+    c = np.corrcoef(s[:-1], s[1:])[0, 1]
+    assert abs(c) < 0.03, c
+    # fields, cells and sub-voxels draw different words
+    a = [L.orc_sr_bits(3, 9, f, 5, 1) for f in range(4)] + [L.orc_sr_bits(4, 9, 0, 5, 1), L.orc_sr_bits(3, 9, 0, 5, 2)]
+    assert len(set(a)) == len(a)
