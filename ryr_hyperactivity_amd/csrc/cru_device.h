@@ -448,49 +448,57 @@ RYR_DEV int lcc_step(int i, double cpx, double dt, double ragg, const MV &m, dou
 }
 
 /* Transition t's count once its first uniform u0 (static slot 8 + t) has
- * failed the early-out: Knuth's product-of-uniforms Poisson count, or the
- * dithered Gaussian with rejection outside Knuth's range
- * (subroutine_cuda.c:332-363 and its seven copies).  The further uniforms
- * come from t's own substream; *nd counts them. */
-RYR_DEV int ryr_transition_job(uint64_t seed, uint32_t cru, uint64_t step, int t, int n, double p, double u0,
-                               int *fails, int *nd)
+ * failed the early-out (subroutine_cuda.c:332-363 and its seven copies):
+ * Knuth's product-of-uniforms Poisson count where ryr_knuth_range holds,
+ * else the dithered Gaussian with rejection.  The further uniforms come from
+ * t's own substream, so the two samplers can run in any order; *nd counts
+ * them. */
+RYR_DEV bool ryr_knuth_range(int n, double p) { return n <= 1 || p < 0.2 || (n <= 5 && p < 0.3); }
+
+RYR_DEV int ryr_knuth_job(uint64_t seed, uint32_t cru, uint64_t step, int t, int n, double p, double u0, int *nd)
+{
+    RYR_STRICT_FP
+    uint32_t j = 0;
+    ryr_u32x4 w = {{0u, 0u, 0u, 0u}};
+    const double lam = ryr_exp(-(n) * p);
+    int kk = 1;
+    double puu = u0;
+    while (puu >= lam && kk < 195) {
+        kk++;
+        if (!(j & 1u)) w = ryr_block(seed, cru, step, ryr_sub_block(t, j));
+        const double re = (j & 1u) ? ryr_slot_uniform(w.v[2], w.v[3]) : ryr_slot_uniform(w.v[0], w.v[1]);
+        ++j;
+        puu = puu * (re);
+    }
+    *nd += (int)j;
+    return kk - 1;
+}
+
+RYR_DEV int ryr_gauss_job(uint64_t seed, uint32_t cru, uint64_t step, int t, int n, double p, double u0, int *fails,
+                          int *nd)
 {
     RYR_STRICT_FP
     int out = -1;
     uint32_t j = 0;
     ryr_u32x4 w = {{0u, 0u, 0u, 0u}};
-    if (n <= 1 || p < 0.2 || (n <= 5 && p < 0.3)) {
-        const double lam = ryr_exp(-(n) * p);
-        int kk = 1;
-        double puu = u0;
-        while (puu >= lam && kk < 195) {
-            kk++;
-            if (!(j & 1u)) w = ryr_block(seed, cru, step, ryr_sub_block(t, j));
-            const double re = (j & 1u) ? ryr_slot_uniform(w.v[2], w.v[3]) : ryr_slot_uniform(w.v[0], w.v[1]);
-            ++j;
-            puu = puu * (re);
-        }
-        out = kk - 1;
-    } else {
-        int kk = 0;
-        while (out < 0) {
-            double u[3];
-            uint32_t r = 0;
+    int kk = 0;
+    while (out < 0) {
+        double u[3];
+        uint32_t r = 0;
 #pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                if (q == 0 && kk == 0) { u[0] = u0; continue; }
-                if (!(j & 1u)) w = ryr_block(seed, cru, step, ryr_sub_block(t, j));
-                const uint32_t wa = (j & 1u) ? w.v[2] : w.v[0], wb = (j & 1u) ? w.v[3] : w.v[1];
-                ++j;
-                u[q] = ryr_slot_uniform(wa, wb);
-                r = wb;
-            }
-            out = (int)(__builtin_floor(n * p + __builtin_sqrt(n * p * (1.0 - p)) *
-                                        __builtin_sqrt(-2.0 * ryr_log(1.0 - u[0])) *
-                                        ryr_cos(2.0 * 3.1415926 * u[1])) + r % 2);
-            kk++;
-            if (kk > 200) { out = 0; (*fails)++; }
+        for (int q = 0; q < 3; ++q) {
+            if (q == 0 && kk == 0) { u[0] = u0; continue; }
+            if (!(j & 1u)) w = ryr_block(seed, cru, step, ryr_sub_block(t, j));
+            const uint32_t wa = (j & 1u) ? w.v[2] : w.v[0], wb = (j & 1u) ? w.v[3] : w.v[1];
+            ++j;
+            u[q] = ryr_slot_uniform(wa, wb);
+            r = wb;
         }
+        out = (int)(__builtin_floor(n * p + __builtin_sqrt(n * p * (1.0 - p)) *
+                                    __builtin_sqrt(-2.0 * ryr_log(1.0 - u[0])) *
+                                    ryr_cos(2.0 * 3.1415926 * u[1])) + r % 2);
+        kk++;
+        if (kk > 200) { out = 0; (*fails)++; }
     }
     *nd += (int)j;
     return out;
@@ -505,6 +513,30 @@ RYR_DEV double csqn_mono(double cjsr, double *roo2_out)
     double roo2 = 5000.0 / (1.0 + ryr_pow(850.0 / (cjsr), 23.0));
     *roo2_out = roo2;
     return (-1.0 + __builtin_sqrt(1.0 + 8.0 * 460.0 * roo2)) / (4.0 * roo2 * 460.0);
+}
+
+/* Per-RyR transition probabilities of one step (subroutine_cuda.c:251-330),
+ * in the order OU->CU, CU->OU, OU->OB, OB->OU, CB->CU, CU->CB, OB->CB,
+ * CB->OB. */
+RYR_DEV void ryr_rates(double Ku, double Ku2, double Kb, double cp, double cjsr, double mono, double dt, double pt[8])
+{
+    RYR_STRICT_FP
+    double q1 = (5000 / cjsr) * (5000 / cjsr);
+    double x2 = Ku / cp;
+    double q2 = x2 * x2;
+    double pku = 4.5 * 1.5 * Ku2 * 1 / (1 + q1) * 1 / (1 + q2);
+    double pkb = 4.5 * 1.5 / 36. * Kb * 1 / (1 + q1) * 1 / (1 + q2);
+    double pkuminus = 1. / 1., pkbminus = 1. / 1.;
+    if (pku * dt > 1.0) pku = 1.0 / dt;
+    if (pkb * dt > 1.0) pkb = 1.0 / dt;
+    if (pkuminus * dt > 1.0) pkuminus = 1.0 / dt;
+    if (pkbminus * dt > 1.0) pkbminus = 1.0 / dt;
+    double aub = mono / (1.0);
+    double abu = 1.0 / (0.2 * 5 * 220.);
+    const double pau = (pkb < 1e-16) ? 0 : aub * dt;
+    const double pbu = (pkb < 1e-16) ? abu * dt * 36 : abu * dt * (pku / pkb);
+    pt[0] = pkuminus * dt; pt[1] = pku * dt; pt[2] = pau; pt[3] = pbu;
+    pt[4] = abu * dt; pt[5] = aub * dt; pt[6] = pkbminus * dt; pt[7] = pkb * dt;
 }
 
 /* The partner lane of a CRU that two lanes share (lane ^ 1, a quad
@@ -542,23 +574,8 @@ RYR_DEV void ryr_gating(uint64_t seed, uint32_t cru, uint64_t step, double Ku, d
                         int *nd, int h = 0)
 {
     RYR_STRICT_FP
-    double q1 = (5000 / cjsr) * (5000 / cjsr);
-    double x2 = Ku / cp;
-    double q2 = x2 * x2;
-    double pku = 4.5 * 1.5 * Ku2 * 1 / (1 + q1) * 1 / (1 + q2);
-    double pkb = 4.5 * 1.5 / 36. * Kb * 1 / (1 + q1) * 1 / (1 + q2);
-    double pkuminus = 1. / 1., pkbminus = 1. / 1.;
-    if (pku * dt > 1.0) pku = 1.0 / dt;
-    if (pkb * dt > 1.0) pkb = 1.0 / dt;
-    if (pkuminus * dt > 1.0) pkuminus = 1.0 / dt;
-    if (pkbminus * dt > 1.0) pkbminus = 1.0 / dt;
-    double aub = mono / (1.0);
-    double abu = 1.0 / (0.2 * 5 * 220.);
-    const double pau = (pkb < 1e-16) ? 0 : aub * dt;
-    const double pbu = (pkb < 1e-16) ? abu * dt * 36 : abu * dt * (pku / pkb);
-
-    /* OU->CU, CU->OU, OU->OB, OB->OU, CB->CU, CU->CB, OB->CB, CB->OB */
-    const double pt[8] = {pkuminus * dt, pku * dt, pau, pbu, abu * dt, aub * dt, pkbminus * dt, pkb * dt};
+    double pt[8];
+    ryr_rates(Ku, Ku2, Kb, cp, cjsr, mono, dt, pt);
     /* source population of transition t: 0 OU, 1 CU, 2 OB, 3 CB */
     const int pop[4] = {nou, ncu, nob, ncb};
     constexpr int src[8] = {0, 1, 0, 2, 3, 1, 2, 3};
@@ -599,6 +616,15 @@ RYR_DEV void ryr_gating(uint64_t seed, uint32_t cru, uint64_t step, double Ku, d
      * below n p + 5 sqrt(-2 log 2^-53) + 1 < 150) */
     uint32_t kw[LPC == 1 ? 2 : 1] = {};
     int ndraw = TPL;
+    /* pending transitions in Knuth's range; the Gaussian ones (n >= 2 and
+     * p >= 0.2, rare at the model's time step) are deferred to a pass after
+     * this loop, which re-forms their rates and first uniforms, so that the
+     * Gaussian sampler's registers do not add to the loop's */
+    uint32_t gpend = 0u;
+#pragma unroll
+    for (int l = 0; l < TPL; ++l)
+        if (((pend >> l) & 1u) && !ryr_knuth_range(nl[l], pl[l])) gpend |= 1u << l;
+    pend &= ~gpend;
     while (pend) {
         const int l = __builtin_ctz(pend);
         pend &= pend - 1u;
@@ -607,8 +633,25 @@ RYR_DEV void ryr_gating(uint64_t seed, uint32_t cru, uint64_t step, double Ku, d
 #pragma unroll
         for (int q = 1; q < TPL; ++q)
             if (l == q) { n = nl[q]; p = pl[q]; u = u0[q]; }
-        const uint32_t k =
-            (uint32_t)ryr_transition_job(seed, cru, step, TPL * h + l, n, p, u, fails, &ndraw) & 255u;
+        const uint32_t k = (uint32_t)ryr_knuth_job(seed, cru, step, TPL * h + l, n, p, u, &ndraw) & 255u;
+        if (LPC == 1 && l >= 4) kw[LPC == 1 ? 1 : 0] |= k << (8 * (l - 4));
+        else kw[0] |= k << (8 * (l & 3));
+    }
+    while (gpend) {
+        const int l = __builtin_ctz(gpend);
+        gpend &= gpend - 1u;
+        const int t = TPL * h + l;
+        double pg[8];
+        ryr_rates(Ku, Ku2, Kb, cp, cjsr, mono, dt, pg);
+        const int popg[4] = {nou, ncu, nob, ncb};
+        int n = popg[src[0]];
+        double p = pg[0];
+#pragma unroll
+        for (int q = 1; q < 8; ++q)
+            if (t == q) { n = popg[src[q]]; p = pg[q]; }
+        const ryr_u32x4 w = ryr_block(seed, cru, step, (uint32_t)(RYR_SLOT_RYR0 / 2 + (t >> 1)));
+        const double u = (t & 1) ? ryr_slot_uniform(w.v[2], w.v[3]) : ryr_slot_uniform(w.v[0], w.v[1]);
+        const uint32_t k = (uint32_t)ryr_gauss_job(seed, cru, step, t, n, p, u, fails, &ndraw) & 255u;
         if (LPC == 1 && l >= 4) kw[LPC == 1 ? 1 : 0] |= k << (8 * (l - 4));
         else kw[0] |= k << (8 * (l & 3));
     }
