@@ -273,7 +273,11 @@ int ryr_kernel_time(ryr_handle *h, double *ms, int64_t *launches);
  * two CRU-parallel kernels (the sub-voxel cytosol/NSR kernel and the CRU
  * kernel; for the CRU kernel also 512: 256-lane workgroups with two lanes per
  * CRU, which split the RyR transitions and the LCC channels between them, for
- * grids too small to fill the GPU), and the schedule of a step's kernels:
+ * grids too small to fill the GPU; for the sub-voxel kernel also 1024: one
+ * lane per CRU holding its eight sub-voxels, so that the stencil partners
+ * inside a CRU and along x come from registers and neighbouring lanes, for
+ * grids that fill the GPU, sequential and concurrent schedules only), and
+ * the schedule of a step's kernels:
  *   RYR_SCHED_SEQ         sub-voxel kernel, CRU kernel, membrane step
  *   RYR_SCHED_FUSED       the membrane step of step n rides in the sub-voxel
  *                         launch of step n+1
@@ -294,9 +298,10 @@ int ryr_kernel_time(ryr_handle *h, double *ms, int64_t *launches);
  * the sizes with each kernel timed by its own events (fastest median kept);
  * then each schedule candidate (the tuned sizes with every schedule, the fused
  * schedule with 256-lane sub-voxel workgroups, the two-lanes-per-CRU kernel,
- * and the concurrent schedule with smaller sizes) runs 12 steps as three replays of a
- * captured 4-step graph, timed by events, and the fastest is kept.  Output
- * steps are stepped over eagerly, so tuning takes about 170 steps.  Every
+ * and the concurrent schedule with smaller sizes and with the one-lane-per-CRU
+ * sweep) runs 12 steps as three replays of a captured 4-step graph, timed by
+ * events, and the fastest is kept.  Output steps are stepped over eagerly, so
+ * tuning takes about 220 steps.  Every
  * shape computes identical values, so tuning never changes a result.  Passing
  * sizes (0, 0) re-arms the tuning; the getter reports sizes 0 and schedule -1
  * while it is pending. */

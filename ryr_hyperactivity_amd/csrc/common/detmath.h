@@ -30,10 +30,21 @@
 #define RYR_DM static inline
 #endif
 
-#if defined(__clang__)
+/* RYR_STRICT_FP: no contraction of a*b+c (the exact FP64 path and the
+ * oracle).  A unit defining RYR_CONTRACT_FAST (the FP32 production unit,
+ * built with -ffp-contract=fast-honor-pragmas) leaves it empty, so the
+ * compiler fuses there as it finds best.  RYR_EXPR_FP: fusion only inside one
+ * expression, decided by the front end, so the same code inlined into two
+ * kernels gives the same bits. */
+#if defined(__clang__) && !defined(RYR_CONTRACT_FAST)
 #define RYR_STRICT_FP _Pragma("clang fp contract(off)")
 #else
 #define RYR_STRICT_FP
+#endif
+#if defined(__clang__)
+#define RYR_EXPR_FP _Pragma("clang fp contract(on)")
+#else
+#define RYR_EXPR_FP
 #endif
 
 RYR_DM double ryr_dm_from_bits(uint64_t u) { union { uint64_t u; double d; } c; c.u = u; return c.d; }

@@ -373,6 +373,7 @@ RYR_DEV float powf_fast(float x, float y)
 }
 RYR_DEV float uptake_f(float ci, float cnsr)
 {
+    RYR_EXPR_FP
     float a = powf_fast(ci * (1.0f / 0.123f), 1.787f), b = powf_fast(cnsr * (1.0f / 1700.0f), 1.787f);
     return 0.3f * (a - b) * __builtin_amdgcn_rcpf(1.0f + a + b);
 }
@@ -544,6 +545,20 @@ RYR_DEV void ryr_rates(double Ku, double Ku2, double Kb, double cp, double cjsr,
 RYR_DEV uint32_t lane_pair_u32(uint32_t x)
 {
     return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, true);
+}
+
+/* The value x of lane src of the wave (ds_bpermute; src taken modulo 64):
+ * every lane of the wave must execute it. */
+RYR_DEV float ryr_lane_from(float x, int src)
+{
+    return __int_as_float(__builtin_amdgcn_ds_bpermute((src & 63) << 2, __float_as_int(x)));
+}
+RYR_DEV double ryr_lane_from(double x, int src)
+{
+    const long long v = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_ds_bpermute((src & 63) << 2, (int)v);
+    const int hi = __builtin_amdgcn_ds_bpermute((src & 63) << 2, (int)(v >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
 
 /* Lane Q of this lane's quad (quad_perm [Q,Q,Q,Q]). */
@@ -1252,6 +1267,7 @@ RYR_DEV void beat_track(RyrMem *M, const RyrCellInfo &C, long long n, double dt,
 /* buffers.c flux with the saturated on-rate 0.95/dt precomputed (sat) */
 RYR_DEV float fbuf(float b, float c, float dt, float sat, float kon, float koff, float B)
 {
+    RYR_EXPR_FP
     const float on = kon * c;
     return (on * dt > 1.0f) ? sat * (B - b) - koff * b : on * (B - b) - koff * b;
 }
@@ -1288,73 +1304,23 @@ RYR_DEV void gating_block(const RyrKArgs &a, int L, long long nstep);
 /* MODE bit 0: the first ncells logical workgroups run the membrane step
  * gstep + km; bit 1: the next nchunks * 256/WG * RYR_GATE_LPC run the RyR
  * gating of step gstep + kc (gating_block).  The sub-voxel sweep follows. */
-/* The sub-voxel sweep for logical workgroup L of a launch: one lane per
- * sub-voxel of plane ii of one part of one 256-CRU chunk. */
-template <typename TF, typename TS, int WG>
-RYR_DEV void cyto_block(const RyrKArgs &a, int L, long long nstep)
+/* The update of sub-voxel ii of CRU n (slot g, element e = ii P + g) from
+ * its centre values and its six stencil partners (N/S along z, E/W along y,
+ * T/B along x; a partner outside the lattice already replaced by the centre
+ * value, or under perm by the bath).  Both sweep forms call it, so they
+ * compute the same bits. */
+template <typename TF, typename TS>
+RYR_DEV void cyto_point(const RyrKArgs &a, const RyrCellInfo &C, long long nstep, bool diag, int ii, int n,
+                        long long g, long long e, TF cc, TS cn, TF inN, TF inS, TF inE, TF inW, TF inT, TF inB,
+                        TS nnN, TS nnS, TS nnE, TS nnW, TS nnT, TS nnB)
 {
-    constexpr int SUB = RYR_BLOCK / WG;
-    const int b = L / (8 * SUB);
-    const int r = L - b * (8 * SUB);
-    const int ii = r & 7;
-    const int part = r >> 3;
-    const RyrCellInfo C = a.chunks[b];
-    const int n = (b - C.first_block) * RYR_BLOCK + part * WG + (int)threadIdx.x;
-    if (n >= C.N) return;
     const int par = (int)(nstep & 1);
-    const bool diag = (nstep % a.spo) == 0;
-    const long long P = a.P, g = C.off + n, e = ii * P + g;
-    const int X = C.X, Y = C.Y, Z = C.Z;
-    /* position in the lattice: i = n mod X, rxy = n mod XY (j = rxy / X) */
-    const int sy = X, sz = X * Y;
-    const int i = n - X * (int)ryr_fastdiv((uint32_t)n, C.dX_m, C.dX_s1, C.dX_s2);
-    const int rxy = n - sz * (int)ryr_fastdiv((uint32_t)n, C.dXY_m, C.dXY_s1, C.dXY_s2);
-    const TF *__restrict__ ci0 = (const TF *)a.ci[par];
     TF *__restrict__ ci1 = (TF *)a.ci[par ^ 1];
-    const TS *__restrict__ cn0 = (const TS *)a.cnsr[par];
     TS *__restrict__ cn1 = (TS *)a.cnsr[par ^ 1];
     TF *catf = (TF *)a.catf, *cacal = (TF *)a.cacal, *casr = (TF *)a.casr;
     TF *camyo = (TF *)a.camyo, *mgmyo = (TF *)a.mgmyo;
     TS *cats = (TS *)a.cats;
     const bool schemeA = a.scheme == 0;
-
-    /* neighbours on the fine grid (bit0 z, bit1 y, bit2 x); outside the
-     * interior the mirrored ghost equals the centre value */
-    const TF cc = ci0[e];
-    const TS cn = cn0[e];
-    /* outside the lattice: the centre value, or under perm the bath ci0
-     * (the reference never refreshes those ghosts, :1662-1689) */
-    const TF cb = (C.variants & RYR_VAR_PERM) ? (TF)C.ci_bath : cc;
-    TF inN, inS, inE, inW, inT, inB;
-    TS nnN, nnS, nnE, nnW, nnT, nnB;
-    if (ii & 1) {
-        const bool in = n + sz < C.N;      /* k + 1 < Z */
-        inN = in ? ci0[(ii - 1) * P + g + sz] : cb; nnN = in ? cn0[(ii - 1) * P + g + sz] : cn;
-        inS = ci0[(ii - 1) * P + g]; nnS = cn0[(ii - 1) * P + g];
-    } else {
-        const bool in = n >= sz;           /* k > 0 */
-        inN = ci0[(ii + 1) * P + g]; nnN = cn0[(ii + 1) * P + g];
-        inS = in ? ci0[(ii + 1) * P + g - sz] : cb; nnS = in ? cn0[(ii + 1) * P + g - sz] : cn;
-    }
-    if (ii & 2) {
-        const bool in = rxy + sy < sz;     /* j + 1 < Y */
-        inE = in ? ci0[(ii - 2) * P + g + sy] : cb; nnE = in ? cn0[(ii - 2) * P + g + sy] : cn;
-        inW = ci0[(ii - 2) * P + g]; nnW = cn0[(ii - 2) * P + g];
-    } else {
-        const bool in = rxy >= sy;         /* j > 0 */
-        inE = ci0[(ii + 2) * P + g]; nnE = cn0[(ii + 2) * P + g];
-        inW = in ? ci0[(ii + 2) * P + g - sy] : cb; nnW = in ? cn0[(ii + 2) * P + g - sy] : cn;
-    }
-    if (ii & 4) {
-        const bool in = i + 1 < X;
-        inT = in ? ci0[(ii - 4) * P + g + 1] : cb; nnT = in ? cn0[(ii - 4) * P + g + 1] : cn;
-        inB = ci0[(ii - 4) * P + g]; nnB = cn0[(ii - 4) * P + g];
-    } else {
-        const bool in = i > 0;
-        inT = ci0[(ii + 4) * P + g]; nnT = cn0[(ii + 4) * P + g];
-        inB = in ? ci0[(ii + 4) * P + g - 1] : cb; nnB = in ? cn0[(ii + 4) * P + g - 1] : cn;
-    }
-
     if constexpr (sizeof(TF) == 8) {
         /* exact path: the reference's operand order throughout */
         const double dt = a.dt;
@@ -1414,7 +1380,10 @@ RYR_DEV void cyto_block(const RyrKArgs &a, int L, long long nstep)
         /* FP32 production path: the same equations with divisions by
          * constants replaced by reciprocal products.  cnsr and slow troponin
          * change by less than a float ulp per step at rest, so their updates
-         * are formed in double and rounded stochastically to float */
+         * are formed in double and rounded stochastically to float.  Fusion
+         * only inside an expression: both sweep forms (and the one-launch
+         * kernel) inline this, and results must not depend on which one ran */
+        RYR_EXPR_FP
         const float dt = (float)a.dt;
         const float sat = 0.95f * __builtin_amdgcn_rcpf(dt);
         const float rIT = (float)(1.0 / ((0.33 * 2.) * 0.7)), rIL = (float)(1.0 / ((0.7 * 2.) * 0.7));
@@ -1472,6 +1441,166 @@ RYR_DEV void cyto_block(const RyrKArgs &a, int L, long long nstep)
         mgmyo[e] = (TF)fmaxf(v_mgmyo + f_mg * dt, 0.0f);
         if (diag) { a.d_up[e] = (double)xiup; a.d_leak[e] = (double)xileak; }
     }
+}
+
+/* The sub-voxel sweep for logical workgroup L of a launch: one lane per
+ * sub-voxel of plane ii of one part of one 256-CRU chunk. */
+template <typename TF, typename TS, int WG>
+RYR_DEV void cyto_block(const RyrKArgs &a, int L, long long nstep)
+{
+    constexpr int SUB = RYR_BLOCK / WG;
+    const int b = L / (8 * SUB);
+    const int r = L - b * (8 * SUB);
+    const int ii = r & 7;
+    const int part = r >> 3;
+    const RyrCellInfo C = a.chunks[b];
+    const int n = (b - C.first_block) * RYR_BLOCK + part * WG + (int)threadIdx.x;
+    if (n >= C.N) return;
+    const int par = (int)(nstep & 1);
+    const bool diag = (nstep % a.spo) == 0;
+    const long long P = a.P, g = C.off + n, e = ii * P + g;
+    const int X = C.X, Y = C.Y, Z = C.Z;
+    /* position in the lattice: i = n mod X, rxy = n mod XY (j = rxy / X) */
+    const int sy = X, sz = X * Y;
+    const int i = n - X * (int)ryr_fastdiv((uint32_t)n, C.dX_m, C.dX_s1, C.dX_s2);
+    const int rxy = n - sz * (int)ryr_fastdiv((uint32_t)n, C.dXY_m, C.dXY_s1, C.dXY_s2);
+    const TF *__restrict__ ci0 = (const TF *)a.ci[par];
+    const TS *__restrict__ cn0 = (const TS *)a.cnsr[par];
+
+    /* neighbours on the fine grid (bit0 z, bit1 y, bit2 x); outside the
+     * interior the mirrored ghost equals the centre value */
+    const TF cc = ci0[e];
+    const TS cn = cn0[e];
+    /* outside the lattice: the centre value, or under perm the bath ci0
+     * (the reference never refreshes those ghosts, :1662-1689) */
+    const TF cb = (C.variants & RYR_VAR_PERM) ? (TF)C.ci_bath : cc;
+    TF inN, inS, inE, inW, inT, inB;
+    TS nnN, nnS, nnE, nnW, nnT, nnB;
+    if (ii & 1) {
+        const bool in = n + sz < C.N;      /* k + 1 < Z */
+        inN = in ? ci0[(ii - 1) * P + g + sz] : cb; nnN = in ? cn0[(ii - 1) * P + g + sz] : cn;
+        inS = ci0[(ii - 1) * P + g]; nnS = cn0[(ii - 1) * P + g];
+    } else {
+        const bool in = n >= sz;           /* k > 0 */
+        inN = ci0[(ii + 1) * P + g]; nnN = cn0[(ii + 1) * P + g];
+        inS = in ? ci0[(ii + 1) * P + g - sz] : cb; nnS = in ? cn0[(ii + 1) * P + g - sz] : cn;
+    }
+    if (ii & 2) {
+        const bool in = rxy + sy < sz;     /* j + 1 < Y */
+        inE = in ? ci0[(ii - 2) * P + g + sy] : cb; nnE = in ? cn0[(ii - 2) * P + g + sy] : cn;
+        inW = ci0[(ii - 2) * P + g]; nnW = cn0[(ii - 2) * P + g];
+    } else {
+        const bool in = rxy >= sy;         /* j > 0 */
+        inE = ci0[(ii + 2) * P + g]; nnE = cn0[(ii + 2) * P + g];
+        inW = in ? ci0[(ii + 2) * P + g - sy] : cb; nnW = in ? cn0[(ii + 2) * P + g - sy] : cn;
+    }
+    if (ii & 4) {
+        const bool in = i + 1 < X;
+        inT = in ? ci0[(ii - 4) * P + g + 1] : cb; nnT = in ? cn0[(ii - 4) * P + g + 1] : cn;
+        inB = ci0[(ii - 4) * P + g]; nnB = cn0[(ii - 4) * P + g];
+    } else {
+        const bool in = i > 0;
+        inT = ci0[(ii + 4) * P + g]; nnT = cn0[(ii + 4) * P + g];
+        inB = in ? ci0[(ii + 4) * P + g - 1] : cb; nnB = in ? cn0[(ii + 4) * P + g - 1] : cn;
+    }
+
+    cyto_point<TF, TS>(a, C, nstep, diag, ii, n, g, e, cc, cn, inN, inS, inE, inW, inT, inB,
+                       nnN, nnS, nnE, nnW, nnT, nnB);
+}
+
+/* The sub-voxel sweep with one lane per CRU, for grids that fill the GPU:
+ * the eight sub-voxels of CRU n are held in registers, so the partners inside
+ * the CRU come from registers, the x partners (CRUs n - 1 and n + 1) from the
+ * neighbouring lanes of the wave (from memory at the wave's two ends), and
+ * only the y and z partners from memory: 4 partner loads per sub-voxel and
+ * field fewer than the per-sub-voxel form's 6.  Logical workgroup b is
+ * chunk b.  Each sub-voxel's update is cyto_point, as in cyto_block. */
+template <typename TF, typename TS>
+RYR_DEV void cyto_cru_block(const RyrKArgs &a, int b, long long nstep)
+{
+    const RyrCellInfo C = a.chunks[b];
+    const int n = (b - C.first_block) * RYR_BLOCK + (int)threadIdx.x;
+    /* a lane past the cell's last CRU holds its padding slot (inside the
+     * chunk) until the lane exchange, then leaves; it loads no partner */
+    const bool active = n < C.N;
+    const int par = (int)(nstep & 1);
+    const bool diag = (nstep % a.spo) == 0;
+    const long long P = a.P, g = C.off + n;
+    const int X = C.X, sy = C.X, sz = C.X * C.Y;
+    const int i = n - X * (int)ryr_fastdiv((uint32_t)n, C.dX_m, C.dX_s1, C.dX_s2);
+    const int rxy = n - sz * (int)ryr_fastdiv((uint32_t)n, C.dXY_m, C.dXY_s1, C.dXY_s2);
+    const TF *__restrict__ ci0 = (const TF *)a.ci[par];
+    const TS *__restrict__ cn0 = (const TS *)a.cnsr[par];
+    TF cc[8];
+    TS cn[8];
+#pragma unroll
+    for (int ii = 0; ii < 8; ++ii) {
+        cc[ii] = ci0[ii * P + g];
+        cn[ii] = cn0[ii * P + g];
+    }
+    /* x partners: plane q of CRU n + 1 (partner of sub-voxel q + 4) and plane
+     * q + 4 of CRU n - 1 (partner of sub-voxel q), q < 4 */
+    const int lane = (int)threadIdx.x & 63;
+    const bool inxp = active && i + 1 < X, inxm = active && i > 0;
+    TF xpc[4], xmc[4];
+    TS xpn[4], xmn[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        xpc[q] = ryr_lane_from(cc[q], lane + 1);
+        xpn[q] = ryr_lane_from(cn[q], lane + 1);
+        xmc[q] = ryr_lane_from(cc[q + 4], lane - 1);
+        xmn[q] = ryr_lane_from(cn[q + 4], lane - 1);
+    }
+    if (!active) return;
+    if (lane == 63 && inxp) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { xpc[q] = ci0[q * P + g + 1]; xpn[q] = cn0[q * P + g + 1]; }
+    }
+    if (lane == 0 && inxm) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { xmc[q] = ci0[(q + 4) * P + g - 1]; xmn[q] = cn0[(q + 4) * P + g - 1]; }
+    }
+    const bool inzp = n + sz < C.N, inzm = n >= sz;     /* k + 1 < Z, k > 0 */
+    const bool inyp = rxy + sy < sz, inym = rxy >= sy;  /* j + 1 < Y, j > 0 */
+    const bool perm = (C.variants & RYR_VAR_PERM) != 0;
+#pragma unroll
+    for (int ii = 0; ii < 8; ++ii) {
+        const long long e = ii * P + g;
+        /* outside the lattice: the centre value, or under perm the bath ci0 */
+        const TF cb = perm ? (TF)C.ci_bath : cc[ii];
+        TF inN, inS, inE, inW, inT, inB;
+        TS nnN, nnS, nnE, nnW, nnT, nnB;
+        if (ii & 1) {
+            inN = inzp ? ci0[(ii - 1) * P + g + sz] : cb; nnN = inzp ? cn0[(ii - 1) * P + g + sz] : cn[ii];
+            inS = cc[ii - 1]; nnS = cn[ii - 1];
+        } else {
+            inN = cc[ii + 1]; nnN = cn[ii + 1];
+            inS = inzm ? ci0[(ii + 1) * P + g - sz] : cb; nnS = inzm ? cn0[(ii + 1) * P + g - sz] : cn[ii];
+        }
+        if (ii & 2) {
+            inE = inyp ? ci0[(ii - 2) * P + g + sy] : cb; nnE = inyp ? cn0[(ii - 2) * P + g + sy] : cn[ii];
+            inW = cc[ii - 2]; nnW = cn[ii - 2];
+        } else {
+            inE = cc[ii + 2]; nnE = cn[ii + 2];
+            inW = inym ? ci0[(ii + 2) * P + g - sy] : cb; nnW = inym ? cn0[(ii + 2) * P + g - sy] : cn[ii];
+        }
+        if (ii & 4) {
+            inT = inxp ? xpc[ii - 4] : cb; nnT = inxp ? xpn[ii - 4] : cn[ii];
+            inB = cc[ii - 4]; nnB = cn[ii - 4];
+        } else {
+            inT = cc[ii + 4]; nnT = cn[ii + 4];
+            inB = inxm ? xmc[ii] : cb; nnB = inxm ? xmn[ii] : cn[ii];
+        }
+        cyto_point<TF, TS>(a, C, nstep, diag, ii, n, g, e, cc[ii], cn[ii], inN, inS, inE, inW, inT, inB,
+                           nnN, nnS, nnE, nnW, nnT, nnB);
+    }
+}
+
+/* the lane-per-CRU sweep on its own: one 256-lane workgroup per chunk */
+template <typename TF, typename TS>
+__global__ __launch_bounds__(RYR_BLOCK) void k_cyto_cru(RyrKArgs a, int kc)
+{
+    cyto_cru_block<TF, TS>(a, xcd_remap((int)blockIdx.x, (int)gridDim.x), *a.gstep + kc);
 }
 
 template <typename TF, typename TS, int WG, int MODE>

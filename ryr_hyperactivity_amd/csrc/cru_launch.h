@@ -3,7 +3,8 @@
  * contraction on outside the strict-FP regions).
  *
  * The sub-voxel kernel and the CRU kernel come in three workgroup sizes
- * (64, 128, 256 lanes).  Every size computes the same values in the same
+ * (64, 128, 256 lanes); the sub-voxel kernel also in a one-lane-per-CRU form
+ * (size code 1024), the CRU kernel in a two-lanes-per-CRU form (512).  Every size computes the same values in the same
  * order (the whole-cell sums are formed per wave), so the shape is a pure
  * speed choice, and so is carrying the membrane step inside the sub-voxel
  * launch.  With a non-null event pair a kernel is launched with
@@ -64,6 +65,13 @@ static hipError_t ryr_go_cyto_w(const RyrKArgs &a, int nchunks, int wg, int kc, 
     case 64: return ryr_go_cyto_m<TF, TS, 64, MODE>(a, nchunks, kc, km, amp, s, ev);
     case 128: return ryr_go_cyto_m<TF, TS, 128, MODE>(a, nchunks, kc, km, amp, s, ev);
     case 256: return ryr_go_cyto_m<TF, TS, 256, MODE>(a, nchunks, kc, km, amp, s, ev);
+    case 1024:
+        /* one lane per CRU (cyto_cru_block), one 256-lane workgroup per chunk; the sweep alone */
+        if constexpr (MODE == 0) {
+            RYR_GO((k_cyto_cru<TF, TS>), dim3((unsigned)nchunks), dim3(RYR_BLOCK), s, ev, a, kc);
+            return hipGetLastError();
+        }
+        return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
     }
 }

@@ -443,6 +443,7 @@ static hipError_t go_null(ryr_handle *h, unsigned grid, unsigned wg, hipStream_t
 static hipError_t go_cyto(ryr_handle *h, int wg, int kc, int km, hipEvent_t *ev, bool gate = false)
 {
     if (h->null_launch) {
+        if (wg == 1024) return go_null(h, (unsigned)h->nblocks, RYR_BLOCK);
         const unsigned per = (unsigned)(RYR_BLOCK / wg) * (unsigned)h->nblocks;
         return go_null(h, (km >= 0 ? (unsigned)h->ncells : 0u) + per * (gate ? 8u + RYR_GATE_LPC : 8u), wg);
     }
@@ -687,8 +688,10 @@ static int tune_step(ryr_handle *h)
 #define RYR_TUNE_GSTEPS 4
 /* whether a shape can run: the concurrent schedule needs cleft scheme A,
  * the gating schedule a CRU size of 128, 256 or 512 */
-static bool shape_ok(const ryr_handle *h, int wg_cru, int sched)
+static bool shape_ok(const ryr_handle *h, int wg_cyto, int wg_cru, int sched)
 {
+    /* the one-lane-per-CRU sweep (1024) runs as a launch of its own */
+    if (wg_cyto == 1024 && sched != RYR_SCHED_SEQ && sched != RYR_SCHED_CONCURRENT) return false;
     if (sched == RYR_SCHED_CONCURRENT) return h->p.scheme == RYR_SCHEME_A;
     if (sched == RYR_SCHED_ONE) return h->p.scheme == RYR_SCHEME_A && (wg_cru == 256 || wg_cru == 512);
     if (sched == RYR_SCHED_FUSED_GATING) return wg_cru == 128 || wg_cru == 256 || wg_cru == 512;
@@ -725,6 +728,9 @@ static void sched_candidates(const ryr_handle *h, std::vector<std::array<int, 3>
         add(256, 64, RYR_SCHED_CONCURRENT);
         add(128, 128, RYR_SCHED_CONCURRENT);
         add(64, 64, RYR_SCHED_CONCURRENT);
+        /* the one-lane-per-CRU sweep beside the CRU kernel (grids that fill the GPU) */
+        add(1024, 64, RYR_SCHED_CONCURRENT);
+        add(1024, 128, RYR_SCHED_CONCURRENT);
     }
 }
 
@@ -1023,12 +1029,13 @@ extern "C" int ryr_set_launch_shape(ryr_handle *h, int wg_cyto, int wg_cru, int 
         h->tune_i = 0;
         h->cand.clear();
         for (auto &k : h->tune_ms) for (auto &q : k) q.clear();
-    } else if (ok(wg_cyto) && ok_cru(wg_cru) && shape_ok(h, wg_cru, sched)) {
+    } else if ((ok(wg_cyto) || wg_cyto == 1024) && ok_cru(wg_cru) && shape_ok(h, wg_cyto, wg_cru, sched)) {
         h->wg_cyto = wg_cyto;
         h->wg_cru = wg_cru;
         h->sched = sched;
     } else {
-        h->err = "workgroup sizes must be 64, 128 or 256 (CRU kernel also 512) and the schedule one of "
+        h->err = "workgroup sizes must be 64, 128 or 256 (sub-voxel kernel also 1024 with the sequential or "
+                 "concurrent schedule, CRU kernel also 512) and the schedule one of "
                  "RYR_SCHED_* (concurrent needs cleft scheme A, gating a CRU size of 128, 256 or 512), "
                  "or sizes 0, 0 to tune";
         return RYR_E_ARG;

@@ -133,8 +133,8 @@ VARIANT_CASES = [
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("shape", [None, (256, 256, 3), (128, 64, 1), (256, 256, 4), (256, 512, 4)],
-                         ids=["tuned", "gating-fused", "fused", "one-launch", "one-launch-2"])
+@pytest.mark.parametrize("shape", [None, (256, 256, 3), (128, 64, 1), (256, 256, 4), (256, 512, 4), (1024, 64, 2)],
+                         ids=["tuned", "gating-fused", "fused", "one-launch", "one-launch-2", "lane-per-cru-sweep"])
 @pytest.mark.parametrize("name,pk,dk", VARIANT_CASES, ids=[c[0] for c in VARIANT_CASES])
 def test_variant_fp64_bitexact_against_the_oracle(name, pk, dk, shape):
     """Every variant, FP64 against the oracle, at the tuned shape and forced
@@ -144,8 +144,8 @@ def test_variant_fp64_bitexact_against_the_oracle(name, pk, dk, shape):
     from ryr_hyperactivity_amd import CellEngine
     print(f"case variant {name} shape {shape}", flush=True)
     p = Params.default(stim_offset=1.0, **pk)
-    if shape is not None and shape[2] == 4 and p.scheme == 1:
-        pytest.skip("the one-launch schedule needs cleft scheme A")
+    if shape is not None and shape[2] in (2, 4) and p.scheme == 1:
+        pytest.skip("the one-launch and concurrent schedules need cleft scheme A")
     d = CellDesc.default(nx=9, ny=5, nz=3, cl=10.0, seed=6, **dk)   # 400 steps per cycle: iso events by step 2200
     o = OracleCell(p, d)
     with CellEngine(p, d) as eng:
