@@ -299,9 +299,9 @@ int ryr_kernel_time(ryr_handle *h, double *ms, int64_t *launches);
  * then each schedule candidate (the tuned sizes with every schedule, the fused
  * schedule with 256-lane sub-voxel workgroups, the two-lanes-per-CRU kernel,
  * and the concurrent schedule with smaller sizes and with the one-lane-per-CRU
- * sweep) runs 12 steps as three replays of a captured 4-step graph, timed by
+ * sweep) runs 24 steps as three replays of a captured 8-step graph, timed by
  * events, and the fastest is kept.  Output steps are stepped over eagerly, so
- * tuning takes about 220 steps.  Every
+ * tuning takes about 450 steps.  Every
  * shape computes identical values, so tuning never changes a result.  Passing
  * sizes (0, 0) re-arms the tuning; the getter reports sizes 0 and schedule -1
  * while it is pending. */

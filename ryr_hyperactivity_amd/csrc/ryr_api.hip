@@ -685,7 +685,10 @@ static int tune_step(ryr_handle *h)
  * around each replay.  The sequential candidate is kept unless another is
  * faster by more than 3%.  Each candidate advances 3 * RYR_TUNE_GSTEPS
  * steps; candidates run while a call has that many steps left. */
-#define RYR_TUNE_GSTEPS 4
+/* 8 steps: in a shorter graph the run's first launch and last membrane step
+ * (one-launch schedule) weigh enough to hide a 7% difference in the steps
+ * between them */
+#define RYR_TUNE_GSTEPS 8
 /* whether a shape can run: the concurrent schedule needs cleft scheme A,
  * the gating schedule a CRU size of 128, 256 or 512 */
 static bool shape_ok(const ryr_handle *h, int wg_cyto, int wg_cru, int sched)

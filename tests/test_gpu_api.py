@@ -33,7 +33,7 @@ def test_step_counts_are_exact_through_the_tuning():
     done = 0
     print("case tuning step counts 12x7x3", flush=True)
     with CellEngine(p, d) as eng:
-        for n in (1, 4, 3, 2, 7, 11, 13, 25, 5, 99, 101, 3, 60, 17):
+        for n in (1, 4, 3, 2, 7, 11, 13, 25, 5, 99, 101, 3, 60, 17, 123, 77):
             eng.step(n)
             o.step(n)
             done += n

@@ -428,12 +428,12 @@ def test_tuning_and_kernel_profile():
         assert eng.launch_shape() == (0, 0, -1)    # tuning pending
         eng.step(5)
         assert eng.launch_shape() == (0, 0, -1)
-        eng.step(260)                              # sizes after 9 steps, then the schedule candidates
+        eng.step(500)                              # sizes after 9 steps, then the schedule candidates
         wc, wr, sc = eng.launch_shape()
         assert wc in (64, 128, 256, 1024) and wr in (64, 128, 256, 512) and sc in (0, 1, 2, 3, 4)
         prof = eng.profile_kernels(120)
-        assert eng.cru_steps() == 385 * d.ncru
-        assert eng.scalars()[0][S["STEP"]] == 385
+        assert eng.cru_steps() == 625 * d.ncru
+        assert eng.scalars()[0][S["STEP"]] == 625
         for k in ("k_cyto", "k_cru_step"):
             us, n = prof[k]
             if sc == 4 and k == "k_cru_step":
@@ -444,7 +444,7 @@ def test_tuning_and_kernel_profile():
         # output steps and at the last step
         us, n = prof["k_membrane"]
         assert (n == 120 if sc in (0, 2) else n == 2) and 0.5 < us < 5e4, (us, n, sc)
-        assert prof["k_diag"][1] == 1              # steps 265..384 hold one output step (300)
+        assert prof["k_diag"][1] == 1              # steps 505..624 hold one output step (600)
         assert eng.latency_floor(100) > 0.0
         eng.set_launch_shape(0, 0, 0)
         assert eng.launch_shape() == (0, 0, -1)
