@@ -547,6 +547,17 @@ RYR_DEV uint32_t lane_pair_u32(uint32_t x)
     return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, true);
 }
 
+/* Streamed state (read once and written once per step, never re-read by a
+ * neighbour): nontemporal loads and stores, which leave the L2 to the values
+ * the stencils re-read (ci and cnsr of the step's start, cp, cs and cjsr). */
+template <typename T> RYR_DEV T ryr_ld_nt(const T *p) { return __builtin_nontemporal_load(p); }
+template <typename T> RYR_DEV void ryr_st_nt(T *p, T v) { __builtin_nontemporal_store(v, p); }
+/* the same where NT holds, plain otherwise (the CRU kernel's per-CRU state:
+ * nontemporal in the instances batches run, which fill the chip; plain in
+ * the single cell's, where the hint measured 1% slower) */
+template <bool NT, typename T> RYR_DEV T ryr_ld_if(const T *p) { if constexpr (NT) return ryr_ld_nt(p); else return *p; }
+template <bool NT, typename T> RYR_DEV void ryr_st_if(T *p, T v) { if constexpr (NT) ryr_st_nt(p, v); else *p = v; }
+
 /* The value x of lane src of the wave (ds_bpermute; src taken modulo 64):
  * every lane of the wave must execute it. */
 RYR_DEV float ryr_lane_from(float x, int src)
@@ -1304,6 +1315,7 @@ RYR_DEV void gating_block(const RyrKArgs &a, int L, long long nstep);
 /* MODE bit 0: the first ncells logical workgroups run the membrane step
  * gstep + km; bit 1: the next nchunks * 256/WG * RYR_GATE_LPC run the RyR
  * gating of step gstep + kc (gating_block).  The sub-voxel sweep follows. */
+
 /* The update of sub-voxel ii of CRU n (slot g, element e = ii P + g) from
  * its centre values and its six stencil partners (N/S along z, E/W along y,
  * T/B along x; a partner outside the lattice already replaced by the centre
@@ -1334,8 +1346,8 @@ RYR_DEV void cyto_point(const RyrKArgs &a, const RyrCellInfo &C, long long nstep
                                 (inE - cc) / ((0.33 * 2.) * 0.7) + (inW - cc) / ((0.33 * 2.) * 0.7) +
                                 (inT - cc) / ((0.7 * 2.) * 0.7) + (inB - cc) / ((0.7 * 2.) * 0.7));
         double dotcnsr = (xiup - xileak) * (0.5 / 8.) / (0.025 / 8.) + xicoupn;
-        const double v_catf = catf[e], v_cats = cats[e], v_cacal = cacal[e], v_casr = casr[e];
-        const double v_camyo = camyo[e], v_mgmyo = mgmyo[e];
+        const double v_catf = ryr_ld_nt(catf + e), v_cats = ryr_ld_nt(cats + e), v_cacal = ryr_ld_nt(cacal + e);
+        const double v_casr = ryr_ld_nt(casr + e), v_camyo = ryr_ld_nt(camyo + e), v_mgmyo = ryr_ld_nt(mgmyo + e);
         const double f_tf = b_tropf<double>(v_catf, cc, dt);
         const double f_ts = b_trops<double>(v_cats, cc, dt);
         const double f_cal = b_bucal<double>(v_cacal, cc, dt);
@@ -1358,9 +1370,9 @@ RYR_DEV void cyto_point(const RyrKArgs &a, const RyrCellInfo &C, long long nstep
         if (ii >= 2 || schemeA) {
             double cin = cc + dotci * dt;
             if (cin < 0) cin = 0;
-            ci1[e] = cin;
+            ryr_st_nt(ci1 + e, (TF)cin);
         }
-        cn1[e] = cn + dotcnsr * dt;
+        ryr_st_nt(cn1 + e, (TS)(cn + dotcnsr * dt));
         double n_catf = v_catf + f_tf * dt;
         if (n_catf < 0) n_catf = 0;
         double n_cats = v_cats + f_ts * dt;
@@ -1373,8 +1385,8 @@ RYR_DEV void cyto_point(const RyrKArgs &a, const RyrCellInfo &C, long long nstep
         if (n_camyo < 0) n_camyo = 0;
         double n_mgmyo = v_mgmyo + b_myomg<double>(n_camyo, v_mgmyo, dt) * dt;
         if (n_mgmyo < 0) n_mgmyo = 0;
-        catf[e] = n_catf; cats[e] = n_cats; cacal[e] = n_cacal;
-        casr[e] = n_casr; camyo[e] = n_camyo; mgmyo[e] = n_mgmyo;
+        ryr_st_nt(catf + e, (TF)n_catf); ryr_st_nt(cats + e, (TS)n_cats); ryr_st_nt(cacal + e, (TF)n_cacal);
+        ryr_st_nt(casr + e, (TF)n_casr); ryr_st_nt(camyo + e, (TF)n_camyo); ryr_st_nt(mgmyo + e, (TF)n_mgmyo);
         if (diag) { a.d_up[e] = xiup; a.d_leak[e] = xileak; }
     } else {
         /* FP32 production path: the same equations with divisions by
@@ -1398,9 +1410,10 @@ RYR_DEV void cyto_point(const RyrKArgs &a, const RyrCellInfo &C, long long nstep
         const float xicoupn = ((float)(nnN - cn) + (float)(nnS - cn) + (float)(nnE - cn) + (float)(nnW - cn)) * rNT +
                               ((float)(nnT - cn) + (float)(nnB - cn)) * rNL;
         float dotcnsr = (xiup - xileak) * (float)((0.5 / 8.) / (0.025 / 8.)) + xicoupn;
-        const float v_catf = (float)catf[e], v_cacal = (float)cacal[e], v_casr = (float)casr[e];
-        const float v_camyo = (float)camyo[e], v_mgmyo = (float)mgmyo[e];
-        const TS v_cats = cats[e];
+        const float v_catf = (float)ryr_ld_nt(catf + e), v_cacal = (float)ryr_ld_nt(cacal + e);
+        const float v_casr = (float)ryr_ld_nt(casr + e);
+        const float v_camyo = (float)ryr_ld_nt(camyo + e), v_mgmyo = (float)ryr_ld_nt(mgmyo + e);
+        const TS v_cats = ryr_ld_nt(cats + e);
         const float f_tf = fbuf(v_catf, cf, dt, sat, 0.0327f, 0.0196f, 70.0f);
         const float f_ts = fbuf((float)v_cats, cf, dt, sat, 0.00254f, 0.000033f, 134.0f);
         const float f_cal = fbuf(v_cacal, cf, dt, sat, 0.0543f, 0.238f, 24.0f);
@@ -1419,26 +1432,28 @@ RYR_DEV void cyto_point(const RyrKArgs &a, const RyrCellInfo &C, long long nstep
                 ((TF *)a.sb)[e] = (TF)dotci;
             }
         }
-        if (ii >= 2 || schemeA) ci1[e] = (TF)fmaxf(cf + dotci * dt, 0.0f);
+        if (ii >= 2 || schemeA) ryr_st_nt(ci1 + e, (TF)fmaxf(cf + dotci * dt, 0.0f));
         const double cn_new = (double)cn + (double)dotcnsr * a.dt;
         if constexpr (sizeof(TS) == 4)
-            cn1[e] = ryr_sr_f32(cn_new, ryr_sr_bits(ryr_sr_key(C.seed, (uint64_t)nstep, RYR_SR_CNSR), (uint32_t)n, (uint32_t)ii));
+            ryr_st_nt(cn1 + e, ryr_sr_f32(cn_new, ryr_sr_bits(ryr_sr_key(C.seed, (uint64_t)nstep, RYR_SR_CNSR), (uint32_t)n,
+                                                              (uint32_t)ii)));
         else
-            cn1[e] = cn_new;
+            ryr_st_nt(cn1 + e, (TS)cn_new);
         const float n_camyo = fmaxf(v_camyo + f_myo * dt, 0.0f);
         const float on_mg = 0.0157f * 500.0f;
         const float f_mg = (on_mg * dt > 1.0f ? sat : on_mg) * (140.0f - n_camyo - v_mgmyo) - 0.057f * v_mgmyo;
-        catf[e] = (TF)fmaxf(v_catf + f_tf * dt, 0.0f);
+        ryr_st_nt(catf + e, (TF)fmaxf(v_catf + f_tf * dt, 0.0f));
         double n_cats = (double)v_cats + (double)f_ts * a.dt;
         if (n_cats < 0.0) n_cats = 0.0;
         if constexpr (sizeof(TS) == 4)
-            cats[e] = ryr_sr_f32(n_cats, ryr_sr_bits(ryr_sr_key(C.seed, (uint64_t)nstep, RYR_SR_CATS), (uint32_t)n, (uint32_t)ii));
+            ryr_st_nt(cats + e, ryr_sr_f32(n_cats, ryr_sr_bits(ryr_sr_key(C.seed, (uint64_t)nstep, RYR_SR_CATS), (uint32_t)n,
+                                                                (uint32_t)ii)));
         else
-            cats[e] = n_cats;
-        cacal[e] = (TF)fmaxf(v_cacal + f_cal * dt, 0.0f);
-        casr[e] = (TF)fmaxf(v_casr + f_sr * dt, 0.0f);
-        camyo[e] = (TF)n_camyo;
-        mgmyo[e] = (TF)fmaxf(v_mgmyo + f_mg * dt, 0.0f);
+            ryr_st_nt(cats + e, (TS)n_cats);
+        ryr_st_nt(cacal + e, (TF)fmaxf(v_cacal + f_cal * dt, 0.0f));
+        ryr_st_nt(casr + e, (TF)fmaxf(v_casr + f_sr * dt, 0.0f));
+        ryr_st_nt(camyo + e, (TF)n_camyo);
+        ryr_st_nt(mgmyo + e, (TF)fmaxf(v_mgmyo + f_mg * dt, 0.0f));
         if (diag) { a.d_up[e] = (double)xiup; a.d_leak[e] = (double)xileak; }
     }
 }
@@ -1747,6 +1762,7 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L, double stim_amp = 0.0)
     static_assert(LPC == 1 || (LPC == 2 && WG >= 128), "paired lanes need two waves per workgroup");
 
     constexpr int CPW = WG / LPC;          /* CRUs per workgroup */
+    constexpr bool NTC = WG < 256;         /* nontemporal per-CRU state (batch instances) */
     constexpr int SUB = RYR_BLOCK / CPW;
     const int bid = L / SUB;               /* 256-CRU chunk */
     const int part = L - bid * SUB;
@@ -1805,7 +1821,7 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L, double stim_amp = 0.0)
     const TF csv = cs0[g];
     TF cpv = cp0[g];
     const TS cjsrv = cjsr0[g];
-    uint32_t lw = a.lccw[g], rw = a.ryrw[g], ax = a.aux[g];
+    uint32_t lw = ryr_ld_if<NTC>(a.lccw + g), rw = ryr_ld_if<NTC>(a.ryrw + g), ax = ryr_ld_if<NTC>(a.aux + g);
 
     /* VNEXT (one-launch schedule, after the run's first step): V of this
      * step and its V-only factors come from this launch's membrane workgroup
@@ -1835,9 +1851,10 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L, double stim_amp = 0.0)
         TF e_sar{}, e_sarh{}, e_sarj{}, e_sarhj{}, e_up{}, e_dn{}, e_sb0{}, e_sb1{}, e_ancx{};
         TS e_cjb{};
         if constexpr (GSPLIT) {
-            e_sar = casar[g]; e_sarh = casarh[g]; e_sarj = casarj[g]; e_sarhj = casarhj[g];
-            e_cjb = cjsrbp[g];
-            e_ancx = ancx[g];
+            e_sar = ryr_ld_if<NTC>(casar + g); e_sarh = ryr_ld_if<NTC>(casarh + g);
+            e_sarj = ryr_ld_if<NTC>(casarj + g); e_sarhj = ryr_ld_if<NTC>(casarhj + g);
+            e_cjb = ryr_ld_if<NTC>(cjsrbp + g);
+            e_ancx = ryr_ld_if<NTC>(ancx + g);
             if (has_up) e_up = cs0[g + sz];
             if (has_dn) e_dn = cs0[g - sz];
             if (SCHB) { e_sb0 = ((const TF *)a.sb)[g]; e_sb1 = ((const TF *)a.sb)[P + g]; }
@@ -1973,11 +1990,11 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L, double stim_amp = 0.0)
         TF ancx_new;
         if (perm) {                                 /* no NCX (:1302) */
             xinaca = 0.0;
-            ancx_new = GSPLIT ? e_ancx : ancx[g];
+            ancx_new = GSPLIT ? e_ancx : ryr_ld_if<NTC>(ancx + g);
         } else if constexpr (sizeof(TF) != 8) {
             const RyrVF vf = vf_early ? vf0 : load_vf(M);
             const float csm = (float)cs_d * 1e-3f;
-            const float An = (float)(GSPLIT ? e_ancx : ancx[g]);
+            const float An = (float)(GSPLIT ? e_ancx : ryr_ld_if<NTC>(ancx + g));
             /* slowncxchange (subroutine_cuda.c:59-62): the exchanger at 5 x the allosteric factor */
             const float Ka = (C.variants & RYR_VAR_SLOWNCX) ? An * 5.0f : An;
             const float t2 = 669921.875f * csm * (1.0f + csm * (float)(1.0 / 0.00359));
@@ -1992,7 +2009,7 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L, double stim_amp = 0.0)
             RYR_STRICT_FP
             const RyrVF vf = vf_early ? vf0 : load_vf(M);
             double cs = cs_d / 1000.0;
-            const double An = (double)(GSPLIT ? e_ancx : ancx[g]);
+            const double An = (double)(GSPLIT ? e_ancx : ryr_ld_if<NTC>(ancx + g));
             double Ka = An;
             if (C.variants & RYR_VAR_SLOWNCX) Ka = An * 5;  /* subroutine_cuda.c:59-62 */
             double t2 = (87.5) * (87.5) * (87.5) * cs * (1.0 + cs / 0.00359);
@@ -2004,7 +2021,7 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L, double stim_amp = 0.0)
             ancx_new = (TF)(An + Ancxdot * dt);
             xinaca = C.svncx * 1.0 * Inaca;
         }
-        if (lead) ancx[g] = ancx_new;
+        if (lead) ryr_st_if<NTC>(ancx + g, ancx_new);
 
         /* exchange fluxes and JSR (:1309-1318) */
         const TF cpT = cpv, csT = csv;
@@ -2031,14 +2048,14 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L, double stim_amp = 0.0)
             double flux = ((-(double)diffjn0 - (double)diffjn1) - xire * (0.00126) * 1.0 / (0.02 * 1));
             dotcjsr_d = beta * flux;
             dotcjsr = (TS)dotcjsr_d;
-            cjsrb_new_d = (double)(GSPLIT ? e_cjb : cjsrbp[g]) + (1 - beta) * flux * dt;
+            cjsrb_new_d = (double)(GSPLIT ? e_cjb : ryr_ld_if<NTC>(cjsrbp + g)) + (1 - beta) * flux * dt;
             cjsrb_new = (TS)cjsrb_new_d;
         }
 
         /* cleft/sub-membrane sub-steps (:1356-1473 / :1475-1597); the
          * cytosol/NSR sub-voxels were advanced by k_cyto */
-        TF v_casar = GSPLIT ? e_sar : casar[g], v_casarh = GSPLIT ? e_sarh : casarh[g];
-        TF v_casarj = GSPLIT ? e_sarj : casarj[g], v_casarhj = GSPLIT ? e_sarhj : casarhj[g];
+        TF v_casar = GSPLIT ? e_sar : ryr_ld_if<NTC>(casar + g), v_casarh = GSPLIT ? e_sarh : ryr_ld_if<NTC>(casarh + g);
+        TF v_casarj = GSPLIT ? e_sarj : ryr_ld_if<NTC>(casarj + g), v_casarhj = GSPLIT ? e_sarhj : ryr_ld_if<NTC>(casarhj + g);
         const TF csb = perm ? (TF)C.ci_bath : csv;             /* perm: bath outside the lattice */
         const TF cs_up = has_up ? (GSPLIT ? e_up : cs0[g + sz]) : csb;   /* k + 1 < Z */
         const TF cs_dn = has_dn ? (GSPLIT ? e_dn : cs0[g - sz]) : csb;   /* k > 0 */
@@ -2149,27 +2166,30 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L, double stim_amp = 0.0)
         if (cpv < TF(0)) { cpv = TF(0); ++nclampcp; }
         if (csn < TF(0)) { csn = TF(1e-6); ++nclampcs; }
         if (lead) {
-            casar[g] = v_casar; casarh[g] = v_casarh; casarj[g] = v_casarj; casarhj[g] = v_casarhj;
-            cp1[g] = cpv;
-            cs1[g] = csn;
+            ryr_st_if<NTC>(casar + g, v_casar); ryr_st_if<NTC>(casarh + g, v_casarh);
+            ryr_st_if<NTC>(casarj + g, v_casarj); ryr_st_if<NTC>(casarhj + g, v_casarhj);
+            ryr_st_if<NTC>(cp1 + g, cpv);
+            ryr_st_if<NTC>(cs1 + g, csn);
             if constexpr (sizeof(TS) == 4) {
                 /* float JSR pools (FP32 production): stochastic rounding of the double update */
-                cjsr1[g] = ryr_sr_f32((double)cjsrv + dotcjsr_d * dt,
-                                      ryr_sr_bits(ryr_sr_key(C.seed, (uint64_t)nstep, RYR_SR_CJSR), (uint32_t)n, 0u));
-                cjsrbp[g] = ryr_sr_f32(cjsrb_new_d,
-                                       ryr_sr_bits(ryr_sr_key(C.seed, (uint64_t)nstep, RYR_SR_CJSRB), (uint32_t)n, 0u));
+                ryr_st_if<NTC>(cjsr1 + g, ryr_sr_f32((double)cjsrv + dotcjsr_d * dt,
+                                                ryr_sr_bits(ryr_sr_key(C.seed, (uint64_t)nstep, RYR_SR_CJSR), (uint32_t)n, 0u)));
+                ryr_st_if<NTC>(cjsrbp + g, ryr_sr_f32(cjsrb_new_d,
+                                                 ryr_sr_bits(ryr_sr_key(C.seed, (uint64_t)nstep, RYR_SR_CJSRB), (uint32_t)n, 0u)));
             } else {
-                cjsr1[g] = cjsrv + dotcjsr * (TS)dtT;
-                cjsrbp[g] = cjsrb_new;
+                ryr_st_if<NTC>(cjsr1 + g, (TS)(cjsrv + dotcjsr * (TS)dtT));
+                ryr_st_if<NTC>(cjsrbp + g, cjsrb_new);
             }
             /* the gating words' bases pass through late_base, so that these
              * stores form their addresses here instead of keeping the loads'
              * 64-bit addresses live across the kernel (which spills in the
              * 168-VGPR instances) */
-            late_base(a.lccw)[g] = lw;
+            ryr_st_if<NTC>(late_base(a.lccw) + g, lw);
             if (!GSPLIT)
-                late_base(a.ryrw)[g] = (uint32_t)nou | ((uint32_t)ncu << 8) | ((uint32_t)nob << 16) | ((uint32_t)ncb << 24);
-            late_base(a.aux)[g] = (uint32_t)(nl & 0xffff) | ((uint32_t)nspark << 16) | (lmask << RYR_AUX_MASK_SHIFT);
+                ryr_st_if<NTC>(late_base(a.ryrw) + g,
+                          (uint32_t)nou | ((uint32_t)ncu << 8) | ((uint32_t)nob << 16) | ((uint32_t)ncb << 24));
+            ryr_st_if<NTC>(late_base(a.aux) + g,
+                      (uint32_t)(nl & 0xffff) | ((uint32_t)nspark << 16) | (lmask << RYR_AUX_MASK_SHIFT));
             if (diag) {
                 a.d_xire[g] = xire;
                 a.d_xinaca[g] = xinaca;
