@@ -176,6 +176,7 @@ def run_gpu(p, cells, steps, warmup, prof_steps, shape=None, floor=False):
         "stored_bytes_per_cru_step": stored_bytes_per_cru_step(p.precision),
         "step_GBs": ncru * UNIT_BYTES[p.precision] / step_s / 1e9,
         "launch_shape": {"k_cyto": shape[0], "k_cru_step": shape[1], "schedule": SCHED_NAMES[shape[2]]},
+        "shape": list(shape),
         "kernels": kernels,
         "dominant": dom,
         "latency_floor_us": floor_us,
@@ -188,7 +189,7 @@ def roofline(r, workload):
     alg = r["ncru"] * r["unit_bytes"]
     ach = alg / (us * 1e-6) / 1e9
     return {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles(workload, k == "k_one"),
+            "traffic": traffic_from_profiles(workload, r["shape"]),
             "kernel": k, "kernel_us": us, "bytes_per_launch": alg, "unit_bytes": r["unit_bytes"],
             "step_GBs": r["step_GBs"], "step_frac": r["step_GBs"] / HBM_PEAK_GBS,
             "stored_bytes_per_cru_step": r["stored_bytes_per_cru_step"],
@@ -243,21 +244,32 @@ def ead_map_from_profiles():
     return d
 
 
-def traffic_from_profiles(workload, one_launch=False):
+def traffic_from_profiles(workload, shape):
     """HBM bytes of one step's per-CRU kernels (the sub-voxel sweep and the
-    CRU kernel, each one launch per step, or the one-launch kernel) on
-    `workload`, from the committed PMC summary (FETCH_SIZE and WRITE_SIZE per
-    dispatch, each scaled by the factor the same counter shows on a copy
-    kernel of known size), or None.  It is the measured counterpart of the
+    CRU kernel, or the one-launch kernel) on `workload` at launch shape
+    `shape` (k_cyto size, k_cru_step size, schedule), from the committed PMC
+    summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by the factor
+    the same counter shows on a copy kernel of known size), or None.  Without
+    an entry for the exact shape, an entry with the same schedule and the same
+    sweep form (one lane per CRU or per sub-voxel) is taken: the bytes do not
+    depend on the workgroup sizes.  It is the measured counterpart of the
     unit's algorithmic bytes, which also cover the whole step."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         t = json.load(f)
-    names = ("k_one",) if one_launch else ("k_cyto", "k_cru_step")
-    parts = [t.get(f"{k}/{workload}", {}).get("hbm_bytes_per_launch") for k in names]
-    return None if None in parts else sum(parts)
+    wc, wr, sc = shape
+    exact = t.get(f"{workload}@{wc},{wr},{sc}")
+    if exact:
+        return exact["hbm_bytes_per_step"]
+    for key, e in t.items():
+        if not key.startswith(workload + "@"):
+            continue
+        kc, kr, ks = (int(x) for x in key.split("@")[1].split(","))
+        if ks == sc and (kc == 1024) == (wc == 1024):
+            return e["hbm_bytes_per_step"]
+    return None
 
 
 def sub_record(r, workload, steps, warmup):
