@@ -119,7 +119,7 @@ def summarize(res: PointResult, ncru: int) -> PointResult:
 # step time of a batched handle (FP32, MI355X): a latency floor per step, or
 # the HBM/VALU-bound rate per CRU once the batch fills the chip
 STEP_FLOOR_S = 30e-6
-STEP_PER_CRU_S = 0.21e-9
+STEP_PER_CRU_S = 0.16e-9          # 256-cell batch: 0.47 ms per step for 3.07 M CRUs
 
 
 def step_time_model(ncru_total: int) -> float:
