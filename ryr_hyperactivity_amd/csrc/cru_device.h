@@ -230,6 +230,10 @@ RYR_DEV double dpp_f64(double x)
     return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
 
+/* x formed by this point: an empty asm statement that reads it keeps the
+ * compiler from sinking its computation past here (it emits nothing) */
+RYR_DEV void ryr_pin(double x) { __asm__ volatile("" : : "v"(x)); }
+
 /* Lane `src` (wave-uniform) of x, broadcast through a scalar register. */
 RYR_DEV double lanev(double x, int src)
 {
@@ -426,17 +430,29 @@ RYR_DEV void vfactors(RyrMem *m, int iso2)
  * first gate whose comparison holds is picked by selects, so every lane of a
  * wave runs one path whatever its channel's state; the values compared are
  * those of the sequential walk. */
+/* the step's parts that need no V: the uniform over dt, and the fCa gate's
+ * closing rate */
+RYR_DEV double lcc_rig(double ragg, double dt)
+{
+    RYR_STRICT_FP
+    return ragg / dt;
+}
+
+RYR_DEV double lcc_betafca(double cpx)
+{
+    RYR_STRICT_FP
+    return 0.175 / (1 + (25 / cpx) * (25 / cpx));
+}
+
 template <typename MV>
-RYR_DEV int lcc_step(int i, double cpx, double dt, double ragg, const MV &m, double betac)
+RYR_DEV int lcc_pick(int i, double rig, double betafca, const MV &m, double betac)
 {
     RYR_STRICT_FP
     const double alphac = 0.22, alphafca = 0.012 / 2.;
-    double betafca = 0.175 / (1 + (25 / cpx) * (25 / cpx));
     const double r1 = (i & 1) ? alphac : betac;
     const double r2 = (i & 2) ? m.alphad : m.betad;
     const double r4 = (i & 4) ? m.alphaf : m.betaf;
     const double r8 = (i & 8) ? alphafca : betafca;
-    double rig = ragg / dt;
     const bool c1 = rig < r1;
     rig -= r1;
     const bool c2 = rig < r2;
@@ -446,6 +462,12 @@ RYR_DEV int lcc_step(int i, double cpx, double dt, double ragg, const MV &m, dou
     const bool c8 = rig < r8;
     const int d = c1 ? 1 : c2 ? 2 : c4 ? 4 : c8 ? 8 : 0;
     return (i & d) ? i - d : i + d;
+}
+
+template <typename MV>
+RYR_DEV int lcc_step(int i, double cpx, double dt, double ragg, const MV &m, double betac)
+{
+    return lcc_pick(i, lcc_rig(ragg, dt), lcc_betafca(cpx), m, betac);
 }
 
 /* Transition t's count once its first uniform u0 (static slot 8 + t) has
@@ -1844,13 +1866,15 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L, double stim_amp = 0.0)
             lpre[0] = ryr_block(C.seed, (uint32_t)n, (uint64_t)nstep, 0u);
             lpre[1] = ryr_block(C.seed, (uint32_t)n, (uint64_t)nstep, 1u);
         }
-        /* the gating-free instances, which have the registers, also issue the
-         * cleft buffer, JSR buffer, scheme-B and vertical-neighbour loads
-         * here rather than at their use */
+        /* the gating-free instances, which have the registers, and the
+         * one-launch ones, whose update after V is the step's critical path,
+         * also issue the cleft buffer, JSR buffer, NCX, scheme-B and
+         * vertical-neighbour loads here rather than at their use */
+        constexpr bool EARLY = GSPLIT || VNEXT;
         const bool has_up = n + sz < C.N, has_dn = n >= sz;
         TF e_sar{}, e_sarh{}, e_sarj{}, e_sarhj{}, e_up{}, e_dn{}, e_sb0{}, e_sb1{}, e_ancx{};
         TS e_cjb{};
-        if constexpr (GSPLIT) {
+        if constexpr (EARLY) {
             e_sar = ryr_ld_if<NTC>(casar + g); e_sarh = ryr_ld_if<NTC>(casarh + g);
             e_sarj = ryr_ld_if<NTC>(casarj + g); e_sarhj = ryr_ld_if<NTC>(casarhj + g);
             e_cjb = ryr_ld_if<NTC>(cjsrbp + g);
@@ -1888,8 +1912,89 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L, double stim_amp = 0.0)
             if constexpr (LPC == 2) nfail += (int)lane_pair_u32((uint32_t)nfail);
             if (prevsp < 5 && (nou + nob) >= 5) nspark = 1;
         };
+        /* spark release (:1286-1293) */
+        double xire;
+        auto release = [&]() {
+            RYR_STRICT_FP
+            double po = (double)(nou + nob) / (double)(RYR_NRYR);
+            double Jmax = (0.0147 * 11.5);
+            xire = 1.0 * Jmax * po * ((double)cjsrv - cp_d) / (0.00126) / 1.0;
+            if (xire < 0) xire = 0;
+        };
+        /* exchange fluxes and JSR (:1309-1318) */
+        TS diffjn0, diffjn1;
+        TS dotcjsr, cjsrb_new;
+        double dotcjsr_d, cjsrb_new_d;
+        double jbeta;                               /* the CSQN buffering factor */
+        auto jsr_beta = [&]() {
+            if constexpr (!vf_early) { cnl[0] = cn0[g]; cnl[1] = cn0[P + g]; }
+            if constexpr (sizeof(TF) == 8) {
+                diffjn0 = (cjsrv - cnl[0]) / TS(25.0 * 1.0 / 4.) / TS(2.);
+                diffjn1 = (cjsrv - cnl[1]) / TS(25.0 * 1.0 / 4.) / TS(2.);
+            } else {
+                diffjn0 = (cjsrv - cnl[0]) * TS(1.0 / (25.0 * 1.0 / 4.) / 2.);
+                diffjn1 = (cjsrv - cnl[1]) * TS(1.0 / (25.0 * 1.0 / 4.) / 2.);
+            }
+            {
+                RYR_STRICT_FP
+                const double cj = (double)cjsrv;
+                double ene = mono * 20 + (1.0 - mono) * 30;
+                double enedot = (20 - 30) * (-mono + 1.0 / (4.0 * 460.0 * mono * roo2 + 1.0)) * (23.0 / cj)
+                                * (1.0 - roo2 / 5000.0);
+                jbeta = 1.0 / (1.0 + (460.0 * 600.0 * ene + 460.0 * enedot * cj * (cj + 600.0))
+                                      / ((600.0 + cj) * (600.0 + cj)));
+            }
+        };
+        auto jsr_flux = [&]() {
+            RYR_STRICT_FP
+            double flux = ((-(double)diffjn0 - (double)diffjn1) - xire * (0.00126) * 1.0 / (0.02 * 1));
+            dotcjsr_d = jbeta * flux;
+            dotcjsr = (TS)dotcjsr_d;
+            cjsrb_new_d = (double)(EARLY ? e_cjb : ryr_ld_if<NTC>(cjsrbp + g)) + (1 - jbeta) * flux * dt;
+            cjsrb_new = (TS)cjsrb_new_d;
+        };
+        /* VNEXT: what needs no V is formed while V is being published.  What
+         * needs no gating result either comes before the gating, so that a
+         * wave whose gating runs long is not delayed further after it: the
+         * LCC draws of channels 0-3 over dt with the fCa closing rate
+         * (lcc_step's V-free parts) and the CSQN buffering factor; then the
+         * release and the JSR update */
+        double rig_pre[LPC == 1 ? 4 : 2] = {};
+        double bfca_pre = 0.0, cpx_pre = 0.0, cpxk_pre = 0.0;
         if constexpr (VNEXT) {
+            if (!perm) {
+                RYR_STRICT_FP
+                cpx_pre = (cp_d + cs_d * (0.5)) / (1 + (0.5));
+                cpxk_pre = cpx_pre / 1000.0;
+                bfca_pre = lcc_betafca(cpx_pre);
+                if constexpr (LPC == 1) {
+#pragma unroll
+                    for (int ll = 0; ll < 4; ++ll) {
+                        const ryr_u32x4 &lb = lpre[ll >> 1];
+                        rig_pre[ll] = lcc_rig((ll & 1) ? ryr_slot_uniform(lb.v[2], lb.v[3])
+                                                       : ryr_slot_uniform(lb.v[0], lb.v[1]), dt);
+                    }
+                } else {
+                    /* this lane's channels hl and hl + 2 draw from blocks 0 and 1 */
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        const ryr_u32x4 lb = ryr_block(C.seed, (uint32_t)n, (uint64_t)nstep, (uint32_t)k);
+                        rig_pre[k] = lcc_rig(hl ? ryr_slot_uniform(lb.v[2], lb.v[3])
+                                                : ryr_slot_uniform(lb.v[0], lb.v[1]), dt);
+                    }
+                }
+            }
+            jsr_beta();
+            /* each value is held where it is formed (ryr_pin), so that the
+             * compiler does not sink it past the gating or the wait to its
+             * use */
+            ryr_pin(jbeta); ryr_pin(bfca_pre); ryr_pin(cpx_pre); ryr_pin(cpxk_pre);
+#pragma unroll
+            for (int k = 0; k < (LPC == 1 ? 4 : 2); ++k) ryr_pin(rig_pre[k]);
             gate();
+            release();
+            jsr_flux();
+            ryr_pin(xire); ryr_pin(dotcjsr_d); ryr_pin(cjsrb_new_d);
             cru_v_next_get(a, C, nstep, stim_amp, &v, &vf0);
         }
 
@@ -1901,7 +2006,7 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L, double stim_amp = 0.0)
             RYR_STRICT_FP
             if (!perm) {
                 const RyrVF vf = vf_early ? vf0 : load_vf(M);
-                const double cpx = (cp_d + cs_d * (0.5)) / (1 + (0.5));
+                const double cpx = VNEXT ? cpx_pre : (cp_d + cs_d * (0.5)) / (1 + (0.5));
                 const double betac = a.iso2 ? 2.0 : 4.0;
                 int nlcp = 0;
                 uint32_t nw = lw;
@@ -1919,7 +2024,8 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L, double stim_amp = 0.0)
                             lb = vf_early ? lpre[ll >> 1]
                                           : ryr_block(C.seed, (uint32_t)n, (uint64_t)nstep, (uint32_t)(ll >> 1));
                         const double ragg = (ll & 1) ? ryr_slot_uniform(lb.v[2], lb.v[3]) : ryr_slot_uniform(lb.v[0], lb.v[1]);
-                        jlo[ll] = lcc_step(st, cpx, dt, ragg, vf, betac);
+                        if constexpr (VNEXT) jlo[ll] = lcc_pick(st, rig_pre[ll], bfca_pre, vf, betac);
+                        else jlo[ll] = lcc_step(st, cpx, dt, ragg, vf, betac);
                     }
 #pragma unroll
                     for (int ll = 0; ll < 4; ++ll)
@@ -1944,9 +2050,16 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L, double stim_amp = 0.0)
                     const int npre = __builtin_ctz(~lmask);
                     for (int ll = hl; ll < npre; ll += 2) {
                         int st = (int)((lw >> (4 * ll)) & 15u);
-                        lb = ryr_block(C.seed, (uint32_t)n, (uint64_t)nstep, (uint32_t)(ll >> 1));
-                        const double ragg = (ll & 1) ? ryr_slot_uniform(lb.v[2], lb.v[3]) : ryr_slot_uniform(lb.v[0], lb.v[1]);
-                        int jj = lcc_step(st, cpx, dt, ragg, vf, betac);
+                        bool pre = false;
+                        if constexpr (VNEXT) pre = ll < 4;
+                        int jj;
+                        if (pre) {
+                            jj = lcc_pick(st, ll < 2 ? rig_pre[0] : rig_pre[1], bfca_pre, vf, betac);
+                        } else {
+                            lb = ryr_block(C.seed, (uint32_t)n, (uint64_t)nstep, (uint32_t)(ll >> 1));
+                            const double ragg = (ll & 1) ? ryr_slot_uniform(lb.v[2], lb.v[3]) : ryr_slot_uniform(lb.v[0], lb.v[1]);
+                            jj = lcc_step(st, cpx, dt, ragg, vf, betac);
+                        }
                         nw = (nw & ~(15u << (4 * ll))) | ((uint32_t)jj << (4 * ll));
                         if (jj == 0) ++nlcp;
                     }
@@ -1960,7 +2073,7 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L, double stim_amp = 0.0)
                 }
                 nl += nlcp;
                 lw = nw;
-                double X_ = cpx / 1000.0 * vf.ghk_e2 - (1 * 1.8);
+                double X_ = (VNEXT ? cpxk_pre : cpx / 1000.0) * vf.ghk_e2 - (1 * 1.8);
                 double ica = vf.ghk_small ? 2.0 * 11.9 * 96.485 * 0.341 * X_ : vf.ghk_A * X_ / vf.ghk_D;
                 if (ica > 0.0) ica = 0.0;
                 icat1 = C.svica * (double)(nlcp) * ica;
@@ -1974,14 +2087,7 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L, double stim_amp = 0.0)
         /* RyR gating (unless done above, or by the gating launch: GSPLIT),
          * spark, release (:1274-1293) */
         if constexpr (!GSPLIT && !VNEXT) gate();
-        double xire;
-        {
-            RYR_STRICT_FP
-            double po = (double)(nou + nob) / (double)(RYR_NRYR);
-            double Jmax = (0.0147 * 11.5);
-            xire = 1.0 * Jmax * po * ((double)cjsrv - cp_d) / (0.00126) / 1.0;
-            if (xire < 0) xire = 0;
-        }
+        if constexpr (!VNEXT) release();
 
         /* NCX (:1294-1304, subroutine_cuda.c:47-75): FP64 in the exact path,
          * float with reciprocals in the production path (the per-CRU current
@@ -1990,11 +2096,11 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L, double stim_amp = 0.0)
         TF ancx_new;
         if (perm) {                                 /* no NCX (:1302) */
             xinaca = 0.0;
-            ancx_new = GSPLIT ? e_ancx : ryr_ld_if<NTC>(ancx + g);
+            ancx_new = EARLY ? e_ancx : ryr_ld_if<NTC>(ancx + g);
         } else if constexpr (sizeof(TF) != 8) {
             const RyrVF vf = vf_early ? vf0 : load_vf(M);
             const float csm = (float)cs_d * 1e-3f;
-            const float An = (float)(GSPLIT ? e_ancx : ryr_ld_if<NTC>(ancx + g));
+            const float An = (float)(EARLY ? e_ancx : ryr_ld_if<NTC>(ancx + g));
             /* slowncxchange (subroutine_cuda.c:59-62): the exchanger at 5 x the allosteric factor */
             const float Ka = (C.variants & RYR_VAR_SLOWNCX) ? An * 5.0f : An;
             const float t2 = 669921.875f * csm * (1.0f + csm * (float)(1.0 / 0.00359));
@@ -2009,7 +2115,7 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L, double stim_amp = 0.0)
             RYR_STRICT_FP
             const RyrVF vf = vf_early ? vf0 : load_vf(M);
             double cs = cs_d / 1000.0;
-            const double An = (double)(GSPLIT ? e_ancx : ryr_ld_if<NTC>(ancx + g));
+            const double An = (double)(EARLY ? e_ancx : ryr_ld_if<NTC>(ancx + g));
             double Ka = An;
             if (C.variants & RYR_VAR_SLOWNCX) Ka = An * 5;  /* subroutine_cuda.c:59-62 */
             double t2 = (87.5) * (87.5) * (87.5) * cs * (1.0 + cs / 0.00359);
@@ -2023,49 +2129,24 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L, double stim_amp = 0.0)
         }
         if (lead) ryr_st_if<NTC>(ancx + g, ancx_new);
 
-        /* exchange fluxes and JSR (:1309-1318) */
+        if constexpr (!VNEXT) { jsr_beta(); jsr_flux(); }
         const TF cpT = cpv, csT = csv;
         const TF dtT = (TF)dt;
-        TS diffjn0, diffjn1;
-        if constexpr (!vf_early) { cnl[0] = cn0[g]; cnl[1] = cn0[P + g]; }
-        if constexpr (sizeof(TF) == 8) {
-            diffjn0 = (cjsrv - cnl[0]) / TS(25.0 * 1.0 / 4.) / TS(2.);
-            diffjn1 = (cjsrv - cnl[1]) / TS(25.0 * 1.0 / 4.) / TS(2.);
-        } else {
-            diffjn0 = (cjsrv - cnl[0]) * TS(1.0 / (25.0 * 1.0 / 4.) / 2.);
-            diffjn1 = (cjsrv - cnl[1]) * TS(1.0 / (25.0 * 1.0 / 4.) / 2.);
-        }
-        TS dotcjsr, cjsrb_new;
-        double dotcjsr_d, cjsrb_new_d;
-        {
-            RYR_STRICT_FP
-            const double cj = (double)cjsrv;
-            double ene = mono * 20 + (1.0 - mono) * 30;
-            double enedot = (20 - 30) * (-mono + 1.0 / (4.0 * 460.0 * mono * roo2 + 1.0)) * (23.0 / cj)
-                            * (1.0 - roo2 / 5000.0);
-            double beta = 1.0 / (1.0 + (460.0 * 600.0 * ene + 460.0 * enedot * cj * (cj + 600.0))
-                                       / ((600.0 + cj) * (600.0 + cj)));
-            double flux = ((-(double)diffjn0 - (double)diffjn1) - xire * (0.00126) * 1.0 / (0.02 * 1));
-            dotcjsr_d = beta * flux;
-            dotcjsr = (TS)dotcjsr_d;
-            cjsrb_new_d = (double)(GSPLIT ? e_cjb : ryr_ld_if<NTC>(cjsrbp + g)) + (1 - beta) * flux * dt;
-            cjsrb_new = (TS)cjsrb_new_d;
-        }
 
         /* cleft/sub-membrane sub-steps (:1356-1473 / :1475-1597); the
          * cytosol/NSR sub-voxels were advanced by k_cyto */
-        TF v_casar = GSPLIT ? e_sar : ryr_ld_if<NTC>(casar + g), v_casarh = GSPLIT ? e_sarh : ryr_ld_if<NTC>(casarh + g);
-        TF v_casarj = GSPLIT ? e_sarj : ryr_ld_if<NTC>(casarj + g), v_casarhj = GSPLIT ? e_sarhj : ryr_ld_if<NTC>(casarhj + g);
+        TF v_casar = EARLY ? e_sar : ryr_ld_if<NTC>(casar + g), v_casarh = EARLY ? e_sarh : ryr_ld_if<NTC>(casarh + g);
+        TF v_casarj = EARLY ? e_sarj : ryr_ld_if<NTC>(casarj + g), v_casarhj = EARLY ? e_sarhj : ryr_ld_if<NTC>(casarhj + g);
         const TF csb = perm ? (TF)C.ci_bath : csv;             /* perm: bath outside the lattice */
-        const TF cs_up = has_up ? (GSPLIT ? e_up : cs0[g + sz]) : csb;   /* k + 1 < Z */
-        const TF cs_dn = has_dn ? (GSPLIT ? e_dn : cs0[g - sz]) : csb;   /* k > 0 */
+        const TF cs_up = has_up ? (EARLY ? e_up : cs0[g + sz]) : csb;   /* k + 1 < Z */
+        const TF cs_dn = has_dn ? (EARLY ? e_dn : cs0[g - sz]) : csb;   /* k > 0 */
         const TF bcgT = (TF)bcgcur;
         const TF icatT = (TF)icat1, xinacaT = (TF)xinaca, xireT = (TF)xire;
         TF csn = csv;
         const TF *sbp = (const TF *)a.sb;
         if constexpr (sizeof(TF) != 8) {
             float dci[2] = {0.0f, 0.0f};
-            if (SCHB) { dci[0] = (float)(GSPLIT ? e_sb0 : sbp[g]); dci[1] = (float)(GSPLIT ? e_sb1 : sbp[P + g]); }
+            if (SCHB) { dci[0] = (float)(EARLY ? e_sb0 : sbp[g]); dci[1] = (float)(EARLY ? e_sb1 : sbp[P + g]); }
             float cpf = (float)cpv, csf = (float)csv;
             float sar = (float)v_casar, sarh = (float)v_casarh, sarj = (float)v_casarj, sarhj = (float)v_casarhj;
             fast_cleft<!SCHB>((float)dt, (float)cil[0], (float)cil[1], (float)csv, (float)cs_up,
@@ -2079,7 +2160,7 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L, double stim_amp = 0.0)
             v_casar = (TF)sar; v_casarh = (TF)sarh; v_casarj = (TF)sarj; v_casarhj = (TF)sarhj;
         } else {
         TF dotci01[2] = {TF(0), TF(0)};
-        if (SCHB) { dotci01[0] = GSPLIT ? e_sb0 : sbp[g]; dotci01[1] = GSPLIT ? e_sb1 : sbp[P + g]; }
+        if (SCHB) { dotci01[0] = EARLY ? e_sb0 : sbp[g]; dotci01[1] = EARLY ? e_sb1 : sbp[P + g]; }
         const TF dth = dtT / TF(5);
         if (!SCHB) {
             /* scheme A (:1396-1464) */
