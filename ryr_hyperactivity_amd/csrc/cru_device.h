@@ -2039,7 +2039,11 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L, double stim_amp = 0.0)
                             int st = (int)((lw >> (4 * ll)) & 15u);
                             if (!(ll & 1)) lb = ryr_block(C.seed, (uint32_t)n, (uint64_t)nstep, (uint32_t)(ll >> 1));
                             const double ragg = (ll & 1) ? ryr_slot_uniform(lb.v[2], lb.v[3]) : ryr_slot_uniform(lb.v[0], lb.v[1]);
-                            int jj = lcc_step(st, cpx, dt, ragg, vf, betac);
+                            /* VNEXT: the fCa rate formed before the wait, which the
+                             * compiler would otherwise hoist here, onto the
+                             * critical path, for channels that are rarely present */
+                            int jj = VNEXT ? lcc_pick(st, lcc_rig(ragg, dt), bfca_pre, vf, betac)
+                                           : lcc_step(st, cpx, dt, ragg, vf, betac);
                             nw = (nw & ~(15u << (4 * ll))) | ((uint32_t)jj << (4 * ll));
                             if (jj == 0) ++nlcp;
                         }
@@ -2058,7 +2062,8 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L, double stim_amp = 0.0)
                         } else {
                             lb = ryr_block(C.seed, (uint32_t)n, (uint64_t)nstep, (uint32_t)(ll >> 1));
                             const double ragg = (ll & 1) ? ryr_slot_uniform(lb.v[2], lb.v[3]) : ryr_slot_uniform(lb.v[0], lb.v[1]);
-                            jj = lcc_step(st, cpx, dt, ragg, vf, betac);
+                            jj = VNEXT ? lcc_pick(st, lcc_rig(ragg, dt), bfca_pre, vf, betac)
+                                       : lcc_step(st, cpx, dt, ragg, vf, betac);
                         }
                         nw = (nw & ~(15u << (4 * ll))) | ((uint32_t)jj << (4 * ll));
                         if (jj == 0) ++nlcp;
