@@ -46,6 +46,16 @@
 #else
 #define RYR_EXPR_FP
 #endif
+/* RYR_EXACT_FP: no contraction in any unit.  The membrane step is FP64 in
+ * both precisions and is formed in several places (the membrane workgroup,
+ * the one-launch kernel's pending step and its per-wave V); without fusion
+ * each of them rounds the same operations in the same order, so every
+ * schedule gets the same bits in the FP32 unit too. */
+#if defined(__clang__)
+#define RYR_EXACT_FP _Pragma("clang fp contract(off)")
+#else
+#define RYR_EXACT_FP
+#endif
 
 RYR_DM double ryr_dm_from_bits(uint64_t u) { union { uint64_t u; double d; } c; c.u = u; return c.d; }
 RYR_DM uint64_t ryr_dm_bits(double d) { union { uint64_t u; double d; } c; c.d = d; return c.u; }

@@ -389,7 +389,7 @@ template <typename T> RYR_DEV T leak_t(T cnsr, T ci)
 /* -------------------------------------------- hoisted V-only factors (FP64) */
 RYR_DEV void vfactors(RyrMem *m, int iso2)
 {
-    RYR_STRICT_FP
+    RYR_EXACT_FP
     const double v = m->m[0], xnai = m->m[12];
     /* reesica.h:21-59 */
     double dv5 = 5, dvk = 8, fv5 = -22.8, fvk = 9.1;
@@ -793,7 +793,7 @@ struct MemIn {
 /* I_Na (:460-502): m, h, j */
 RYR_DEV void mem_ina(const MemIn &I, int lane, double *sh)
 {
-    RYR_STRICT_FP
+    RYR_EXACT_FP
     const double v = I.s[0], xm = I.s[1], xh = I.s[2], xj = I.s[3], xnai = I.s[12], dt = I.dt;
     const double frt = RYR_FRT;
     const bool lo = v < -40.0;
@@ -883,7 +883,7 @@ RYR_DEV void mem_ina(const MemIn &I, int lane, double *sh)
  * sub-membrane mean, in mem_sums) */
 RYR_DEV void mem_ikr_iks(const MemIn &I, int lane, double *sh)
 {
-    RYR_STRICT_FP
+    RYR_EXACT_FP
     const double v = I.s[0], xr = I.s[4], xs1 = I.s[5], xs2 = I.s[6], qks = I.s[7], xnai = I.s[12];
     const double dt = I.dt;
     const double xki = RYR_XKI, xko = RYR_XKO;
@@ -967,7 +967,7 @@ RYR_DEV void mem_ikr_iks(const MemIn &I, int lane, double *sh)
 /* I_K1 (:590-600), I_to,s and I_to,f (:604-659) */
 RYR_DEV void mem_ik1_ito(const MemIn &I, int lane, double *sh)
 {
-    RYR_STRICT_FP
+    RYR_EXACT_FP
     const double v = I.s[0], xtos = I.s[8], ytos = I.s[9], xtof = I.s[10], ytof = I.s[11], dt = I.dt;
     const double xki = RYR_XKI, xko = RYR_XKO;
     const double frt = RYR_FRT;
@@ -1055,7 +1055,7 @@ RYR_DEV void mem_ik1_ito(const MemIn &I, int lane, double *sh)
 /* I_NaK (:665-676) and the whole-cell Ca currents from the CRU sums */
 RYR_DEV void mem_inak(const MemIn &I, int lane, double *sh)
 {
-    RYR_STRICT_FP
+    RYR_EXACT_FP
     const double v = I.s[0], xnai = I.s[12];
     const double xko = RYR_XKO;
     const double frt = RYR_FRT;
@@ -1087,7 +1087,7 @@ RYR_DEV void mem_inak(const MemIn &I, int lane, double *sh)
  * whole-cell Ca currents I_CaL and I_NCX (:447-456) */
 RYR_DEV void mem_sums(const MemIn &I, int lane, double *sh)
 {
-    RYR_STRICT_FP
+    RYR_EXACT_FP
     const double qks = I.s[7], dt = I.dt;
     double n = 1.0, d = 1.0;
     switch (lane) {
@@ -1116,7 +1116,7 @@ RYR_DEV void mem_sums(const MemIn &I, int lane, double *sh)
 RYR_DEV void mem_finish(const double *s, const double *sh, double dt, double stim, double *v_out,
                         double *nai_out)
 {
-    RYR_STRICT_FP
+    RYR_EXACT_FP
     const double xina = sh[RYR_MS_INA], xik1 = sh[RYR_MS_IK1], xikr = sh[RYR_MS_IKR], xiks = sh[RYR_MS_IKS];
     const double xitos = sh[RYR_MS_ITOS], xitof = sh[RYR_MS_ITOF], xinacato = sh[RYR_MS_INACA];
     const double xicatto = sh[RYR_MS_ICAT], xinak = sh[RYR_MS_INAK];
@@ -1139,7 +1139,7 @@ RYR_DEV void mem_finish(const double *s, const double *sh, double dt, double sti
  * divisions spread over the lanes; every lane gets them. */
 RYR_DEV RyrVF vfactors_regs(int iso2, int lane, const double v, const double xnai)
 {
-    RYR_STRICT_FP
+    RYR_EXACT_FP
     double dv5 = 5, dvk = 8, fv5 = -22.8, fvk = 9.1;
     if (iso2) { dv5 = 0; fv5 = -28; fvk = 8.5; }
     const double q = 0.0337 * (v + 10.5);
@@ -1259,7 +1259,7 @@ RYR_DEV void beat_close(RyrMem *M, double *beats, double complete)
 RYR_DEV void beat_track(RyrMem *M, const RyrCellInfo &C, long long n, double dt, double vo, double vn,
                         double *beats)
 {
-    RYR_STRICT_FP
+    RYR_EXACT_FP
     const bool onset = C.ncl > 0 && n >= C.noff && ((n - C.noff) % C.ncl) == 0 &&
                        (C.nstim_end < 0 || n < C.nstim_end);
     if (onset) {
@@ -2675,7 +2675,7 @@ RYR_DEV void membrane_next_block(const RyrKArgs &a, int c, long long nstep, doub
 RYR_DEV void cru_v_next_wave(const RyrKArgs &a, const RyrCellInfo &C, long long nstep, double stim_amp,
                              double *v_out, RyrVF *vf_out)
 {
-    RYR_STRICT_FP
+    RYR_EXACT_FP
     const long long np = nstep - 1;
     const RyrPend *pd = a.pend + 2LL * C.cell + (np & 1);
     const int lane = threadIdx.x & 63;

@@ -185,10 +185,11 @@ def test_steered_map_tops_up_uncertain_training_points(monkeypatch):
 def test_steered_map_spends_less_than_the_plain_map():
     """The learned part against the method without it, on a small grid: the
     plain method simulates all 18 points; the steered map simulates the 10
-    training points and the points the surrogate is unsure of, and takes the
-    prediction elsewhere.  Both use the same seeds, so a simulated point has
-    the same value either way and the plain results serve as the cache.
-    16 replicas x 4 scored beats put the plain estimate's own noise near 0.06."""
+    training points and the points the surrogate is unsure of, in batches of
+    its own, and takes the prediction elsewhere.  Both use the same seeds, so
+    every replica the steered map simulates has the plain run's beat rows
+    exactly (a cell's trajectory does not depend on its batch).  16 replicas x
+    4 scored beats put the plain estimate's own noise near 0.06."""
     from ryr_hyperactivity_amd.sweep import Cost, simulate_points, steered_map
     print("case steered map 16x6x4", flush=True)
     lat, R, NB = (16, 6, 4), 16, 5
@@ -198,12 +199,16 @@ def test_steered_map_spends_less_than_the_plain_map():
     train = grid_points([1.0, 4.0], ikr[::2], [1000.0])
     calls = []
 
-    def cached(pts, r0, r1):
+    def resimulate(pts, r0, r1):
         calls.append(len(pts))
-        return ({p: summarize(PointResult(p, plain[p].beats[r0:r1]), 16 * 6 * 4) for p in pts},
-                Cost(cell_steps=sum(point_cost(p, r1 - r0, NB) for p in pts)))
+        res, cost = simulate_points(pts, r1, NB, lat, r0=r0, progress=lambda s: None)
+        for p in pts:
+            assert len(res[p].beats) == r1 - r0
+            for a, b in zip(res[p].beats, plain[p].beats[r0:r1]):
+                assert np.array_equal(a, b), p
+        return res, cost
 
-    sm = steered_map(grid, train, R, NB, lat, simulate=cached, train_steps=2000)
+    sm = steered_map(grid, train, R, NB, lat, simulate=resimulate, train_steps=2000)
     srcs = [sm.source[p] for p in grid]
     n_sur = srcs.count("surrogate")
     ratio = sm.cost.cell_steps / pcost.cell_steps
