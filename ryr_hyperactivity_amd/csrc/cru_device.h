@@ -1483,14 +1483,18 @@ RYR_DEV void cyto_point(const RyrKArgs &a, const RyrCellInfo &C, long long nstep
 /* The sub-voxel sweep for logical workgroup L of a launch: one lane per
  * sub-voxel of plane ii of one part of one 256-CRU chunk. */
 template <typename TF, typename TS, int WG>
-RYR_DEV void cyto_block(const RyrKArgs &a, int L, long long nstep)
+RYR_DEV void cyto_block(const RyrKArgs &a, int L, const RyrCellInfo *recs, const long long *gbase, int kc)
 {
     constexpr int SUB = RYR_BLOCK / WG;
     const int b = L / (8 * SUB);
     const int r = L - b * (8 * SUB);
     const int ii = r & 7;
     const int part = r >> 3;
-    const RyrCellInfo C = a.chunks[b];
+    /* the chunk record (recs: a.chunks) and the step base (gbase: a.gstep),
+     * passed by the caller, which may hold them in registers; both loads
+     * are issued before either is used */
+    const RyrCellInfo C = recs[b];
+    const long long nstep = *gbase + kc;
     const int n = (b - C.first_block) * RYR_BLOCK + part * WG + (int)threadIdx.x;
     if (n >= C.N) return;
     const int par = (int)(nstep & 1);
@@ -1664,7 +1668,7 @@ void k_cyto(RyrKArgs a, int kc, int km, double stim_amp)
         }
         L -= ngate;
     }
-    cyto_block<TF, TS, WG>(a, L, base + kc);
+    cyto_block<TF, TS, WG>(a, L, a.chunks, a.gstep, kc);
 }
 
 /* Cleft/sub-membrane sub-steps of the FP32 production path (scheme A
@@ -1777,9 +1781,12 @@ RYR_DEV void gating_block(const RyrKArgs &a, int L, long long nstep)
 }
 
 /* ------------------------------------------------------------ step kernel */
-/* The CRU update for logical workgroup L of a launch (step gstep + kc). */
+/* The CRU update for logical workgroup L of a launch (step *gbase + kc;
+ * recs and gbase are a.chunks and a.gstep, passed by the caller, which may
+ * hold them in registers). */
 template <typename TF, typename TS, int WG, bool SCHB, int LPC, bool GSPLIT, bool VNEXT = false>
-RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L, double stim_amp = 0.0)
+RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L, double stim_amp, const RyrCellInfo *recs,
+                       const long long *gbase)
 {
     static_assert(LPC == 1 || (LPC == 2 && WG >= 128), "paired lanes need two waves per workgroup");
 
@@ -1788,10 +1795,10 @@ RYR_DEV void cru_block(const RyrKArgs &a, int kc, int L, double stim_amp = 0.0)
     constexpr int SUB = RYR_BLOCK / CPW;
     const int bid = L / SUB;               /* 256-CRU chunk */
     const int part = L - bid * SUB;
-    const RyrCellInfo C = a.chunks[bid];
+    const RyrCellInfo C = recs[bid];
     const int c = C.cell;
     const RyrMem *M = a.mem + c;
-    const long long nstep = *a.gstep + kc;
+    const long long nstep = *gbase + kc;
     const int par = (int)(nstep & 1);
     const int spo = a.spo;
     const bool reset = (nstep == 0) || ((nstep - 1) % spo == 0);
@@ -2331,7 +2338,7 @@ template <typename TF, typename TS, int WG, bool SCHB, int LPC = 1, bool GSPLIT 
 __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu((WG >= 256 || SCHB) ? 1 : 3)))
 void k_cru_step(RyrKArgs a, int kc)
 {
-    cru_block<TF, TS, WG, SCHB, LPC, GSPLIT>(a, kc, xcd_remap((int)blockIdx.x, (int)gridDim.x));
+    cru_block<TF, TS, WG, SCHB, LPC, GSPLIT>(a, kc, xcd_remap((int)blockIdx.x, (int)gridDim.x), 0.0, a.chunks, a.gstep);
 }
 
 /* ------------------------------------------------------------ membrane kernel */
@@ -2581,9 +2588,10 @@ RYR_DEV void pend_groups(const RyrCellInfo &C, const double *s, double dt, RyrPe
 /* Membrane workgroup of the first launch of a run (step nstep): the current
  * groups of the complete state, pending for the next launch. */
 template <int NW>
-RYR_DEV void membrane_first_block(const RyrKArgs &a, int c, long long nstep)
+RYR_DEV void membrane_first_block(const RyrKArgs &a, int c, const long long *gbase, int kc, const RyrCellInfo *cells)
 {
-    const RyrCellInfo C = a.cells[c];
+    const RyrCellInfo C = cells[c];
+    const long long nstep = *gbase + kc;
     __shared__ double s_st[13];
     if (threadIdx.x < 13) s_st[threadIdx.x] = a.mem[c].m[threadIdx.x];
     /* no V is published within this launch, and a value left by an earlier
@@ -2601,9 +2609,11 @@ RYR_DEV void membrane_first_block(const RyrKArgs &a, int c, long long nstep)
  * step's currents, tracks the beat (as membrane_block), then forms the
  * current groups of the new state, pending for the next launch. */
 template <int NW>
-RYR_DEV void membrane_next_block(const RyrKArgs &a, int c, long long nstep, double stim_amp)
+RYR_DEV void membrane_next_block(const RyrKArgs &a, int c, const long long *gbase, int kc, double stim_amp,
+                                 const RyrCellInfo *cells)
 {
-    const RyrCellInfo C = a.cells[c];
+    const RyrCellInfo C = cells[c];
+    const long long nstep = *gbase + kc;
     RyrMem *M = a.mem + c;
     const long long np = nstep - 1;
     const RyrPend *pd = a.pend + 2LL * c + (np & 1);
@@ -2799,24 +2809,32 @@ __global__ __launch_bounds__(RYR_BLOCK) void k_membrane(RyrKArgs a, int km, doub
  * a chunk's eight planes share one L2. */
 __host__ __device__ inline int one_head(int ncells, int nchunks) { return (ncells + nchunks + 7) & ~7; }
 
+/* The step base, the record tables, the step offset and the grid's counts
+ * (a.gstep, a.chunks, a.cells, kc, a.ncells, a.P / 256) lead the argument
+ * list as plain values, so that they arrive in scalar registers with the
+ * wave (kernel-argument preload) rather than by a load from the argument
+ * segment: each workgroup's first dependent loads, the step base and its
+ * record, then issue one memory round trip earlier. */
 template <typename TF, typename TS, bool NEXT, int LPC>
-__global__ __launch_bounds__(RYR_BLOCK) void k_one(RyrKArgs a, int kc, double stim_amp)
+__global__ __launch_bounds__(RYR_BLOCK) void k_one(const long long *gstep, const RyrCellInfo *chunks,
+                                                   const RyrCellInfo *cells, int kc, int ncells, int nchunks,
+                                                   RyrKArgs a, double stim_amp)
 {
-    const long long nstep = *a.gstep + kc;
-    const int nchunks = (int)(a.P >> 8);
+    /* each part reads the step base beside its own record, so that the two
+     * loads are in flight together */
     const int b = (int)blockIdx.x;
-    const int head = one_head(a.ncells, LPC * nchunks);
-    if (b < a.ncells) {
-        if constexpr (NEXT) membrane_next_block<RYR_BLOCK / 64>(a, b, nstep, stim_amp);
-        else membrane_first_block<RYR_BLOCK / 64>(a, b, nstep);
+    const int head = one_head(ncells, LPC * nchunks);
+    if (b < ncells) {
+        if constexpr (NEXT) membrane_next_block<RYR_BLOCK / 64>(a, b, gstep, kc, stim_amp, cells);
+        else membrane_first_block<RYR_BLOCK / 64>(a, b, gstep, kc, cells);
         return;
     }
-    if (b < a.ncells + LPC * nchunks) {
-        cru_block<TF, TS, RYR_BLOCK, false, LPC, false, NEXT>(a, kc, b - a.ncells, stim_amp);
+    if (b < ncells + LPC * nchunks) {
+        cru_block<TF, TS, RYR_BLOCK, false, LPC, false, NEXT>(a, kc, b - ncells, stim_amp, chunks, gstep);
         return;
     }
     if (b < head) return;                          /* padding */
-    cyto_block<TF, TS, RYR_BLOCK>(a, xcd_remap(b - head, (int)gridDim.x - head), nstep);
+    cyto_block<TF, TS, RYR_BLOCK>(a, xcd_remap(b - head, (int)gridDim.x - head), chunks, gstep, kc);
 }
 
 /* ------------------------------------------------------- output-row kernel */

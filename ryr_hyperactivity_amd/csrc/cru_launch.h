@@ -144,12 +144,18 @@ static hipError_t ryr_go_cru(const RyrKArgs &a, int nchunks, int wg, int kc, boo
          * multiple of eight), then eight sweep workgroups per chunk */                         \
         if (lpc != 1 && lpc != 2) return hipErrorInvalidValue;                                   \
         const dim3 grid((unsigned)one_head(a.ncells, lpc * nchunks) + 8u * (unsigned)nchunks);   \
+        const long long *gs = a.gstep;                                                           \
+        const int nck = (int)(a.P >> 8);                                                         \
         if (lpc == 2) {                                                                          \
-            if (next) RYR_GO((k_one<TF, TS, true, 2>), grid, dim3(RYR_BLOCK), s, ev, a, kc, amp); \
-            else RYR_GO((k_one<TF, TS, false, 2>), grid, dim3(RYR_BLOCK), s, ev, a, kc, amp);    \
+            if (next) RYR_GO((k_one<TF, TS, true, 2>), grid, dim3(RYR_BLOCK), s, ev, gs, a.chunks, \
+                             a.cells, kc, a.ncells, nck, a, amp);                                \
+            else RYR_GO((k_one<TF, TS, false, 2>), grid, dim3(RYR_BLOCK), s, ev, gs, a.chunks,    \
+                        a.cells, kc, a.ncells, nck, a, amp);                                     \
         } else {                                                                                 \
-            if (next) RYR_GO((k_one<TF, TS, true, 1>), grid, dim3(RYR_BLOCK), s, ev, a, kc, amp); \
-            else RYR_GO((k_one<TF, TS, false, 1>), grid, dim3(RYR_BLOCK), s, ev, a, kc, amp);    \
+            if (next) RYR_GO((k_one<TF, TS, true, 1>), grid, dim3(RYR_BLOCK), s, ev, gs, a.chunks, \
+                             a.cells, kc, a.ncells, nck, a, amp);                                \
+            else RYR_GO((k_one<TF, TS, false, 1>), grid, dim3(RYR_BLOCK), s, ev, gs, a.chunks,    \
+                        a.cells, kc, a.ncells, nck, a, amp);                                     \
         }                                                                                        \
         return hipGetLastError();                                                                \
     }                                                                                            \
