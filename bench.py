@@ -89,16 +89,10 @@ def make_cell(prec):
 
 
 def make_batch(prec, ncells=256, seed=2024):
-    from ryr_hyperactivity_amd import CellDesc, Params
-    rng = np.random.default_rng(seed)
-    dims = [(50, 10, 10), (75, 15, 10), (100, 20, 10)]
-    cells = []
-    for c in range(ncells):
-        nx, ny, nz = dims[rng.integers(0, 3)]
-        cells.append(CellDesc.default(nx=nx, ny=ny, nz=nz, Ku2=float(rng.uniform(1, 3)),
-                                      ikr_scale=float(rng.uniform(0, 1)),
-                                      cl=float([1000.0, 2000.0, 3000.0][rng.integers(0, 3)]), seed=c))
-    return Params.default(precision=prec, lqt=0), cells    # lqt off: the I_Kr scale applies
+    """The uneven ensemble's cells (ryr_hyperactivity_amd.ensemble.uneven_cells)."""
+    from ryr_hyperactivity_amd import Params
+    from ryr_hyperactivity_amd.ensemble import uneven_cells
+    return Params.default(precision=prec, lqt=0), uneven_cells(ncells, seed)    # lqt off: the I_Kr scale applies
 
 
 def window_start(steps, spo):
