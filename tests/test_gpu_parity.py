@@ -222,6 +222,33 @@ def test_batch_equals_cells_run_alone(shape):
                     assert np.array_equal(one.field(0, f), batch.field(c, f))
 
 
+@pytest.mark.parametrize("shape", [None, (256, 256, 4), (256, 512, 4), (1024, 64, 2), (128, 64, 2)],
+                         ids=["tuned", "one-launch", "one-launch-2", "lane-per-cru-sweep", "concurrent"])
+def test_fp32_batch_equals_cells_run_alone(shape):
+    """FP32 production mode: a cell's trajectory does not depend on the other
+    cells of its batch or on the launch shape (the stochastic rounding's bits
+    are keyed by cell seed, step and the cell's own CRU index; the membrane
+    step is contraction-free), through the upstroke and 100 ms of plateau,
+    per-cell I_Kr scale applied (lqt off); every field, the scalars and the
+    beat rows."""
+    p = Params.default(stim_offset=2.0, precision=PREC_F32, lqt=0)
+    cells = [CellDesc.default(nx=9, ny=4, nz=3, cl=1000.0, seed=1, Ku2=1.0),
+             CellDesc.default(nx=30, ny=10, nz=2, cl=2000.0, seed=2, Ku2=2.5, ikr_scale=0.3),
+             CellDesc.default(nx=5, ny=5, nz=5, cl=1000.0, seed=3, Ku=3.333, Kb=18.0, ikr_scale=0.0)]
+    print(f"case fp32 batch shape={shape}", flush=True)
+    with CellEngine(p, cells) as batch:
+        if shape:
+            batch.set_launch_shape(*shape)
+        batch.step(4000)
+        for c, d in enumerate(cells):
+            with CellEngine(p, d) as one:
+                one.step(4000)
+                assert np.array_equal(one.scalars()[0], batch.scalars()[c]), f"cell {c}: scalars"
+                for f in FIELDS:
+                    assert np.array_equal(one.field(0, f), batch.field(c, f)), f"cell {c}: {f}"
+                assert np.array_equal(one.beats(0), batch.beats(c)), f"cell {c}: beats"
+
+
 def test_full_size_scheme_a_conserves_calcium_up_to_sarcolemmal_flux():
     """The GPU path at the full 100x20x10 size, checked by a property that
     holds at any size: under scheme A the change of Total_Ca over a step
